@@ -1,0 +1,259 @@
+"""MI355X (gfx950) element-stiffness and global-assembly path of the Rust crate
+`finite_element_method`, behind the crate's model API.
+
+    from finite_element_method_amd import FEM
+    m = FEM.create(1e-4, 1e-12, 2)          # FEM::<f64>::create  (src/fem/fem.rs:34)
+    m.add_node(1, 0.0, 0.0, 0.0)            # add_node            (methods_for_node_data_handle.rs:66)
+    m.add_node(2, 30.0, 0.0, 0.0)
+    m.add_truss(1, 1, 2, 1e6, 2.0, None)    # add_truss           (methods_for_truss_data_handle.rs:49)
+    row_ptr, col_idx, val = m.stiffness_csr()   # the global K, assembled on the GPU
+
+Errors the crate returns as `Err(String)` raise ``FemError`` with the crate's
+message.  Everything here is a thin ctypes layer over ``libfem_amd.so`` (C ABI
+in include/fem_amd.h); there is no CPU fallback: without the library or a GPU
+the assembly raises.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["FEM", "FemError", "DeviceError", "TRUSS", "BEAM", "PLATE", "lib"]
+
+TRUSS, BEAM, PLATE = 0, 1, 2
+FEM_OK, FEM_ERR_VALIDATION, FEM_ERR_STATE, FEM_ERR_DEVICE, FEM_ERR_ARGUMENT = 0, 1, 2, 3, 4
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfem_amd.so")
+_lib = None
+
+
+class FemError(Exception):
+    """A validation error: the crate would return Err(String) with this message."""
+
+
+class DeviceError(RuntimeError):
+    """The device path failed (no GPU, HIP error, or library missing)."""
+
+
+def lib():
+    """Loads libfem_amd.so (built by finite_element_method_amd.build)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise DeviceError("libfem_amd.so is not built: run `python -m finite_element_method_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, sz, i64, u32, dbl, ci = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int64, ctypes.c_uint32, ctypes.c_double, ctypes.c_int
+    psz = ctypes.POINTER(ctypes.c_size_t)
+    sigs = {
+        "fem_create": (ci, [dbl, dbl, u32, ctypes.POINTER(vp)]),
+        "fem_reset": (ci, [vp, u32]),
+        "fem_destroy": (None, [vp]),
+        "fem_last_error": (ctypes.c_char_p, [vp]),
+        "fem_add_nodes": (ci, [vp, sz, vp, vp, psz]),
+        "fem_add_trusses": (ci, [vp, sz, vp, vp, vp, vp, vp, psz]),
+        "fem_add_beams": (ci, [vp, sz, vp, vp, vp, psz]),
+        "fem_add_plates": (ci, [vp, sz, vp, vp, vp, psz]),
+        "fem_get_rotation": (ci, [vp, ci, u32, vp]),
+        "fem_assemble": (ci, [vp, vp]),
+        "fem_assemble_into": (ci, [vp, vp, vp]),
+        "fem_build_pattern": (ci, [vp]),
+        "fem_csr_info": (ci, [vp, ctypes.POINTER(i64), ctypes.POINTER(i64)]),
+        "fem_csr_device": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "fem_csr_to_host": (ci, [vp, vp, vp, vp]),
+        "fem_last_assembly_ms": (dbl, [vp]),
+        "fem_assembly_path": (ctypes.c_char_p, [vp]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _stream_handle(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream)
+    return ctypes.c_void_p(stream.cuda_stream)  # torch.cuda.Stream
+
+
+class FEM:
+    """The crate's ``FEM<f64>`` (src/fem/fem.rs:15-65) with GPU assembly."""
+
+    def __init__(self, rel_tol, abs_tol, nodes_number):
+        h = ctypes.c_void_p()
+        rc = lib().fem_create(float(rel_tol), float(abs_tol), int(nodes_number), ctypes.byref(h))
+        if rc != FEM_OK:
+            raise DeviceError("fem_create failed")
+        self._h = h
+        self.rel_tol, self.abs_tol, self.nodes_number = float(rel_tol), float(abs_tol), int(nodes_number)
+
+    @classmethod
+    def create(cls, rel_tol, abs_tol, nodes_number):
+        return cls(rel_tol, abs_tol, nodes_number)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.fem_destroy(h)
+            self._h = None
+
+    def _raise(self, rc):
+        msg = lib().fem_last_error(self._h).decode()
+        if rc == FEM_ERR_VALIDATION:
+            raise FemError(msg)
+        raise DeviceError(msg)
+
+    def _call(self, rc):
+        if rc != FEM_OK:
+            self._raise(rc)
+
+    def reset(self, nodes_number):
+        """FEM::reset (src/fem/fem.rs:155-169)."""
+        self._call(lib().fem_reset(self._h, int(nodes_number)))
+        self.nodes_number = int(nodes_number)
+
+    # ----------------------------------------------------------- batched (SoA) adds
+    def _batched(self, fn, n, *arrays):
+        n_ok = ctypes.c_size_t(0)
+        rc = fn(self._h, n, *[_ptr(a) for a in arrays], ctypes.byref(n_ok))
+        if rc != FEM_OK:
+            self._raise(rc)
+        return n_ok.value
+
+    def add_nodes(self, numbers, xyz):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        if len(numbers) != len(xyz):
+            raise ValueError("numbers and xyz differ in length")
+        return self._batched(lib().fem_add_nodes, len(numbers), numbers, xyz)
+
+    def add_trusses(self, numbers, conn, E, A, A2=None):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        n = len(numbers)
+        conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(n, 2)
+        E = np.ascontiguousarray(np.broadcast_to(np.asarray(E, dtype=np.float64), (n,)))
+        A = np.ascontiguousarray(np.broadcast_to(np.asarray(A, dtype=np.float64), (n,)))
+        A2 = None if A2 is None else np.ascontiguousarray(np.broadcast_to(np.asarray(A2, dtype=np.float64), (n,)))
+        return self._batched(lib().fem_add_trusses, n, numbers, conn, E, A, A2)
+
+    def add_beams(self, numbers, conn, props):
+        """props: (n, 11) = E, nu, A, I11, I22, I12, It, shear_factor, v1x, v1y, v1z."""
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        n = len(numbers)
+        conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(n, 2)
+        props = np.ascontiguousarray(props, dtype=np.float64).reshape(n, 11)
+        return self._batched(lib().fem_add_beams, n, numbers, conn, props)
+
+    def add_plates(self, numbers, conn, props):
+        """props: (n, 4) = E, nu, thickness, shear_factor."""
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        n = len(numbers)
+        conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(n, 4)
+        props = np.ascontiguousarray(props, dtype=np.float64).reshape(n, 4)
+        return self._batched(lib().fem_add_plates, n, numbers, conn, props)
+
+    # ----------------------------------------------------------- the crate's API
+    def add_node(self, number, x, y, z):
+        """FEM::add_node (methods_for_node_data_handle.rs:66-79)."""
+        self.add_nodes([number], [[x, y, z]])
+
+    def add_truss(self, number, node_1_number, node_2_number, young_modulus, area, optional_area_2=None):
+        """FEM::add_truss (methods_for_truss_data_handle.rs:49-128)."""
+        self.add_trusses([number], [[node_1_number, node_2_number]], [young_modulus], [area],
+                         None if optional_area_2 is None else [optional_area_2])
+
+    def add_beam(self, number, node_1_number, node_2_number, young_modulus, poisson_ratio, area, i11, i22, i12, it,
+                 shear_factor, local_axis_1_direction):
+        """FEM::add_beam (methods_for_beam_data_handle.rs:49-142)."""
+        self.add_beams([number], [[node_1_number, node_2_number]],
+                       [[young_modulus, poisson_ratio, area, i11, i22, i12, it, shear_factor,
+                         *local_axis_1_direction]])
+
+    def add_plate(self, number, node_1_number, node_2_number, node_3_number, node_4_number, young_modulus,
+                  poisson_ratio, thickness, shear_factor):
+        """FEM::add_plate (methods_for_plate_data_handle.rs:62-217)."""
+        self.add_plates([number], [[node_1_number, node_2_number, node_3_number, node_4_number]],
+                        [[young_modulus, poisson_ratio, thickness, shear_factor]])
+
+    def _rotation(self, etype, number):
+        out = np.zeros(9)
+        self._call(lib().fem_get_rotation(self._h, etype, int(number), _ptr(out)))
+        return out
+
+    def get_truss_rotation_matrix_elements(self, number):
+        """src/fem/fem.rs:171-180."""
+        return self._rotation(TRUSS, number)
+
+    def get_beam_rotation_matrix_elements(self, number):
+        """src/fem/fem.rs:182-191."""
+        return self._rotation(BEAM, number)
+
+    def get_plate_rotation_matrix_elements(self, number):
+        """src/fem/fem.rs:193-202."""
+        return self._rotation(PLATE, number)
+
+    # ----------------------------------------------------------- assembly
+    def build_pattern(self):
+        self._call(lib().fem_build_pattern(self._h))
+
+    def assemble(self, stream=None):
+        """Assembles the global stiffness on the GPU (asynchronous on `stream`)."""
+        self._call(lib().fem_assemble(self._h, _stream_handle(stream)))
+
+    def assemble_into(self, val, stream=None):
+        """Assembles into a caller-owned float64 CUDA tensor of nnz values."""
+        self._call(lib().fem_assemble_into(self._h, ctypes.c_void_p(val.data_ptr()), _stream_handle(stream)))
+
+    def csr_info(self):
+        n_rows, nnz = ctypes.c_int64(), ctypes.c_int64()
+        self._call(lib().fem_csr_info(self._h, ctypes.byref(n_rows), ctypes.byref(nnz)))
+        return n_rows.value, nnz.value
+
+    def stiffness_csr(self, assemble=True):
+        """Global stiffness as host CSR arrays (row_ptr int64, col_idx int32, val float64)."""
+        if assemble:
+            self.assemble()
+        n_rows, nnz = self.csr_info()
+        rp = np.zeros(n_rows + 1, dtype=np.int64)
+        ci = np.zeros(nnz, dtype=np.int32)
+        v = np.zeros(nnz, dtype=np.float64)
+        self._call(lib().fem_csr_to_host(self._h, _ptr(rp), _ptr(ci), _ptr(v)))
+        return rp, ci, v
+
+    def device_csr(self):
+        """Borrowed device pointers (row_ptr, col_idx, val) of the library's CSR."""
+        rp, ci, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._call(lib().fem_csr_device(self._h, ctypes.byref(rp), ctypes.byref(ci), ctypes.byref(v)))
+        return rp.value, ci.value, v.value
+
+    def last_assembly_ms(self):
+        return lib().fem_last_assembly_ms(self._h)
+
+    def assembly_path(self):
+        return lib().fem_assembly_path(self._h).decode()
+
+    # ----------------------------------------------------------- convenience
+    def load_mesh(self, mesh):
+        """Adds a finite_element_method_amd.meshgen.Mesh (nodes then elements)."""
+        self.add_nodes(mesh.node_numbers, mesh.xyz)
+        if mesh.kind == "truss":
+            self.add_trusses(mesh.numbers, mesh.conn, mesh.props[:, 0], mesh.props[:, 1], mesh.props[:, 2])
+        elif mesh.kind == "beam":
+            self.add_beams(mesh.numbers, mesh.conn, mesh.props)
+        else:
+            self.add_plates(mesh.numbers, mesh.conn, mesh.props)
+        return self
+
+    @classmethod
+    def from_mesh(cls, mesh):
+        return cls(mesh.rel_tol, mesh.abs_tol, mesh.n_nodes).load_mesh(mesh)

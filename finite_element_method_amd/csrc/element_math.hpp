@@ -1,0 +1,267 @@
+// element_math.hpp -- device (gfx950) element kinematics and stiffness blocks.
+//
+// All arithmetic is FP64.  Each function cites the crate file it computes the
+// same quantity as; the formulas are evaluated in closed form per node-pair
+// block instead of the crate's dense map-backed products, so results agree with
+// the crate to rounding (the parity bar is 1e-12 relative).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace fem {
+
+constexpr int NODE_DOF = 6;
+constexpr double PI_F32 = 3.1415927410125732;  // std::f32::consts::PI widened (beam.rs:7,149)
+// sqrt(f64(1f32/3f32)): the crate's Gauss abscissa (plate.rs:1066-1091)
+constexpr float ONE_THIRD_F32 = 1.0f / 3.0f;
+constexpr double KROT6 = 1.0;  // plate.rs:25
+
+__host__ __device__ inline double cwt(double v, double abs_tol) {  // math_functions.rs:3-12
+  return fabs(v) < abs_tol ? 0.0 : v;
+}
+
+__host__ __device__ inline double norm3(double x, double y, double z) { return sqrt((x * x + y * y) + z * z); }
+
+// Rotation R with R.(a/|a|) = b/|b| (minimal rotation about a x b), entries
+// snapped with abs_tol; restates extended_matrix's
+// Vector3::rotation_matrix_to_align_with_vector (call sites truss.rs:80-84,
+// beam.rs:179-183, quadrilateral_4n_element_functions.rs:158-162).
+// Returns false for a zero-length vector.
+__host__ __device__ inline bool rotation_to_align(const double a[3], const double b[3], double abs_tol,
+                                                  double R[9]) {
+  double na = norm3(a[0], a[1], a[2]), nb = norm3(b[0], b[1], b[2]);
+  if (na == 0.0 || nb == 0.0) return false;
+  double ua[3] = {a[0] / na, a[1] / na, a[2] / na};
+  double ub[3] = {b[0] / nb, b[1] / nb, b[2] / nb};
+  double v[3] = {ua[1] * ub[2] - ua[2] * ub[1], ua[2] * ub[0] - ua[0] * ub[2], ua[0] * ub[1] - ua[1] * ub[0]};
+  double s = norm3(v[0], v[1], v[2]);
+  double c = (ua[0] * ub[0] + ua[1] * ub[1]) + ua[2] * ub[2];
+  if (s == 0.0) {
+    if (c > 0.0) {
+      for (int i = 0; i < 9; ++i) R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+    } else {
+      double e[3] = {1.0, 0.0, 0.0};
+      if (fabs(ua[0]) > fabs(ua[1]) || fabs(ua[0]) > fabs(ua[2])) {
+        if (fabs(ua[1]) <= fabs(ua[2])) { e[0] = 0.0; e[1] = 1.0; }
+        else { e[0] = 0.0; e[2] = 1.0; }
+      }
+      double n[3] = {ua[1] * e[2] - ua[2] * e[1], ua[2] * e[0] - ua[0] * e[2], ua[0] * e[1] - ua[1] * e[0]};
+      double nn = norm3(n[0], n[1], n[2]);
+      for (int i = 0; i < 3; ++i) n[i] /= nn;
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[i * 3 + j] = 2.0 * n[i] * n[j] - (i == j ? 1.0 : 0.0);
+    }
+  } else {
+    double k[3] = {v[0] / s, v[1] / s, v[2] / s};
+    double K[9] = {0.0, -k[2], k[1], k[2], 0.0, -k[0], -k[1], k[0], 0.0};
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        R[i * 3 + j] = (i == j ? c : 0.0) + s * K[i * 3 + j] + (1.0 - c) * k[i] * k[j];
+  }
+  for (int i = 0; i < 9; ++i) R[i] = cwt(R[i], abs_tol);
+  return true;
+}
+
+// ------------------------------------------------------------------ truss
+// Truss::create rotation (truss.rs:66-93)
+__host__ __device__ inline bool truss_rotation(const double d[3], double abs_tol, double R[9]) {
+  double L = norm3(d[0], d[1], d[2]);
+  double t[3] = {L, 0.0, 0.0};
+  return rotation_to_align(d, t, abs_tol, R);
+}
+
+// bar Jacobian at r = 0 (bar_2n_element_functions.rs:35-83): dx/dr = L/2
+__host__ __device__ inline double bar_jacobian(double L) {
+  double x1 = -1.0 * L / 2.0, x2 = L / 2.0;
+  return ((0.0 - x1 * 0.5) + 0.0) + x2 * 0.5;
+}
+
+// ------------------------------------------------------------------ beam
+// principal moments of inertia (beam.rs:119-160)
+__host__ __device__ inline void beam_principal(double i11, double i22, double i12, double rel_tol, double& i11p,
+                                               double& i22p, double& angle) {
+  if (i11 != i22) {
+    angle = atan(2.0 * i12 / (i22 - i11)) / 2.0;
+  } else {
+    double i11m, i22m;
+    if (i22 < i11) { i11m = i11; i22m = (fabs(i22) - fabs(i22) * rel_tol) * i22 / fabs(i22); }
+    else { i11m = (fabs(i11) - fabs(i11) * rel_tol) * i11 / fabs(i11); i22m = i22; }
+    angle = atan(2.0 * i12 / (i22m - i11m)) / 2.0;
+  }
+  double c = cos(angle), s = sin(angle), s2 = sin(2.0 * angle);
+  i11p = i11 * (c * c) + i22 * (s * s) - i12 * s2;
+  i22p = i11 * (s * s) + i22 * (c * c) + i12 * s2;
+  for (int i = 1; i11p < i22p && i <= 64; ++i) {
+    angle = (atan(2.0 * i12 / (i22 - i11)) + PI_F32 * double(float(i))) / 2.0;
+    c = cos(angle); s = sin(angle); s2 = sin(2.0 * angle);
+    i11p = i11 * (c * c) + i22 * (s * s) - i12 * s2;
+    i22p = i11 * (s * s) + i22 * (c * c) + i12 * s2;
+  }
+}
+
+// beam direction cosines with the section roll (beam.rs:162-258)
+__host__ __device__ inline bool beam_rotation(const double d[3], const double v1[3], double angle, double abs_tol,
+                                              double out[9]) {
+  double L = norm3(d[0], d[1], d[2]);
+  double R0[9];
+  double t[3] = {L, 0.0, 0.0};
+  if (!rotation_to_align(d, t, abs_tol, R0)) return false;
+  double f = ((v1[0] * d[0] + v1[1] * d[1]) + v1[2] * d[2]) / ((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+  double p[3] = {v1[0] - f * d[0], v1[1] - f * d[1], v1[2] - f * d[2]};
+  double tp[3];
+  for (int i = 0; i < 3; ++i) tp[i] = (R0[i * 3] * p[0] + R0[i * 3 + 1] * p[1]) + R0[i * 3 + 2] * p[2];
+  double cosang = ((0.0 * tp[0] + 0.0 * tp[1]) + 1.0 * tp[2]) / (1.0 * norm3(tp[0], tp[1], tp[2]));
+  double total = angle + acos(cosang);
+  double cx = cwt(d[0] / L, abs_tol), cy = cwt(d[1] / L, abs_tol), cz = cwt(d[2] / L, abs_tol);
+  double cxz = cwt(sqrt(cx * cx + cz * cz), abs_tol);
+  double c = cwt(cos(total), abs_tol), s = cwt(sin(total), abs_tol);
+  if (cxz != 0.0) {
+    out[0] = cx; out[1] = cy; out[2] = cz;
+    out[3] = (-1.0 * cx * cy * c - cz * s) / cxz;
+    out[4] = cxz * c;
+    out[5] = (-1.0 * cy * cz * c + cx * s) / cxz;
+    out[6] = (cx * cy * s - cz * c) / cxz;
+    out[7] = -1.0 * cxz * s;
+    out[8] = (cy * cz * s + cx * c) / cxz;
+  } else {
+    out[0] = 0.0; out[1] = cy; out[2] = 0.0;
+    out[3] = -1.0 * cy * c; out[4] = 0.0; out[5] = s;
+    out[6] = cy * s; out[7] = 0.0; out[8] = c;
+  }
+  return true;
+}
+
+// ------------------------------------------------------------------ plate
+// frame of a quadrilateral from nodes 2, 3, 4 (quadrilateral_4n_element_functions.rs:131-171)
+__host__ __device__ inline bool quad_rotation(const double p2[3], const double p3[3], const double p4[3],
+                                              double abs_tol, double R[9]) {
+  double e34[3] = {p4[0] - p3[0], p4[1] - p3[1], p4[2] - p3[2]};
+  double e32[3] = {p2[0] - p3[0], p2[1] - p3[1], p2[2] - p3[2]};
+  double n[3] = {e34[1] * e32[2] - e34[2] * e32[1], e34[2] * e32[0] - e34[0] * e32[2],
+                 e34[0] * e32[1] - e34[1] * e32[0]};
+  double t[3] = {0.0, 0.0, norm3(n[0], n[1], n[2])};
+  return rotation_to_align(n, t, abs_tol, R);
+}
+
+// Per-integration-point kinematics of the 4-node plate in its local frame
+// (node 3 at the origin): dh/dx, dh/dy (quad:613-653), det J (quad:477-503),
+// MITC4 transverse-shear rows (plate.rs:392-511).
+struct PlateIP {
+  double w;        // det J * alpha
+  double dx[4], dy[4];
+  double sr[4][3];  // gamma_rz row: (w, theta_x, theta_y) per node
+  double ss[4][3];  // gamma_sz row
+};
+
+// xl = {x1, y1, x2, y2, x4, y4}: local coordinates of nodes 1, 2, 4 (node 3 = origin)
+__host__ __device__ inline void plate_ip(const double xl[6], double r, double s, PlateIP& o) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double q = 0.25;
+  // dx/dr etc. (quad:252-338) with x3 = y3 = 0
+  double x1q = x1 * q, x2q = x2 * q, x4q = x4 * q, y1q = y1 * q, y2q = y2 * q, y4q = y4 * q;
+  double j00 = ((((x1q + x1q * s) - x2q) - x2q * s) + 0.0 * s) + x4q - x4q * s;
+  double j01 = ((((y1q + y1q * s) - y2q) - y2q * s) + 0.0 * s) + y4q - y4q * s;
+  double j10 = ((((x1q + x1q * r) + x2q) - x2q * r) - x4q) - x4q * r;
+  double j11 = ((((y1q + y1q * r) + y2q) - y2q * r) - y4q) - y4q * r;
+  double det = j00 * j11 - j01 * j10;
+  double i00 = j11 / det, i01 = -1.0 * j01 / det, i10 = -1.0 * j10 / det, i11 = j00 / det;
+  // shape-function derivatives (quad:505-583)
+  double dr[4] = {q + q * s, -(q + q * s), -(q - q * s), q - q * s};
+  double ds[4] = {q + q * r, q - q * r, -(q - q * r), -(q + q * r)};
+  for (int a = 0; a < 4; ++a) {
+    o.dx[a] = i00 * dr[a] + i01 * ds[a];
+    o.dy[a] = i10 * dr[a] + i11 * ds[a];
+  }
+  o.w = det;  // alpha = 1 * 1 for the 2x2 rule
+  double ax = x1 - x2 + x4, bx = x1 - x2 - x4, cx = x1 + x2 - x4;
+  double ay = y1 - y2 + y4, by = y1 - y2 - y4, cy = y1 + y2 - y4;
+  double mr = sqrt((cx + r * bx) * (cx + r * bx) + (cy + r * by) * (cy + r * by)) / (8.0 * det);
+  double ms = sqrt((ax + s * bx) * (ax + s * bx) + (ay + s * by) * (ay + s * by)) / (8.0 * det);
+  double ps = (1.0 + s), msn = (1.0 - s), pr = (1.0 + r), mrn = (1.0 - r);
+  double tx12 = -(y1 - y2) / 4.0, ty12 = (x1 - x2) / 4.0;
+  double tx43 = -(y4) / 4.0, ty43 = (x4) / 4.0;
+  double tx14 = -(y1 - y4) / 4.0, ty14 = (x1 - x4) / 4.0;
+  double tx23 = -(y2) / 4.0, ty23 = (x2) / 4.0;
+  o.sr[0][0] = ps / 2.0 * mr;  o.sr[0][1] = ps * tx12 * mr;  o.sr[0][2] = ps * ty12 * mr;
+  o.sr[1][0] = -ps / 2.0 * mr; o.sr[1][1] = ps * tx12 * mr;  o.sr[1][2] = ps * ty12 * mr;
+  o.sr[2][0] = -msn / 2.0 * mr; o.sr[2][1] = msn * tx43 * mr; o.sr[2][2] = msn * ty43 * mr;
+  o.sr[3][0] = msn / 2.0 * mr; o.sr[3][1] = msn * tx43 * mr; o.sr[3][2] = msn * ty43 * mr;
+  o.ss[0][0] = pr / 2.0 * ms;  o.ss[0][1] = pr * tx14 * ms;  o.ss[0][2] = pr * ty14 * ms;
+  o.ss[1][0] = mrn / 2.0 * ms; o.ss[1][1] = mrn * tx23 * ms; o.ss[1][2] = mrn * ty23 * ms;
+  o.ss[2][0] = -mrn / 2.0 * ms; o.ss[2][1] = mrn * tx23 * ms; o.ss[2][2] = mrn * ty23 * ms;
+  o.ss[3][0] = -pr / 2.0 * ms; o.ss[3][1] = pr * tx14 * ms;  o.ss[3][2] = pr * ty14 * ms;
+}
+
+struct PlateMat {
+  double cm00, cm01, cm22;  // membrane: E t/(1-nu^2) * [1, nu, (1-nu)/2]
+  double cb00, cb01, cb22;  // bending:  E t^3/(12(1-nu^2)) * [...]
+  double cs;                // shear:    E t k/(2(1+nu)) (alpha = 1)
+};
+// plate.rs:532-636
+__host__ __device__ inline PlateMat plate_material(double E, double nu, double t, double k) {
+  PlateMat m;
+  double cm = E * t / (1.0 - nu * nu);
+  double cb = E * (t * t * t) / (12.0 * (1.0 - nu * nu));
+  double half = (1.0 - nu) / 2.0;
+  m.cm00 = 1.0 * cm; m.cm01 = nu * cm; m.cm22 = half * cm;
+  m.cb00 = 1.0 * cb; m.cb01 = nu * cb; m.cb22 = half * cb;
+  m.cs = E * t * k / (2.0 * (1.0 + nu));
+  return m;
+}
+
+// Local-frame 6x6 block (p, q) of the plate stiffness accumulated over one
+// integration point: membrane (u, v), bending (theta_x, theta_y) and MITC4
+// shear (w, theta_x, theta_y) terms of plate.rs:513-672.
+__host__ __device__ inline void plate_block_add_ip(const PlateIP& g, const PlateMat& m, int p, int q,
+                                                   double kb[36]) {
+  const double w = g.w;
+  const double dxp = g.dx[p], dyp = g.dy[p], dxq = g.dx[q], dyq = g.dy[q];
+  // membrane: B_mem columns u_a = (dx_a, 0, dy_a), v_a = (0, dy_a, dx_a)
+  double muu = m.cm00 * dxp * dxq + m.cm22 * dyp * dyq;
+  double muv = m.cm01 * dxp * dyq + m.cm22 * dyp * dxq;
+  double mvu = m.cm01 * dyp * dxq + m.cm22 * dxp * dyq;
+  double mvv = m.cm00 * dyp * dyq + m.cm22 * dxp * dxq;
+  // bending: B_bend columns tx_a = (0, dy_a, dx_a), ty_a = (-dx_a, 0, -dy_a)
+  double bxx = m.cb00 * dyp * dyq + m.cb22 * dxp * dxq;
+  double bxy = -(m.cb01 * dyp * dxq + m.cb22 * dxp * dyq);
+  double byx = -(m.cb01 * dxp * dyq + m.cb22 * dyp * dxq);
+  double byy = m.cb00 * dxp * dxq + m.cb22 * dyp * dyq;
+  kb[0 * 6 + 0] += muu * w; kb[0 * 6 + 1] += muv * w;
+  kb[1 * 6 + 0] += mvu * w; kb[1 * 6 + 1] += mvv * w;
+  // shear: (w, tx, ty) x (w, tx, ty)
+  const double cs = m.cs;
+  for (int i = 0; i < 3; ++i)
+    for (int j = 0; j < 3; ++j) {
+      double sh = cs * (g.sr[p][i] * g.sr[q][j] + g.ss[p][i] * g.ss[q][j]);
+      double be = 0.0;
+      if (i == 1 && j == 1) be = bxx;
+      else if (i == 1 && j == 2) be = bxy;
+      else if (i == 2 && j == 1) be = byx;
+      else if (i == 2 && j == 2) be = byy;
+      kb[(2 + i) * 6 + 2 + j] += be * w + sh * w;
+    }
+}
+
+// Gauss abscissa of the 2x2 rule in the crate's point order (g,g), (-g,g), (-g,-g), (g,-g)
+__host__ __device__ inline double plate_gauss() { return sqrt(double(ONE_THIRD_F32)); }
+
+// G = R6^T kb R6 with R6 = diag(R, R), in place (no-op when R is the identity)
+__host__ __device__ inline void rotate_block(double kb[36], const double R[9]) {
+  bool ident = R[0] == 1.0 && R[4] == 1.0 && R[8] == 1.0 && R[1] == 0.0 && R[2] == 0.0 && R[3] == 0.0 &&
+               R[5] == 0.0 && R[6] == 0.0 && R[7] == 0.0;
+  if (ident) return;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj) {
+      double A[9], T[9];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[i * 3 + j] = kb[(3 * bi + i) * 6 + 3 * bj + j];
+      // T = R^T A
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) T[i * 3 + j] = (R[0 * 3 + i] * A[0 * 3 + j] + R[1 * 3 + i] * A[1 * 3 + j]) + R[2 * 3 + i] * A[2 * 3 + j];
+      // kb = T R
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j)
+          kb[(3 * bi + i) * 6 + 3 * bj + j] = (T[i * 3 + 0] * R[0 * 3 + j] + T[i * 3 + 1] * R[1 * 3 + j]) + T[i * 3 + 2] * R[2 * 3 + j];
+    }
+}
+
+}  // namespace fem

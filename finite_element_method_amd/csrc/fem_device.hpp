@@ -1,0 +1,77 @@
+// fem_device.hpp -- interface between the host model (fem_host.cpp) and the
+// device runtime (fem_device.hip).  Plain pointers and sizes only.
+#pragma once
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace femdev {
+
+struct Device;  // opaque: device buffers, maps and streams of one model
+
+Device* create_device(std::string& err);
+void destroy_device(Device* d);
+
+// element type ids (match fem_element_type)
+constexpr int TRUSS = 0, BEAM = 1, PLATE = 2;
+constexpr int NODES_PER[3] = {2, 2, 4};
+constexpr int PARAMS_PER[3] = {3, 11, 4};
+
+// Uploads node coordinates (n x 3) and one element type's connectivity (node
+// indices, n x nodes_per) and parameters (n x params_per).
+std::string upload_nodes(Device* d, const double* xyz, size_t n);
+std::string upload_elements(Device* d, int type, size_t n, const uint32_t* conn_idx, const double* params);
+
+// Runs the per-element setup on the device (rotation frames; beam principal
+// moments) and returns the zero pattern of each element's 3x3 rotation (bit k
+// set when R[k] != 0) in `rzero` (n entries).  A set bit 15 flags an element
+// whose frame could not be formed (zero-length vector).
+std::string run_setup(Device* d, int type, size_t n, double rel_tol, double abs_tol, std::vector<uint16_t>& rzero);
+
+// Rotation matrix of element `pos` (insertion position within its type).
+std::string get_rotation(Device* d, int type, size_t pos, double out[9]);
+
+// CSR pattern upload (row_ptr: n_rows + 1, col_idx: nnz).
+std::string upload_pattern(Device* d, int64_t n_rows, const int64_t* row_ptr, int64_t nnz, const int32_t* col_idx);
+
+// Generic value path: element items write their structural entries into a
+// contribution buffer, slots gather them in element-insertion order.
+struct GenericMaps {
+  // per element class: per pair mask (bit 6*i+j of the node-pair block) and the
+  // offset of the pair's first entry within the element record
+  std::vector<uint64_t> class_mask;  // n_class * 16
+  std::vector<uint32_t> class_off;   // n_class * 16
+  // per type: element class id and record base in the contribution buffer
+  std::vector<uint16_t> elem_class[3];
+  std::vector<int64_t> elem_base[3];
+  int64_t n_contrib = 0;
+  std::vector<int64_t> slot_ptr;   // nnz + 1
+  std::vector<uint32_t> slot_src;  // n_contrib
+};
+std::string upload_generic(Device* d, const GenericMaps& g);
+
+// Plate fused path: per patch (one workgroup) the element list, owned nodes
+// and gather lists; see fem_device.hip for the layout.
+struct PlateMaps {
+  int n_patches = 0;
+  std::vector<uint32_t> patch_elem_ptr, patch_elems;    // elements touching the patch's nodes
+  std::vector<uint32_t> patch_node_ptr, patch_nodes;    // owned nodes (ascending)
+  std::vector<uint32_t> node_blk_ptr;                   // per owned node (flattened), block range
+  std::vector<uint32_t> blk_info;                       // per block: contributor count | diag flag
+  std::vector<uint16_t> blk_contrib;                    // per block: up to 4 (local element << 4 | p << 2 | q)
+  int max_patch_elems = 0;
+};
+std::string upload_plate_maps(Device* d, const PlateMaps& pm);
+
+// Value kernels.  `val` is nnz doubles of device memory, `stream` a hipStream_t.
+std::string assemble_generic(Device* d, double* val, void* stream);
+std::string assemble_plate_fused(Device* d, double* val, void* stream);
+double last_ms(Device* d);
+
+double* values(Device* d);
+const int64_t* row_ptr(Device* d);
+const int32_t* col_idx(Device* d);
+std::string copy_csr_to_host(Device* d, int64_t n_rows, int64_t nnz, int64_t* rp, int32_t* ci, double* v);
+
+}  // namespace femdev

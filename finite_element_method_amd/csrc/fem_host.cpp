@@ -1,0 +1,759 @@
+// fem_host.cpp -- host side of the assembly library: the crate's FEM<V> model
+// (nodes, elements, validation in the crate's order and with its messages),
+// the sparsity-pattern / element-to-slot map builder, and the C ABI of
+// include/fem_amd.h.  All element arithmetic runs on the device
+// (fem_device.hip, plate_fused.hip); the host only validates and indexes.
+#include <algorithm>
+#include <array>
+#include <charconv>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/fem_amd.h"
+#include "element_math.hpp"
+#include "fem_device.hpp"
+
+using femdev::BEAM;
+using femdev::PLATE;
+using femdev::TRUSS;
+
+namespace {
+
+// Rust `{:?}` of an f64 (shortest round-trip digits; exponential form when
+// 0 < |x| < 1e-4 or |x| >= 1e16), used in the crate's error messages.
+std::string rust_debug(double x) {
+  if (std::isnan(x)) return "NaN";
+  if (std::isinf(x)) return x < 0 ? "-inf" : "inf";
+  char buf[64];
+  auto r = std::to_chars(buf, buf + 64, std::fabs(x), std::chars_format::scientific);
+  std::string s(buf, r.ptr);
+  size_t e = s.find('e');
+  int ex = std::stoi(s.substr(e + 1));
+  std::string digits;
+  for (size_t i = 0; i < e; ++i)
+    if (s[i] != '.') digits += s[i];
+  while (digits.size() > 1 && digits.back() == '0') digits.pop_back();
+  std::string out = std::signbit(x) ? "-" : "";
+  double ax = std::fabs(x);
+  if (ax == 0.0) return out + "0.0";
+  if (ax < 1e-4 || ax >= 1e16) {
+    out += digits[0];
+    if (digits.size() > 1) out += "." + digits.substr(1);
+    return out + "e" + std::to_string(ex);
+  }
+  int point = ex + 1;
+  if (point <= 0) return out + "0." + std::string(size_t(-point), '0') + digits;
+  if (int(digits.size()) <= point) return out + digits + std::string(size_t(point) - digits.size(), '0') + ".0";
+  return out + digits.substr(0, size_t(point)) + "." + digits.substr(size_t(point));
+}
+
+std::string u(uint64_t v) { return std::to_string(v); }
+
+uint64_t coord_key(double x, double y, double z) {
+  auto h = [](double v) {
+    if (v == 0.0) v = 0.0;
+    uint64_t b;
+    std::memcpy(&b, &v, 8);
+    return b;
+  };
+  return h(x) * 0x9E3779B97F4A7C15ull ^ (h(y) + 0x7F4A7C15ull) * 0xC2B2AE3D27D4EB4Full ^ h(z);
+}
+uint64_t pair_key(uint32_t a, uint32_t b) { return a < b ? (uint64_t(a) << 32 | b) : (uint64_t(b) << 32 | a); }
+
+struct ElementSet {
+  std::vector<uint32_t> number;
+  std::vector<uint32_t> conn;    // node indices
+  std::vector<double> params;
+  std::vector<uint64_t> seq;     // global insertion sequence
+  std::unordered_map<uint32_t, uint32_t> by_number;
+  size_t size() const { return number.size(); }
+};
+
+}  // namespace
+
+struct fem_model {
+  double rel_tol = 0, abs_tol = 0;
+  uint32_t nodes_number = 0;
+  std::string err;
+  std::unordered_map<uint32_t, uint32_t> node_idx;
+  std::vector<uint32_t> node_number;
+  std::vector<double> xyz;
+  std::unordered_map<uint64_t, std::vector<uint32_t>> coord_hash;
+  ElementSet set[3];
+  std::unordered_map<uint64_t, uint32_t> truss_pairs, beam_pairs;
+  std::vector<std::vector<uint32_t>> node_plates;
+  uint64_t seq = 0;
+  // device state
+  femdev::Device* dev = nullptr;
+  bool dirty = true;           // host model changed since the last device upload
+  bool pattern_valid = false;
+  std::vector<uint16_t> rzero[3];
+  int64_t nnz = 0;
+  std::string path = "none";
+};
+
+namespace {
+
+int fail(fem_model* m, int code, const std::string& msg) {
+  m->err = msg;
+  return code;
+}
+
+void clear_model(fem_model* m, uint32_t nodes_number) {
+  m->nodes_number = nodes_number;
+  m->node_idx.clear();
+  m->node_number.clear();
+  m->xyz.clear();
+  m->coord_hash.clear();
+  for (auto& s : m->set) s = ElementSet();
+  m->truss_pairs.clear();
+  m->beam_pairs.clear();
+  m->node_plates.clear();
+  m->seq = 0;
+  m->dirty = true;
+  m->pattern_valid = false;
+  m->nnz = 0;
+  for (auto& r : m->rzero) r.clear();
+}
+
+bool node_exists(fem_model* m, uint32_t number) {  // methods_for_node_data_handle.rs:81-86
+  if (!m->node_idx.count(number)) {
+    m->err = "Node with number " + u(number) + " does not exist!";
+    return false;
+  }
+  return true;
+}
+
+const double* node_xyz(const fem_model* m, uint32_t idx) { return &m->xyz[3 * size_t(idx)]; }
+
+bool positive_or(fem_model* m, double v, const char* what, double shown) {
+  if (v <= 0.0 || std::isnan(v)) {
+    if (!(v <= 0.0)) return true;  // NaN compares false in the crate too
+    m->err = std::string(what) + " " + rust_debug(shown) + " is less or equal to zero!";
+    return false;
+  }
+  return true;
+}
+
+// -------------------------------------------------------- plate geometry checks
+// quadrilateral_4n_element_functions.rs:14-86
+bool quad_on_line(const double* p[4]) {
+  const int pairs[4][3] = {{0, 1, 3}, {1, 0, 2}, {2, 1, 3}, {3, 2, 0}};
+  for (auto& pr : pairs) {
+    const double *o = p[pr[0]], *a = p[pr[1]], *b = p[pr[2]];
+    double v1[3] = {a[0] - o[0], a[1] - o[1], a[2] - o[2]};
+    double v2[3] = {b[0] - o[0], b[1] - o[1], b[2] - o[2]};
+    double c[3] = {v1[1] * v2[2] - v1[2] * v2[1], v1[2] * v2[0] - v1[0] * v2[2], v1[0] * v2[1] - v1[1] * v2[0]};
+    if (fem::norm3(c[0], c[1], c[2]) == 0.0) return true;
+  }
+  return false;
+}
+// quadrilateral_4n_element_functions.rs:88-129
+bool quad_on_plane(const double* p[4], double abs_tol) {
+  const double *p1 = p[0], *p2 = p[1], *p3 = p[2], *p4 = p[3];
+  double v32[3] = {p2[0] - p3[0], p2[1] - p3[1], p2[2] - p3[2]};
+  double v34[3] = {p4[0] - p3[0], p4[1] - p3[1], p4[2] - p3[2]};
+  double a = v32[1] * v34[2] - v32[2] * v34[1], b = v32[2] * v34[0] - v32[0] * v34[2],
+         c = v32[0] * v34[1] - v32[1] * v34[0];
+  double d = -1.0 * (a * p3[0] + b * p3[1] + c * p3[2]);
+  return fem::cwt(a * p1[0] + b * p1[1] + c * p1[2] + d, abs_tol) == 0.0;
+}
+
+// convex_hull_on_plane.rs:14-214: polar-angle quicksort about the lowest point,
+// then removal of non-left turns; the crate requires 4 hull points.
+struct HP {
+  uint32_t n;
+  double x, y;
+};
+double hp_angle(const HP& p) {
+  double rhs = std::sqrt((0.0 - p.x) * (0.0 - p.x) + (0.0 - p.y) * (0.0 - p.y));
+  double sp = (0.0 - 1.0) * (0.0 - p.x) + (0.0 - 0.0) * (0.0 - p.y);
+  return std::acos(sp / (1.0 * rhs)) * (180.0 / M_PI);
+}
+void hp_sort(HP* a, long low, long high) {
+  if (low >= high) return;
+  long pivot = high, store = low - 1, last = high;
+  while (true) {
+    store += 1;
+    while (hp_angle(a[store]) < hp_angle(a[pivot])) store += 1;
+    last -= 1;
+    while (last >= 0 && hp_angle(a[last]) > hp_angle(a[pivot])) last -= 1;
+    if (store >= last) break;
+    std::swap(a[store], a[last]);
+  }
+  std::swap(a[store], a[pivot]);
+  hp_sort(a, low, store - 1);
+  hp_sort(a, store + 1, high);
+}
+size_t hull_size(std::vector<HP> u) {
+  double sx = u[0].x, my = u[0].y;
+  size_t mp = 0;
+  for (size_t i = 0; i < u.size(); ++i)
+    if (u[i].y < my) { sx = u[i].x; my = u[i].y; mp = i; }
+  std::swap(u[0], u[mp]);
+  for (auto& p : u) { p.x -= sx; p.y -= my; }
+  hp_sort(u.data() + 1, 0, long(u.size()) - 2);
+  size_t i = 0;
+  while (i + 2 < u.size()) {
+    double area = (u[i + 1].x - u[i].x) * (u[i + 2].y - u[i].y) - (u[i + 1].y - u[i].y) * (u[i + 2].x - u[i].x);
+    if (area <= 0.0) u.erase(u.begin() + long(i) + 1);
+    else i += 1;
+  }
+  return u.size();
+}
+// quadrilateral_4n_element_functions.rs:173-250; returns -1 when no frame exists
+int quad_hull_points(const uint32_t num[4], const double* p[4], double abs_tol) {
+  double R[9];
+  if (!fem::quad_rotation(p[1], p[2], p[3], abs_tol, R)) return -1;
+  auto tr = [&](const double* q, double& x, double& y) {
+    double d[3] = {q[0] - p[2][0], q[1] - p[2][1], q[2] - p[2][2]};
+    x = (R[0] * d[0] + R[1] * d[1]) + R[2] * d[2];
+    y = (R[3] * d[0] + R[4] * d[1]) + R[5] * d[2];
+  };
+  std::vector<HP> hp(4);
+  hp[0].n = num[0]; tr(p[0], hp[0].x, hp[0].y);
+  hp[1].n = num[1]; tr(p[1], hp[1].x, hp[1].y);
+  hp[2] = {num[2], 0.0, 0.0};
+  hp[3].n = num[3]; tr(p[3], hp[3].x, hp[3].y);
+  return int(hull_size(hp));
+}
+
+// -------------------------------------------------------- structural patterns
+// Node-pair block masks: bit 6*i+j of the 6x6 block of K_e for local nodes (p, q).
+using Mask = uint64_t;
+inline bool bit(Mask m, int i, int j) { return (m >> (6 * i + j)) & 1ull; }
+inline void setb(Mask& m, int i, int j) { m |= 1ull << (6 * i + j); }
+
+// structural pattern of R6^T L R6 from the zero pattern of R
+Mask rotate_mask(Mask L, uint16_t rz) {
+  auto r = [&](int k, int i) { return (rz >> (3 * k + i)) & 1; };  // R[k][i] != 0
+  Mask out = 0;
+  for (int bi = 0; bi < 2; ++bi)
+    for (int bj = 0; bj < 2; ++bj)
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+          bool nz = false;
+          for (int k = 0; k < 3 && !nz; ++k)
+            for (int l = 0; l < 3 && !nz; ++l)
+              nz = r(k, i) && bit(L, 3 * bi + k, 3 * bj + l) && r(l, j);
+          if (nz) setb(out, 3 * bi + i, 3 * bj + j);
+        }
+  return out;
+}
+Mask element_pair_mask(int type, uint16_t rz, int p, int q) {
+  Mask L = 0;
+  if (type == TRUSS) {
+    // truss.rs: only u-u is non-zero locally; global block R0^T R0 scaled
+    Mask out = 0;
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        if (((rz >> i) & 1) && ((rz >> j) & 1)) setb(out, i, j);
+    return out;
+  }
+  if (type == BEAM) {
+    // beam.rs:495-563: u | (v, tw) | (w, tv) | tu | tv | tw
+    setb(L, 0, 0);
+    for (int a : {1, 5})
+      for (int b : {1, 5}) setb(L, a, b);
+    for (int a : {2, 4})
+      for (int b : {2, 4}) setb(L, a, b);
+    setb(L, 3, 3);
+    Mask out = rotate_mask(L, rz);
+    // shear isotropy: kv (r1 r1^T + r2 r2^T) = kv (I - r0 r0^T) in the translational block
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j)
+        if (i != j) {
+          bool nz = ((rz >> i) & 1) && ((rz >> j) & 1);
+          out &= ~(1ull << (6 * i + j));
+          if (nz) setb(out, i, j);
+        }
+    return out;
+  }
+  // plate.rs:160-511: membrane (u, v), bending + MITC4 shear (w, tx, ty), drilling on the diagonal
+  for (int a : {0, 1})
+    for (int b : {0, 1}) setb(L, a, b);
+  for (int a : {2, 3, 4})
+    for (int b : {2, 3, 4}) setb(L, a, b);
+  if (p == q) setb(L, 5, 5);
+  return rotate_mask(L, rz);
+}
+
+// -------------------------------------------------------- pattern builder
+struct PairRef {
+  uint32_t b;
+  Mask mask;
+};
+
+std::string build_pattern(fem_model* m) {
+  std::string e;
+  const size_t N = m->nodes_number;
+  const int64_t n_rows = int64_t(N) * 6;
+  // element classes (type, rzero) -> per pair masks
+  std::unordered_map<uint32_t, uint16_t> class_id;
+  femdev::GenericMaps g;
+  auto get_class = [&](int t, uint16_t rz) -> uint16_t {
+    uint32_t key = uint32_t(t) << 16 | rz;
+    auto it = class_id.find(key);
+    if (it != class_id.end()) return it->second;
+    uint16_t id = uint16_t(class_id.size());
+    class_id[key] = id;
+    int nn = femdev::NODES_PER[t];
+    uint32_t off = 0;
+    for (int pq = 0; pq < 16; ++pq) {
+      Mask mk = 0;
+      if (pq < nn * nn) mk = element_pair_mask(t, rz, pq / nn, pq % nn);
+      g.class_mask.push_back(mk);
+      g.class_off.push_back(off);
+      off += uint32_t(__builtin_popcountll(mk));
+    }
+    return id;
+  };
+  // element visiting order = global insertion order
+  struct ERef {
+    uint64_t seq;
+    int t;
+    uint32_t pos;
+  };
+  std::vector<ERef> order;
+  for (int t = 0; t < 3; ++t)
+    for (uint32_t i = 0; i < m->set[t].size(); ++i) order.push_back({m->set[t].seq[i], t, i});
+  std::sort(order.begin(), order.end(), [](const ERef& a, const ERef& b) { return a.seq < b.seq; });
+  for (int t = 0; t < 3; ++t) {
+    g.elem_class[t].resize(m->set[t].size());
+    g.elem_base[t].resize(m->set[t].size());
+  }
+  int64_t ncontrib = 0;
+  std::vector<uint32_t> deg(N + 1, 0);
+  for (auto& r : order) {
+    uint16_t rz = m->rzero[r.t][r.pos];
+    if (rz & 0x8000) return "Zero length vector could not be aligned!";
+    uint16_t cls = get_class(r.t, rz);
+    g.elem_class[r.t][r.pos] = cls;
+    g.elem_base[r.t][r.pos] = ncontrib;
+    ncontrib += g.class_off[cls * 16 + 15] + __builtin_popcountll(g.class_mask[cls * 16 + 15]);
+    int nn = femdev::NODES_PER[r.t];
+    for (int p = 0; p < nn; ++p) deg[m->set[r.t].conn[size_t(r.pos) * nn + p]] += nn;
+  }
+  g.n_contrib = ncontrib;
+  if (ncontrib > int64_t(UINT32_MAX)) return "too many element contributions for 32-bit slot maps";
+  // node -> (neighbour, mask) lists, merged
+  std::vector<uint64_t> nptr(N + 1, 0);
+  for (size_t a = 0; a < N; ++a) nptr[a + 1] = nptr[a] + deg[a];
+  std::vector<PairRef> pr(nptr[N]);
+  std::vector<uint64_t> fill(nptr.begin(), nptr.end() - 1);
+  for (auto& r : order) {
+    int nn = femdev::NODES_PER[r.t];
+    const uint32_t* c = &m->set[r.t].conn[size_t(r.pos) * nn];
+    uint16_t cls = g.elem_class[r.t][r.pos];
+    for (int p = 0; p < nn; ++p)
+      for (int q = 0; q < nn; ++q) pr[fill[c[p]]++] = {c[q], g.class_mask[cls * 16 + p * nn + q]};
+  }
+  std::vector<uint64_t> bptr(N + 1, 0);
+  std::vector<PairRef> blocks;
+  blocks.reserve(pr.size() / 2 + 1);
+  for (size_t a = 0; a < N; ++a) {
+    auto* b0 = pr.data() + nptr[a];
+    auto* b1 = pr.data() + nptr[a + 1];
+    std::sort(b0, b1, [](const PairRef& x, const PairRef& y) { return x.b < y.b; });
+    for (auto* it = b0; it != b1;) {
+      PairRef acc = *it;
+      for (++it; it != b1 && it->b == acc.b; ++it) acc.mask |= it->mask;
+      if (acc.mask) blocks.push_back(acc);
+    }
+    bptr[a + 1] = blocks.size();
+  }
+  pr.clear();
+  pr.shrink_to_fit();
+  // rows: length and per-block row offsets
+  std::vector<int64_t> row_ptr(static_cast<size_t>(n_rows) + 1, 0);
+  std::vector<uint16_t> boff(blocks.size() * 6);
+  for (size_t a = 0; a < N; ++a)
+    for (int i = 0; i < 6; ++i) {
+      int64_t len = 0;
+      for (uint64_t k = bptr[a]; k < bptr[a + 1]; ++k) {
+        if (len > 65535) return "row too long for the slot map";
+        boff[k * 6 + i] = uint16_t(len);
+        len += __builtin_popcountll((blocks[k].mask >> (6 * i)) & 63ull);
+      }
+      row_ptr[6 * a + i + 1] = len;
+    }
+  for (int64_t r = 0; r < n_rows; ++r) row_ptr[r + 1] += row_ptr[r];
+  const int64_t nnz = row_ptr[n_rows];
+  if (nnz > int64_t(INT32_MAX) * 4) return "pattern too large";
+  std::vector<int32_t> col(static_cast<size_t>(nnz));
+  for (size_t a = 0; a < N; ++a)
+    for (int i = 0; i < 6; ++i) {
+      int64_t pos = row_ptr[6 * a + i];
+      for (uint64_t k = bptr[a]; k < bptr[a + 1]; ++k)
+        for (int j = 0; j < 6; ++j)
+          if (bit(blocks[k].mask, i, j)) col[size_t(pos++)] = int32_t(6 * int64_t(blocks[k].b) + j);
+    }
+  e = femdev::upload_pattern(m->dev, n_rows, row_ptr.data(), nnz, col.data());
+  if (!e.empty()) return e;
+  m->nnz = nnz;
+  // slot of every contribution, then gather lists in insertion order
+  std::vector<uint32_t> slot_of(static_cast<size_t>(ncontrib));
+  for (auto& r : order) {
+    int nn = femdev::NODES_PER[r.t];
+    const uint32_t* c = &m->set[r.t].conn[size_t(r.pos) * nn];
+    uint16_t cls = g.elem_class[r.t][r.pos];
+    int64_t base = g.elem_base[r.t][r.pos];
+    for (int p = 0; p < nn; ++p)
+      for (int q = 0; q < nn; ++q) {
+        int pq = p * nn + q;
+        Mask mk = g.class_mask[cls * 16 + pq];
+        if (!mk) continue;
+        uint32_t a = c[p], b = c[q];
+        auto* k0 = blocks.data() + bptr[a];
+        auto* k1 = blocks.data() + bptr[a + 1];
+        auto* kb = std::lower_bound(k0, k1, b, [](const PairRef& x, uint32_t v) { return x.b < v; });
+        size_t k = size_t(kb - blocks.data());
+        Mask um = blocks[k].mask;
+        int64_t ci = base + g.class_off[cls * 16 + pq];
+        for (int i = 0; i < 6; ++i)
+          for (int j = 0; j < 6; ++j)
+            if (bit(mk, i, j)) {
+              int below = __builtin_popcountll(((um >> (6 * i)) & 63ull) & ((1ull << j) - 1));
+              slot_of[size_t(ci++)] = uint32_t(row_ptr[6 * a + i] + boff[k * 6 + i] + below);
+            }
+      }
+  }
+  g.slot_ptr.assign(size_t(nnz) + 1, 0);
+  for (uint32_t s : slot_of) g.slot_ptr[s + 1] += 1;
+  for (int64_t v = 0; v < nnz; ++v) g.slot_ptr[v + 1] += g.slot_ptr[v];
+  g.slot_src.resize(size_t(ncontrib));
+  {
+    std::vector<int64_t> f(g.slot_ptr.begin(), g.slot_ptr.end() - 1);
+    for (int64_t c = 0; c < ncontrib; ++c) g.slot_src[size_t(f[slot_of[size_t(c)]]++)] = uint32_t(c);
+  }
+  e = femdev::upload_generic(m->dev, g);
+  if (!e.empty()) return e;
+  m->path = "generic";
+  m->pattern_valid = true;
+  return "";
+}
+
+// upload + device setup + pattern, as needed
+int ensure_ready(fem_model* m) {
+  if (!m->dev) {
+    std::string err;
+    m->dev = femdev::create_device(err);
+    if (!m->dev) return fail(m, FEM_ERR_DEVICE, err);
+  }
+  if (m->dirty) {
+    auto e = femdev::upload_nodes(m->dev, m->xyz.data(), m->node_number.size());
+    for (int t = 0; t < 3 && e.empty(); ++t) {
+      e = femdev::upload_elements(m->dev, t, m->set[t].size(), m->set[t].conn.data(), m->set[t].params.data());
+      if (e.empty()) e = femdev::run_setup(m->dev, t, m->set[t].size(), m->rel_tol, m->abs_tol, m->rzero[t]);
+    }
+    if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+    m->dirty = false;
+    m->pattern_valid = false;
+  }
+  if (!m->pattern_valid) {
+    auto e = build_pattern(m);
+    if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  }
+  return FEM_OK;
+}
+
+}  // namespace
+
+// =========================================================================== C ABI
+extern "C" {
+
+int fem_create(double rel_tol, double abs_tol, uint32_t nodes_number, fem_model** out) {
+  if (!out) return FEM_ERR_ARGUMENT;
+  auto* m = new fem_model();
+  m->rel_tol = rel_tol;
+  m->abs_tol = abs_tol;
+  clear_model(m, nodes_number);
+  *out = m;
+  return FEM_OK;
+}
+
+int fem_reset(fem_model* m, uint32_t nodes_number) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  clear_model(m, nodes_number);
+  return FEM_OK;
+}
+
+void fem_destroy(fem_model* m) {
+  if (!m) return;
+  femdev::destroy_device(m->dev);
+  delete m;
+}
+
+const char* fem_last_error(const fem_model* m) { return m ? m->err.c_str() : "null handle"; }
+
+int fem_add_nodes(fem_model* m, size_t n, const uint32_t* numbers, const double* xyz, size_t* n_ok) {
+  if (!m || (n && (!numbers || !xyz))) return FEM_ERR_ARGUMENT;
+  size_t i = 0;
+  int rc = FEM_OK;
+  for (; i < n; ++i) {
+    uint32_t number = numbers[i];
+    double x = xyz[3 * i], y = xyz[3 * i + 1], z = xyz[3 * i + 2];
+    size_t index = m->node_number.size();
+    // methods_for_node_data_handle.rs:39-64
+    if (m->nodes_number == 0 || index > size_t(m->nodes_number) - 1) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Nodes number could not be greater than " + u(m->nodes_number) + "!");
+      break;
+    }
+    if (m->node_idx.count(number)) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Node with number " + u(number) + " already exists!");
+      break;
+    }
+    uint64_t key = coord_key(x, y, z);
+    bool dup = false;
+    auto it = m->coord_hash.find(key);
+    if (it != m->coord_hash.end())
+      for (uint32_t o : it->second) {
+        const double* c = node_xyz(m, o);
+        if (c[0] == x && c[1] == y && c[2] == z) dup = true;
+      }
+    if (dup) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Node with coordinates x: " + rust_debug(x) + ", y: " + rust_debug(y) +
+                                           ", z: " + rust_debug(z) + " already exists!");
+      break;
+    }
+    m->node_idx[number] = uint32_t(index);
+    m->node_number.push_back(number);
+    m->xyz.insert(m->xyz.end(), {x, y, z});
+    m->coord_hash[key].push_back(uint32_t(index));
+    m->node_plates.emplace_back();
+    m->dirty = true;
+  }
+  if (n_ok) *n_ok = i;
+  return rc;
+}
+
+int fem_add_trusses(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* E,
+                    const double* A, const double* A2, size_t* n_ok) {
+  if (!m || (n && (!numbers || !conn || !E || !A))) return FEM_ERR_ARGUMENT;
+  size_t i = 0;
+  int rc = FEM_OK;
+  auto& S = m->set[TRUSS];
+  for (; i < n; ++i) {
+    uint32_t number = numbers[i], n1 = conn[2 * i], n2 = conn[2 * i + 1];
+    double a2 = A2 ? A2[i] : NAN;
+    bool has_a2 = !std::isnan(a2);
+    // methods_for_truss_data_handle.rs:49-67, truss.rs:44-64
+    if (!node_exists(m, n1) || !node_exists(m, n2)) { rc = FEM_ERR_VALIDATION; break; }
+    if (S.by_number.count(number)) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Truss element with number " + u(number) + " already exists!");
+      break;
+    }
+    if (m->truss_pairs.count(pair_key(n1, n2))) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Truss element with node number " + u(n1) + " and " + u(n2) + " already exists!");
+      break;
+    }
+    if (!positive_or(m, E[i], "Young's modulus", E[i]) || !positive_or(m, A[i], "Area", A[i]) ||
+        (has_a2 && !positive_or(m, a2, "Area2", a2))) {
+      rc = FEM_ERR_VALIDATION;
+      break;
+    }
+    uint32_t i1 = m->node_idx[n1], i2 = m->node_idx[n2];
+    const double *p1 = node_xyz(m, i1), *p2 = node_xyz(m, i2);
+    if (p1[0] == p2[0] && p1[1] == p2[1] && p1[2] == p2[2]) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Zero length vector could not be aligned!");
+      break;
+    }
+    S.by_number[number] = uint32_t(S.size());
+    S.number.push_back(number);
+    S.conn.insert(S.conn.end(), {i1, i2});
+    S.params.insert(S.params.end(), {E[i], A[i], has_a2 ? a2 : NAN});
+    S.seq.push_back(m->seq++);
+    m->truss_pairs[pair_key(n1, n2)] = number;
+    m->dirty = true;
+  }
+  if (n_ok) *n_ok = i;
+  return rc;
+}
+
+int fem_add_beams(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* props,
+                  size_t* n_ok) {
+  if (!m || (n && (!numbers || !conn || !props))) return FEM_ERR_ARGUMENT;
+  size_t i = 0;
+  int rc = FEM_OK;
+  auto& S = m->set[BEAM];
+  for (; i < n; ++i) {
+    uint32_t number = numbers[i], n1 = conn[2 * i], n2 = conn[2 * i + 1];
+    const double* p = props + 11 * i;
+    // methods_for_beam_data_handle.rs:49-68, beam.rs:63-117
+    if (!node_exists(m, n1) || !node_exists(m, n2)) { rc = FEM_ERR_VALIDATION; break; }
+    if (S.by_number.count(number)) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Beam element with number " + u(number) + " already exists!");
+      break;
+    }
+    if (m->beam_pairs.count(pair_key(n1, n2))) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Beam element with node number " + u(n1) + " and " + u(n2) + " already exists!");
+      break;
+    }
+    if (!positive_or(m, p[0], "Young's modulus", p[0]) || !positive_or(m, p[1], "Poisson's ratio", p[0]) ||
+        !positive_or(m, p[2], "Area", p[2]) || !positive_or(m, p[3], "I11", p[3]) ||
+        !positive_or(m, p[4], "I22", p[4]) || !positive_or(m, p[6], "It", p[6]) ||
+        !positive_or(m, p[7], "Shear factor", p[7])) {
+      rc = FEM_ERR_VALIDATION;
+      break;
+    }
+    uint32_t i1 = m->node_idx[n1], i2 = m->node_idx[n2];
+    const double *a = node_xyz(m, i1), *b = node_xyz(m, i2);
+    double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    double f = ((p[8] * d[0] + p[9] * d[1]) + p[10] * d[2]) / ((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]);
+    double pp[3] = {p[8] - f * d[0], p[9] - f * d[1], p[10] - f * d[2]};
+    if (fem::norm3(pp[0], pp[1], pp[2]) == 0.0) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Local axis 1 direction [" + rust_debug(p[8]) + ", " + rust_debug(p[9]) + ", " +
+                                           rust_debug(p[10]) + "] parallel to element " + u(number) + "!");
+      break;
+    }
+    if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Zero length vector could not be aligned!");
+      break;
+    }
+    S.by_number[number] = uint32_t(S.size());
+    S.number.push_back(number);
+    S.conn.insert(S.conn.end(), {i1, i2});
+    S.params.insert(S.params.end(), p, p + 11);
+    S.seq.push_back(m->seq++);
+    m->beam_pairs[pair_key(n1, n2)] = number;
+    m->dirty = true;
+  }
+  if (n_ok) *n_ok = i;
+  return rc;
+}
+
+int fem_add_plates(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* props,
+                   size_t* n_ok) {
+  if (!m || (n && (!numbers || !conn || !props))) return FEM_ERR_ARGUMENT;
+  size_t i = 0;
+  int rc = FEM_OK;
+  auto& S = m->set[PLATE];
+  for (; i < n; ++i) {
+    uint32_t number = numbers[i];
+    const uint32_t* nn = conn + 4 * i;
+    const double* p = props + 4 * i;
+    // methods_for_plate_data_handle.rs:62-102, plate.rs:58-158
+    bool ok = true;
+    for (int k = 0; k < 4 && ok; ++k) ok = node_exists(m, nn[k]);
+    if (!ok) { rc = FEM_ERR_VALIDATION; break; }
+    if (S.by_number.count(number)) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Plate element with number " + u(number) + " already exists!");
+      break;
+    }
+    uint32_t idx[4];
+    for (int k = 0; k < 4; ++k) idx[k] = m->node_idx[nn[k]];
+    bool same = false;
+    for (uint32_t cand : m->node_plates[idx[0]]) {
+      const uint32_t* c = &S.conn[4 * size_t(cand)];
+      bool all = true;
+      for (int k = 0; k < 4 && all; ++k) all = idx[k] == c[0] || idx[k] == c[1] || idx[k] == c[2] || idx[k] == c[3];
+      if (all) { same = true; break; }
+    }
+    if (same) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Plate element with nodes numbers [" + u(nn[0]) + ", " + u(nn[1]) + ", " +
+                                           u(nn[2]) + ", " + u(nn[3]) + "] already exists!");
+      break;
+    }
+    if (!positive_or(m, p[0], "Young's modulus", p[0]) || !positive_or(m, p[1], "Poisson's ratio", p[0]) ||
+        !positive_or(m, p[2], "Thickness", p[0]) || !positive_or(m, p[3], "Shear factor", p[3])) {
+      rc = FEM_ERR_VALIDATION;
+      break;
+    }
+    const double* pts[4] = {node_xyz(m, idx[0]), node_xyz(m, idx[1]), node_xyz(m, idx[2]), node_xyz(m, idx[3])};
+    if (quad_on_line(pts)) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Some nodes of " + u(number) + " element lie on the line!");
+      break;
+    }
+    if (!quad_on_plane(pts, m->abs_tol)) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Not all nodes of element " + u(number) + " lie on the plane!");
+      break;
+    }
+    int hull = quad_hull_points(nn, pts, m->abs_tol);
+    if (hull < 0) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Zero length vector could not be aligned!");
+      break;
+    }
+    if (hull != 4) {
+      rc = fail(m, FEM_ERR_VALIDATION, "Element " + u(number) + " non-convex!");
+      break;
+    }
+    uint32_t pos = uint32_t(S.size());
+    S.by_number[number] = pos;
+    S.number.push_back(number);
+    S.conn.insert(S.conn.end(), idx, idx + 4);
+    S.params.insert(S.params.end(), p, p + 4);
+    S.seq.push_back(m->seq++);
+    for (int k = 0; k < 4; ++k)
+      if (k == 0 || std::find(idx, idx + k, idx[k]) == idx + k) m->node_plates[idx[k]].push_back(pos);
+    m->dirty = true;
+  }
+  if (n_ok) *n_ok = i;
+  return rc;
+}
+
+int fem_get_rotation(fem_model* m, int type, uint32_t number, double out[9]) {
+  if (!m || !out || type < 0 || type > 2) return FEM_ERR_ARGUMENT;
+  static const char* names[3] = {"Truss", "Beam", "Plate"};
+  auto it = m->set[type].by_number.find(number);
+  if (it == m->set[type].by_number.end())
+    return fail(m, FEM_ERR_VALIDATION, std::string(names[type]) + " element with number " + u(number) + " does not exist!");
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  auto e = femdev::get_rotation(m->dev, type, it->second, out);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+int fem_build_pattern(fem_model* m) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  return ensure_ready(m);
+}
+
+int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
+  if (!m || (!val_device && m->nnz)) return FEM_ERR_ARGUMENT;
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
+                                           : femdev::assemble_generic(m->dev, val_device, stream);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+int fem_assemble(fem_model* m, void* stream) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  return fem_assemble_into(m, femdev::values(m->dev), stream);
+}
+
+int fem_csr_info(fem_model* m, int64_t* n_rows, int64_t* nnz) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->pattern_valid || m->dirty) return fail(m, FEM_ERR_STATE, "the pattern is not built: call fem_assemble");
+  if (n_rows) *n_rows = int64_t(m->nodes_number) * 6;
+  if (nnz) *nnz = m->nnz;
+  return FEM_OK;
+}
+
+int fem_csr_device(fem_model* m, const int64_t** rp, const int32_t** ci, const double** v) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->pattern_valid || m->dirty) return fail(m, FEM_ERR_STATE, "the pattern is not built: call fem_assemble");
+  if (rp) *rp = femdev::row_ptr(m->dev);
+  if (ci) *ci = femdev::col_idx(m->dev);
+  if (v) *v = femdev::values(m->dev);
+  return FEM_OK;
+}
+
+int fem_csr_to_host(fem_model* m, int64_t* rp, int32_t* ci, double* v) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->pattern_valid || m->dirty) return fail(m, FEM_ERR_STATE, "the pattern is not built: call fem_assemble");
+  auto e = femdev::copy_csr_to_host(m->dev, int64_t(m->nodes_number) * 6, m->nnz, rp, ci, v);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+double fem_last_assembly_ms(fem_model* m) { return m && m->dev ? femdev::last_ms(m->dev) : -1.0; }
+
+const char* fem_assembly_path(fem_model* m) { return m ? m->path.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,117 @@
+/* fem_amd.h -- C ABI of the MI355X (gfx950) element-stiffness and global-assembly
+ * library, a drop-in for the assembly path of the Rust crate
+ * `finite_element_method` (0.9.12).
+ *
+ * Conventions
+ *   - Opaque handle, `int` status: 0 = FEM_OK.  On failure the handle owns a
+ *     message readable with fem_last_error(); validation messages are the
+ *     crate's own strings.
+ *   - Batched adds take SoA arrays and commit a prefix: elements before the
+ *     first failing one are added, `*n_ok` receives how many.  This is the
+ *     crate's per-call Result semantics applied element by element.
+ *   - Assembly is deferred: the crate assembles eagerly inside add_*, but its
+ *     global matrix is crate-private (src/fem/fem.rs:170-176) and is only seen
+ *     through separate_stiffness_matrix_*, so deferring is unobservable.
+ *   - The global matrix is 6N x 6N (N = nodes_number, dof = 6 * node_index + k,
+ *     k in X, Y, Z, ThX, ThY, ThZ), stored as CSR: int64 row_ptr, int32
+ *     col_idx, double values, rows and columns ascending.
+ *   - Streams are passed as `void*` (a hipStream_t; NULL = default stream).
+ *   - One handle per thread; a handle is not thread-safe.
+ */
+#ifndef FEM_AMD_H
+#define FEM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fem_model fem_model;
+
+enum fem_status {
+  FEM_OK = 0,
+  FEM_ERR_VALIDATION = 1, /* the crate would return Err(String) */
+  FEM_ERR_STATE = 2,      /* call not valid in the handle's current state */
+  FEM_ERR_DEVICE = 3,     /* a HIP call failed or no device is available */
+  FEM_ERR_ARGUMENT = 4    /* null pointer, size mismatch, limits */
+};
+
+enum fem_element_type { FEM_TRUSS = 0, FEM_BEAM = 1, FEM_PLATE = 2 };
+
+/* ---- model life cycle ------------------------------------------------- */
+
+/* FEM::<V>::create(rel_tol, abs_tol, nodes_number)      src/fem/fem.rs:34-65 */
+int fem_create(double rel_tol, double abs_tol, uint32_t nodes_number, fem_model** out);
+/* FEM::reset(nodes_number)                               src/fem/fem.rs:155-169 */
+int fem_reset(fem_model* m, uint32_t nodes_number);
+void fem_destroy(fem_model* m);
+/* message of the last failing call (owned by the handle, valid until the next call) */
+const char* fem_last_error(const fem_model* m);
+
+/* ---- nodes and elements ----------------------------------------------- */
+
+/* FEM::add_node(number, x, y, z), n nodes            methods_for_node_data_handle.rs:66-79
+ * xyz: n x 3 row-major */
+int fem_add_nodes(fem_model* m, size_t n, const uint32_t* numbers, const double* xyz, size_t* n_ok);
+
+/* FEM::add_truss(number, n1, n2, E, A, Option<A2>)     methods_for_truss_data_handle.rs:49-128
+ * conn: n x 2 node numbers; A2[i] = NaN means None; A2 may be NULL (all None) */
+int fem_add_trusses(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* E,
+                    const double* A, const double* A2, size_t* n_ok);
+
+/* FEM::add_beam(number, n1, n2, E, nu, A, I11, I22, I12, It, shear_factor,
+ *               local_axis_1_direction)                 methods_for_beam_data_handle.rs:49-142
+ * props: n x 11 row-major = E, nu, A, I11, I22, I12, It, shear_factor, v1x, v1y, v1z */
+int fem_add_beams(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* props,
+                  size_t* n_ok);
+
+/* FEM::add_plate(number, n1, n2, n3, n4, E, nu, thickness, shear_factor)
+ *                                                        methods_for_plate_data_handle.rs:62-217
+ * conn: n x 4 node numbers; props: n x 4 = E, nu, thickness, shear_factor */
+int fem_add_plates(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* props,
+                   size_t* n_ok);
+
+/* FEM::get_{truss,beam,plate}_rotation_matrix_elements(number) -> [V; 9]
+ *                                                        src/fem/fem.rs:171-202
+ * Computed on the device; row-major. */
+int fem_get_rotation(fem_model* m, int type, uint32_t number, double out[9]);
+
+/* ---- assembly ---------------------------------------------------------- */
+
+/* Computes the global stiffness on the device (replaces the eager scatter of
+ * add_truss/add_beam/add_plate, methods_for_*_data_handle.rs).  The first call
+ * after a model change builds the sparsity pattern and the element-to-slot
+ * maps; later calls only recompute values.  Asynchronous on `stream`. */
+int fem_assemble(fem_model* m, void* stream);
+
+/* Same, writing the values into caller-owned device memory of nnz doubles
+ * (the pattern must exist: call fem_assemble or fem_build_pattern first). */
+int fem_assemble_into(fem_model* m, double* val_device, void* stream);
+
+/* Builds the pattern only (synchronous). */
+int fem_build_pattern(fem_model* m);
+
+/* Pattern size: n_rows = 6 * nodes_number. */
+int fem_csr_info(fem_model* m, int64_t* n_rows, int64_t* nnz);
+
+/* Borrowed device pointers to the library's CSR (valid until the next model
+ * change, reset or destroy). */
+int fem_csr_device(fem_model* m, const int64_t** row_ptr, const int32_t** col_idx, const double** val);
+
+/* Synchronous copy of the CSR to host buffers (any pointer may be NULL). */
+int fem_csr_to_host(fem_model* m, int64_t* row_ptr, int32_t* col_idx, double* val);
+
+/* Timing of the last fem_assemble / fem_assemble_into on its stream, from
+ * HIP events around the value kernels only (ms); -1 when unavailable. */
+double fem_last_assembly_ms(fem_model* m);
+
+/* Name of the value path used by the last assembly ("plate_fused" or "generic"). */
+const char* fem_assembly_path(fem_model* m);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FEM_AMD_H */

@@ -1,0 +1,180 @@
+"""ctypes wrapper of the CPU oracle (oracle/fem_oracle.cpp).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, never by the product package.  It restates the
+crate `finite_element_method`'s element-stiffness and assembly path; see the
+header of fem_oracle.cpp for what each function follows.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "libfem_oracle.so")
+_lib = None
+
+
+class OracleError(Exception):
+    """Raised with the crate-identical message when the oracle rejects an input."""
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        vp, i64, u32, dbl = ctypes.c_void_p, ctypes.c_int64, ctypes.c_uint32, ctypes.c_double
+        L.om_create.restype = vp
+        L.om_create.argtypes = [dbl, dbl, u32, ctypes.c_int]
+        L.om_destroy.argtypes = [vp]
+        L.om_last_error.restype = ctypes.c_char_p
+        L.om_last_error.argtypes = [vp]
+        for name, nargs in [("om_add_nodes", 3), ("om_add_trusses", 6), ("om_add_beams", 4), ("om_add_plates", 4)]:
+            f = getattr(L, name)
+            f.restype = i64
+            f.argtypes = [vp, i64] + [vp] * (nargs - 1)
+        L.om_get_rotation.restype = ctypes.c_int
+        L.om_get_rotation.argtypes = [vp, ctypes.c_int, u32, vp]
+        L.om_element_matrix.restype = ctypes.c_int
+        L.om_element_matrix.argtypes = [vp, ctypes.c_int, u32, ctypes.c_int, vp]
+        L.om_principal_moments.argtypes = [dbl, dbl, dbl, dbl, vp]
+        L.om_rust_debug.restype = ctypes.c_int
+        L.om_rust_debug.argtypes = [dbl, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
+        L.om_export.restype = i64
+        L.om_export.argtypes = [vp, ctypes.c_int]
+        L.om_ndof.restype = i64
+        L.om_ndof.argtypes = [vp]
+        L.om_export_copy.argtypes = [vp, vp, vp, vp]
+        L.om_num_contributions.restype = i64
+        L.om_num_contributions.argtypes = [vp]
+        L.om_structural_violations.restype = i64
+        L.om_structural_violations.argtypes = [vp]
+        L.om_nonstructural_max_rel.restype = ctypes.c_double
+        L.om_nonstructural_max_rel.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def rust_debug(x, f32=False):
+    buf = ctypes.create_string_buffer(64)
+    lib().om_rust_debug(float(x), int(f32), buf, 64)
+    return buf.value.decode()
+
+
+def principal_moments(i11, i22, i12, rel_tol):
+    out = np.zeros(3)
+    lib().om_principal_moments(i11, i22, i12, rel_tol, _ptr(out))
+    return out
+
+
+class OracleModel:
+    """The crate's FEM<V> (src/fem/fem.rs:15-65) restated on the CPU."""
+
+    TRUSS, BEAM, PLATE = 0, 1, 2
+
+    def __init__(self, rel_tol, abs_tol, nodes_number, f32=False):
+        self._h = lib().om_create(rel_tol, abs_tol, nodes_number, int(f32))
+        self.nodes_number = nodes_number
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().om_destroy(self._h)
+            self._h = None
+
+    def _err(self):
+        return lib().om_last_error(self._h).decode()
+
+    def _check(self, n_ok, n):
+        if n_ok != n:
+            raise OracleError(self._err())
+        return n_ok
+
+    # --- batched SoA adds: prefix-commit, raise on the first failing element
+    def add_nodes(self, numbers, xyz):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        return self._check(lib().om_add_nodes(self._h, len(numbers), _ptr(numbers), _ptr(xyz)), len(numbers))
+
+    def add_trusses(self, numbers, conn, E, A, A2=None):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        n = len(numbers)
+        conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(n, 2)
+        E = np.ascontiguousarray(np.broadcast_to(E, (n,)), dtype=np.float64)
+        A = np.ascontiguousarray(np.broadcast_to(A, (n,)), dtype=np.float64)
+        A2 = np.full(n, np.nan) if A2 is None else np.ascontiguousarray(np.broadcast_to(A2, (n,)), dtype=np.float64)
+        return self._check(lib().om_add_trusses(self._h, n, _ptr(numbers), _ptr(conn), _ptr(E), _ptr(A), _ptr(A2)), n)
+
+    def add_beams(self, numbers, conn, props):
+        """props: (n, 11) = E, nu, A, I11, I22, I12, It, shear_factor, v1x, v1y, v1z."""
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        n = len(numbers)
+        conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(n, 2)
+        props = np.ascontiguousarray(props, dtype=np.float64).reshape(n, 11)
+        return self._check(lib().om_add_beams(self._h, n, _ptr(numbers), _ptr(conn), _ptr(props)), n)
+
+    def add_plates(self, numbers, conn, props):
+        """props: (n, 4) = E, nu, thickness, shear_factor."""
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        n = len(numbers)
+        conn = np.ascontiguousarray(conn, dtype=np.uint32).reshape(n, 4)
+        props = np.ascontiguousarray(props, dtype=np.float64).reshape(n, 4)
+        return self._check(lib().om_add_plates(self._h, n, _ptr(numbers), _ptr(conn), _ptr(props)), n)
+
+    # --- the crate's one-at-a-time API
+    def add_node(self, number, x, y, z):
+        self.add_nodes([number], [[x, y, z]])
+
+    def add_truss(self, number, n1, n2, E, A, A2=None):
+        self.add_trusses([number], [[n1, n2]], [E], [A], None if A2 is None else [A2])
+
+    def add_beam(self, number, n1, n2, E, nu, A, i11, i22, i12, it, k, v1):
+        self.add_beams([number], [[n1, n2]], [[E, nu, A, i11, i22, i12, it, k, *v1]])
+
+    def add_plate(self, number, n1, n2, n3, n4, E, nu, t, k):
+        self.add_plates([number], [[n1, n2, n3, n4]], [[E, nu, t, k]])
+
+    def rotation(self, etype, number):
+        out = np.zeros(9)
+        if lib().om_get_rotation(self._h, etype, number, _ptr(out)) != 0:
+            raise OracleError(self._err())
+        return out
+
+    def element_matrix(self, etype, number, global_frame=True):
+        out = np.zeros(24 * 24)
+        n = lib().om_element_matrix(self._h, etype, number, int(global_frame), _ptr(out))
+        if n < 0:
+            raise OracleError("element %d of type %d does not exist" % (number, etype))
+        return out[: n * n].reshape(n, n)
+
+    def csr(self, structural=True):
+        """Global 6N x 6N stiffness as CSR (row_ptr int64, col_idx int32, val f64).
+
+        structural=False gives the crate's value-based matrix (exact zeros dropped)."""
+        nnz = lib().om_export(self._h, 1 if structural else 0)
+        ndof = lib().om_ndof(self._h)
+        row_ptr = np.zeros(ndof + 1, dtype=np.int64)
+        col = np.zeros(nnz, dtype=np.int32)
+        val = np.zeros(nnz, dtype=np.float64)
+        lib().om_export_copy(self._h, _ptr(row_ptr), _ptr(col), _ptr(val))
+        return row_ptr, col, val
+
+    def structural_violations(self):
+        return lib().om_structural_violations(self._h)
+
+    def nonstructural_max_rel(self):
+        return lib().om_nonstructural_max_rel(self._h)
+
+    def num_contributions(self):
+        return lib().om_num_contributions(self._h)
