@@ -1,0 +1,65 @@
+// device_state.hpp -- device-side state of one model, shared by the .hip files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+
+namespace femdev {
+
+struct Device {
+  hipStream_t stream = nullptr;
+  int dev = 0;
+  double* xyz = nullptr;
+  size_t n_nodes = 0;
+  uint32_t* conn[3] = {nullptr, nullptr, nullptr};
+  double* params[3] = {nullptr, nullptr, nullptr};
+  double* R[3] = {nullptr, nullptr, nullptr};
+  double* derived = nullptr;  // beams: i11p, i22p
+  uint16_t* rzero[3] = {nullptr, nullptr, nullptr};
+  size_t n_elem[3] = {0, 0, 0};
+  // CSR
+  int64_t* row_ptr = nullptr;
+  int32_t* col_idx = nullptr;
+  double* val = nullptr;
+  int64_t n_rows = 0, nnz = 0;
+  // generic maps
+  uint64_t* class_mask = nullptr;
+  uint32_t* class_off = nullptr;
+  uint16_t* eclass[3] = {nullptr, nullptr, nullptr};
+  int64_t* ebase[3] = {nullptr, nullptr, nullptr};
+  double* contrib = nullptr;
+  int64_t n_contrib = 0;
+  int64_t* slot_ptr = nullptr;
+  uint32_t* slot_src = nullptr;
+  // plate fused maps
+  int n_patches = 0;
+  uint32_t *pf_node_ptr = nullptr, *pf_node = nullptr, *pf_info = nullptr, *pf_blk_ptr = nullptr, *pf_blk = nullptr,
+           *pf_pair_ptr = nullptr, *pf_pairs = nullptr;
+  // timing
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+};
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess) return std::string(#expr) + ": " + hipGetErrorString(_e);  \
+  } while (0)
+
+template <class T>
+inline std::string dalloc(T*& p, size_t n) {
+  if (p) { hipFree(p); p = nullptr; }
+  if (n == 0) return "";
+  HIP_TRY(hipMalloc(&p, n * sizeof(T)));
+  return "";
+}
+template <class T>
+inline std::string dupload(T*& p, const T* src, size_t n) {
+  auto e = dalloc(p, n);
+  if (!e.empty()) return e;
+  if (n) HIP_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  return "";
+}
+
+}  // namespace femdev

@@ -241,6 +241,54 @@ __host__ __device__ inline void plate_block_add_ip(const PlateIP& g, const Plate
     }
 }
 
+__host__ __device__ inline double sel4(double a, double b, double c, double d, int p) {
+  return p == 0 ? a : (p == 1 ? b : (p == 2 ? c : d));
+}
+
+// Local-frame row block of node p (columns of all four nodes q) of a flat
+// plate, accumulated over one integration point.  Layout: block q at 13 q,
+// entries (u,u) (u,v) | (v,u) (v,v) | (w,w) (w,tx) (w,ty) | (tx,w) (tx,tx)
+// (tx,ty) | (ty,w) (ty,tx) (ty,ty).  The drilling entry (tz,tz) = KROT6 is
+// handled by the caller.  Same per-entry arithmetic as plate_block_add_ip.
+__host__ __device__ inline void plate_row_add_ip(const PlateIP& g, const PlateMat& m, int p, double acc[52]) {
+  const double w = g.w;
+  const double dxp = sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p);
+  const double dyp = sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
+  double srp[3], ssp[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    srp[i] = sel4(g.sr[0][i], g.sr[1][i], g.sr[2][i], g.sr[3][i], p);
+    ssp[i] = sel4(g.ss[0][i], g.ss[1][i], g.ss[2][i], g.ss[3][i], p);
+  }
+  const double cs = m.cs;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double* o = acc + 13 * q;
+    const double dxq = g.dx[q], dyq = g.dy[q];
+    double muu = m.cm00 * dxp * dxq + m.cm22 * dyp * dyq;
+    double muv = m.cm01 * dxp * dyq + m.cm22 * dyp * dxq;
+    double mvu = m.cm01 * dyp * dxq + m.cm22 * dxp * dyq;
+    double mvv = m.cm00 * dyp * dyq + m.cm22 * dxp * dxq;
+    double bxx = m.cb00 * dyp * dyq + m.cb22 * dxp * dxq;
+    double bxy = -(m.cb01 * dyp * dxq + m.cb22 * dxp * dyq);
+    double byx = -(m.cb01 * dxp * dyq + m.cb22 * dyp * dxq);
+    double byy = m.cb00 * dxp * dxq + m.cb22 * dyp * dyq;
+    o[0] += muu * w; o[1] += muv * w; o[2] += mvu * w; o[3] += mvv * w;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) {
+        double sh = cs * (srp[i] * g.sr[q][j] + ssp[i] * g.ss[q][j]);
+        double be = 0.0;
+        if (i == 1 && j == 1) be = bxx;
+        else if (i == 1 && j == 2) be = bxy;
+        else if (i == 2 && j == 1) be = byx;
+        else if (i == 2 && j == 2) be = byy;
+        o[4 + 3 * i + j] += be * w + sh * w;
+      }
+  }
+}
+
 // Gauss abscissa of the 2x2 rule in the crate's point order (g,g), (-g,g), (-g,-g), (g,-g)
 __host__ __device__ inline double plate_gauss() { return sqrt(double(ONE_THIRD_F32)); }
 

@@ -21,67 +21,13 @@
 #include <string>
 #include <vector>
 
+#include "device_state.hpp"
 #include "element_math.hpp"
 #include "fem_device.hpp"
 
 namespace femdev {
 
 using fem::NODE_DOF;
-
-struct Device {
-  hipStream_t stream = nullptr;
-  int dev = 0;
-  double* xyz = nullptr;
-  size_t n_nodes = 0;
-  uint32_t* conn[3] = {nullptr, nullptr, nullptr};
-  double* params[3] = {nullptr, nullptr, nullptr};
-  double* R[3] = {nullptr, nullptr, nullptr};
-  double* derived = nullptr;  // beams: i11p, i22p
-  uint16_t* rzero[3] = {nullptr, nullptr, nullptr};
-  size_t n_elem[3] = {0, 0, 0};
-  // CSR
-  int64_t* row_ptr = nullptr;
-  int32_t* col_idx = nullptr;
-  double* val = nullptr;
-  int64_t n_rows = 0, nnz = 0;
-  // generic maps
-  uint64_t* class_mask = nullptr;
-  uint32_t* class_off = nullptr;
-  uint16_t* eclass[3] = {nullptr, nullptr, nullptr};
-  int64_t* ebase[3] = {nullptr, nullptr, nullptr};
-  double* contrib = nullptr;
-  int64_t n_contrib = 0;
-  int64_t* slot_ptr = nullptr;
-  uint32_t* slot_src = nullptr;
-  // plate fused maps
-  int n_patches = 0, max_patch_elems = 0;
-  uint32_t *pe_ptr = nullptr, *pe = nullptr, *pn_ptr = nullptr, *pn = nullptr, *nb_ptr = nullptr, *binfo = nullptr;
-  uint16_t* bcontrib = nullptr;
-  // timing
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
-};
-
-#define HIP_TRY(expr)                                                                 \
-  do {                                                                                \
-    hipError_t _e = (expr);                                                           \
-    if (_e != hipSuccess) return std::string(#expr) + ": " + hipGetErrorString(_e);  \
-  } while (0)
-
-template <class T>
-static std::string dalloc(T*& p, size_t n) {
-  if (p) { hipFree(p); p = nullptr; }
-  if (n == 0) return "";
-  HIP_TRY(hipMalloc(&p, n * sizeof(T)));
-  return "";
-}
-template <class T>
-static std::string dupload(T*& p, const T* src, size_t n) {
-  auto e = dalloc(p, n);
-  if (!e.empty()) return e;
-  if (n) HIP_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
-  return "";
-}
 
 Device* create_device(std::string& err) {
   int count = 0;
@@ -105,8 +51,8 @@ void destroy_device(Device* d) {
   f(d->xyz);
   for (int t = 0; t < 3; ++t) { f(d->conn[t]); f(d->params[t]); f(d->R[t]); f(d->rzero[t]); f(d->eclass[t]); f(d->ebase[t]); }
   f(d->derived); f(d->row_ptr); f(d->col_idx); f(d->val); f(d->class_mask); f(d->class_off); f(d->contrib);
-  f(d->slot_ptr); f(d->slot_src); f(d->pe_ptr); f(d->pe); f(d->pn_ptr); f(d->pn); f(d->nb_ptr); f(d->binfo);
-  f(d->bcontrib);
+  f(d->slot_ptr); f(d->slot_src); f(d->pf_node_ptr); f(d->pf_node); f(d->pf_info); f(d->pf_blk_ptr); f(d->pf_blk);
+  f(d->pf_pair_ptr); f(d->pf_pairs);
   if (d->ev0) hipEventDestroy(d->ev0);
   if (d->ev1) hipEventDestroy(d->ev1);
   delete d;
@@ -180,7 +126,9 @@ __global__ void plate_setup_kernel(const uint32_t* __restrict__ conn, const doub
   bool ok = fem::quad_rotation(p2, p3, p4, abs_tol, R);
   if (!ok) for (int k = 0; k < 9; ++k) R[k] = 0.0;
   for (int k = 0; k < 9; ++k) Rout[9 * e + k] = R[k];
-  rz[e] = ok ? zero_bits(R) : uint16_t(0x8000);
+  bool ident = R[0] == 1.0 && R[4] == 1.0 && R[8] == 1.0 && R[1] == 0.0 && R[2] == 0.0 && R[3] == 0.0 &&
+               R[5] == 0.0 && R[6] == 0.0 && R[7] == 0.0;
+  rz[e] = ok ? uint16_t(zero_bits(R) | (ident ? 0x4000 : 0)) : uint16_t(0x8000);
 }
 
 std::string run_setup(Device* d, int type, size_t n, double rel_tol, double abs_tol, std::vector<uint16_t>& rzero) {
@@ -409,5 +357,4 @@ std::string copy_csr_to_host(Device* d, int64_t n_rows, int64_t nnz, int64_t* rp
   return "";
 }
 
-// plate fused path: defined in plate_fused.hip
 }  // namespace femdev

@@ -51,16 +51,22 @@ struct GenericMaps {
 };
 std::string upload_generic(Device* d, const GenericMaps& g);
 
-// Plate fused path: per patch (one workgroup) the element list, owned nodes
-// and gather lists; see fem_device.hip for the layout.
+// Plate fused path (flat plates whose frame is exactly the identity): one
+// workgroup per patch of owned nodes.  For every owned node, its elements
+// (insertion order = rank r) and its row blocks; see plate_fused.hip.
+constexpr int PF_MAX_NODES = 32;   // owned nodes per patch
+constexpr int PF_MAX_PAIRS = 128;  // (owned node, element) pairs per patch = threads per workgroup
+constexpr int PF_MAX_DEG = 8;      // elements per node
+constexpr int PF_MAX_CONTRIB = 5;  // elements sharing one off-diagonal node pair
 struct PlateMaps {
   int n_patches = 0;
-  std::vector<uint32_t> patch_elem_ptr, patch_elems;    // elements touching the patch's nodes
-  std::vector<uint32_t> patch_node_ptr, patch_nodes;    // owned nodes (ascending)
-  std::vector<uint32_t> node_blk_ptr;                   // per owned node (flattened), block range
-  std::vector<uint32_t> blk_info;                       // per block: contributor count | diag flag
-  std::vector<uint16_t> blk_contrib;                    // per block: up to 4 (local element << 4 | p << 2 | q)
-  int max_patch_elems = 0;
+  std::vector<uint32_t> patch_node_ptr;  // n_patches + 1, into own_*
+  std::vector<uint32_t> own_node;        // owned node index
+  std::vector<uint32_t> own_info;        // deg | nb << 8 | kself << 16 | slot_base << 24
+  std::vector<uint32_t> own_blk_ptr;     // n_own + 1, into blk
+  std::vector<uint32_t> blk;             // per node-pair block: count | (r:3, q:2) << (3 + 5 m)
+  std::vector<uint32_t> patch_pair_ptr;  // n_patches + 1, into pairs
+  std::vector<uint32_t> pairs;           // element | p << 30 (p = local index of the owned node)
 };
 std::string upload_plate_maps(Device* d, const PlateMaps& pm);
 

@@ -288,6 +288,133 @@ struct PairRef {
   Mask mask;
 };
 
+// -------------------------------------------------------- plate fused maps
+constexpr Mask plate_off_mask() {
+  Mask mk = 0;
+  for (int a = 0; a < 2; ++a)
+    for (int b = 0; b < 2; ++b) mk |= 1ull << (6 * a + b);
+  for (int a = 2; a < 5; ++a)
+    for (int b = 2; b < 5; ++b) mk |= 1ull << (6 * a + b);
+  return mk;
+}
+
+uint64_t spread3(uint64_t v) {  // 21 bits -> every third bit
+  uint64_t x = v & 0x1FFFFFull;
+  x = (x | x << 32) & 0x1F00000000FFFFull;
+  x = (x | x << 16) & 0x1F0000FF0000FFull;
+  x = (x | x << 8) & 0x100F00F00F00F00Full;
+  x = (x | x << 4) & 0x10C30C30C30C30C3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+
+// Maps of the fused plate kernel: only plates, every frame exactly the
+// identity, standard flat-plate block masks, bounded valence.  Returns false
+// (and the generic path is used) otherwise.
+bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
+                      const std::vector<int64_t>& row_ptr, femdev::PlateMaps& pm) {
+  const auto& S = m->set[PLATE];
+  if (m->set[TRUSS].size() || m->set[BEAM].size() || S.size() == 0 || S.size() >= (1u << 30)) return false;
+  for (uint16_t rz : m->rzero[PLATE])
+    if (!(rz & 0x4000)) return false;
+  const size_t N = m->nodes_number, E = S.size();
+  const Mask MOFF = plate_off_mask(), MSELF = plate_off_mask() | (1ull << 35);
+  // node -> (element | p << 30) in insertion order
+  std::vector<uint32_t> deg(N + 1, 0);
+  for (size_t e = 0; e < E; ++e)
+    for (int p = 0; p < 4; ++p) deg[S.conn[4 * e + p]]++;
+  std::vector<uint32_t> nptr(N + 1, 0);
+  for (size_t a = 0; a < N; ++a) {
+    if (deg[a] > uint32_t(femdev::PF_MAX_DEG)) return false;
+    nptr[a + 1] = nptr[a] + deg[a];
+  }
+  std::vector<uint32_t> nel(nptr[N]);
+  {
+    std::vector<uint32_t> f(nptr.begin(), nptr.end() - 1);
+    for (size_t e = 0; e < E; ++e)
+      for (int p = 0; p < 4; ++p) nel[f[S.conn[4 * e + p]]++] = uint32_t(e) | uint32_t(p) << 30;
+  }
+  // Morton order of the nodes that carry elements
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (size_t a = 0; a < N; ++a)
+    for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], m->xyz[3 * a + k]); hi[k] = std::max(hi[k], m->xyz[3 * a + k]); }
+  std::vector<std::pair<uint64_t, uint32_t>> order;
+  order.reserve(N);
+  for (size_t a = 0; a < N; ++a) {
+    if (!deg[a]) continue;
+    uint64_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      double span = hi[k] - lo[k];
+      uint64_t qv = span > 0 ? uint64_t((m->xyz[3 * a + k] - lo[k]) / span * 2097151.0) : 0;
+      code |= spread3(qv) << k;
+    }
+    order.push_back({code, uint32_t(a)});
+  }
+  std::sort(order.begin(), order.end());
+  // greedy patches
+  pm.patch_node_ptr.push_back(0);
+  pm.patch_pair_ptr.push_back(0);
+  pm.own_blk_ptr.push_back(0);
+  std::vector<uint32_t> cur;
+  uint32_t cur_pairs = 0;
+  auto flush = [&]() -> bool {
+    if (cur.empty()) return true;
+    std::sort(cur.begin(), cur.end());
+    uint32_t slot = 0;
+    for (uint32_t a : cur) {
+      const uint32_t* el = &nel[nptr[a]];
+      uint32_t d = deg[a];
+      uint64_t k0 = bptr[a], k1 = bptr[a + 1];
+      uint32_t nb = uint32_t(k1 - k0);
+      if (nb > 255) return false;
+      int kself = -1;
+      for (uint64_t k = k0; k < k1; ++k) {
+        uint32_t b = blocks[k].b;
+        if (b == a) {
+          if (blocks[k].mask != MSELF) return false;
+          kself = int(k - k0);
+          pm.blk.push_back(0);
+          continue;
+        }
+        if (blocks[k].mask != MOFF) return false;
+        uint32_t desc = 0, cnt = 0;
+        for (uint32_t r = 0; r < d; ++r) {
+          uint32_t e = el[r] & 0x3FFFFFFFu;
+          for (int q = 0; q < 4; ++q)
+            if (S.conn[4 * size_t(e) + q] == b) {
+              if (cnt == uint32_t(femdev::PF_MAX_CONTRIB)) return false;
+              desc |= (r | uint32_t(q) << 3) << (3 + 5 * cnt);
+              cnt++;
+            }
+        }
+        if (!cnt) return false;
+        pm.blk.push_back(desc | cnt);
+      }
+      if (kself < 0) return false;
+      if (row_ptr[6 * size_t(a) + 6] - row_ptr[6 * size_t(a)] != int64_t(13 * nb + 1)) return false;
+      pm.own_node.push_back(a);
+      pm.own_info.push_back(d | nb << 8 | uint32_t(kself) << 16 | slot << 24);
+      pm.own_blk_ptr.push_back(uint32_t(pm.blk.size()));
+      for (uint32_t r = 0; r < d; ++r) pm.pairs.push_back(el[r]);
+      slot += d;
+    }
+    pm.patch_node_ptr.push_back(uint32_t(pm.own_node.size()));
+    pm.patch_pair_ptr.push_back(uint32_t(pm.pairs.size()));
+    pm.n_patches++;
+    cur.clear();
+    cur_pairs = 0;
+    return true;
+  };
+  for (auto& o : order) {
+    uint32_t a = o.second;
+    if (cur.size() == size_t(femdev::PF_MAX_NODES) || cur_pairs + deg[a] > uint32_t(femdev::PF_MAX_PAIRS))
+      if (!flush()) return false;
+    cur.push_back(a);
+    cur_pairs += deg[a];
+  }
+  return flush();
+}
+
 std::string build_pattern(fem_model* m) {
   std::string e;
   const size_t N = m->nodes_number;
@@ -395,6 +522,18 @@ std::string build_pattern(fem_model* m) {
   e = femdev::upload_pattern(m->dev, n_rows, row_ptr.data(), nnz, col.data());
   if (!e.empty()) return e;
   m->nnz = nnz;
+  col.clear();
+  col.shrink_to_fit();
+  {
+    femdev::PlateMaps pm;
+    if (build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
+      e = femdev::upload_plate_maps(m->dev, pm);
+      if (!e.empty()) return e;
+      m->path = "plate_fused";
+      m->pattern_valid = true;
+      return "";
+    }
+  }
   // slot of every contribution, then gather lists in insertion order
   std::vector<uint32_t> slot_of(static_cast<size_t>(ncontrib));
   for (auto& r : order) {

@@ -75,9 +75,12 @@ def truss_lattice(n=11, seed=0, spacing=1.0):
 
 
 def plate_grid(nx=512, ny=None, seed=0, jitter=0.1, thickness=(0.005, 0.02), E=E_ALU, nu=0.3,
-               shear=5.0 / 6.0, lx=1.0, ly=1.0):
+               shear=5.0 / 6.0, lx=1.0, ly=1.0, rotation=None, shuffle=False):
     """(nx+1)(ny+1) nodes on [0,lx]x[0,ly] in z = 0, interior nodes jittered by
-    U[-jitter, jitter] * spacing in x and y; quads ordered 1(+,+) 2(-,+) 3(-,-) 4(+,-)."""
+    U[-jitter, jitter] * spacing in x and y; quads ordered 1(+,+) 2(-,+) 3(-,-) 4(+,-).
+
+    rotation: optional 3x3 matrix applied to the coordinates (a tilted plate);
+    shuffle: insert nodes and elements in a random order (irregular numbering)."""
     ny = nx if ny is None else ny
     rng = np.random.default_rng(seed)
     hx, hy = lx / nx, ly / ny
@@ -97,6 +100,16 @@ def plate_grid(nx=512, ny=None, seed=0, jitter=0.1, thickness=(0.005, 0.02), E=E
     ne = len(conn_idx)
     t = rng.uniform(thickness[0], thickness[1], ne)
     props = np.stack([np.full(ne, E), np.full(ne, nu), t, np.full(ne, shear)], 1)
+    if rotation is not None:
+        xyz = xyz @ np.asarray(rotation, dtype=np.float64).T
+    if shuffle:
+        perm = rng.permutation(xyz.shape[0])          # new position -> old index
+        inv = np.empty_like(perm)
+        inv[perm] = np.arange(len(perm))
+        xyz = xyz[perm]
+        conn_idx = inv[conn_idx]
+        eperm = rng.permutation(ne)
+        conn_idx, props = conn_idx[eperm], props[eperm]
     return Mesh("plate", np.arange(1, xyz.shape[0] + 1, dtype=np.uint32), xyz,
                 np.arange(1, ne + 1, dtype=np.uint32), (conn_idx + 1).astype(np.uint32), props,
                 meta={"workload": "plate_grid", "nx": nx, "ny": ny, "seed": seed})

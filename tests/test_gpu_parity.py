@@ -16,7 +16,19 @@ CASES = {
     "beam_frame_4x5x3": lambda: meshgen.beam_frame(4, 5, 3, seed=5),
     "plate_8": lambda: meshgen.plate_grid(8, seed=1),
     "plate_17x9": lambda: meshgen.plate_grid(17, 9, seed=2),
+    "plate_64_shuffled": lambda: meshgen.plate_grid(64, seed=7, shuffle=True),
+    "plate_12_tilted": lambda: meshgen.plate_grid(12, seed=8, rotation=_tilt()),
 }
+
+EXPECTED_PATH = {"plate_8": "plate_fused", "plate_17x9": "plate_fused", "plate_64_shuffled": "plate_fused",
+                 "plate_12_tilted": "generic", "beam_frame_6": "generic", "truss_lattice_5": "generic"}
+
+
+def _tilt():
+    a, b = 0.3, -0.7
+    rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    rz = np.array([[np.cos(b), -np.sin(b), 0], [np.sin(b), np.cos(b), 0], [0, 0, 1]])
+    return rz @ rx
 
 
 @pytest.mark.parametrize("name", sorted(CASES))
@@ -28,11 +40,13 @@ def test_assembly_matches_oracle(gpu, name):
     got = m.stiffness_csr()
     r = compare(got, ref)
     print(name, m.assembly_path(), r, flush=True)
+    if name in EXPECTED_PATH:
+        assert m.assembly_path() == EXPECTED_PATH[name]
     assert r["pattern_equal"]
     assert r["max_ratio"] <= 1.0
 
 
-@pytest.mark.parametrize("name", ["plate_8", "beam_frame_6", "truss_lattice_5"])
+@pytest.mark.parametrize("name", ["plate_8", "plate_64_shuffled", "beam_frame_6", "truss_lattice_5"])
 def test_assembly_deterministic(gpu, name):
     mesh = CASES[name]()
     m = FEM.from_mesh(mesh)
