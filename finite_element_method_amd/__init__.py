@@ -65,6 +65,7 @@ def lib():
         "fem_csr_to_host": (ci, [vp, vp, vp, vp]),
         "fem_last_assembly_ms": (dbl, [vp]),
         "fem_assembly_path": (ctypes.c_char_p, [vp]),
+        "fem_set_option": (ci, [vp, ctypes.c_char_p, i64]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -238,6 +239,10 @@ class FEM:
 
     def last_assembly_ms(self):
         return lib().fem_last_assembly_ms(self._h)
+
+    def set_option(self, key, value):
+        """Tuning knobs of the device paths (see fem_set_option in include/fem_amd.h)."""
+        self._call(lib().fem_set_option(self._h, key.encode(), int(value)))
 
     def assembly_path(self):
         return lib().fem_assembly_path(self._h).decode()

@@ -94,6 +94,8 @@ struct fem_model {
   std::vector<uint16_t> rzero[3];
   int64_t nnz = 0;
   std::string path = "none";
+  int pf_pairs = 128;  // fused plate patch size (64, 128 or 256), see fem_set_option
+  bool force_generic = false;
 };
 
 namespace {
@@ -360,7 +362,7 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
   auto flush = [&]() -> bool {
     if (cur.empty()) return true;
     std::sort(cur.begin(), cur.end());
-    uint32_t slot = 0;
+    uint32_t slot = 0, nloc = 0;
     for (uint32_t a : cur) {
       const uint32_t* el = &nel[nptr[a]];
       uint32_t d = deg[a];
@@ -373,7 +375,7 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
         if (b == a) {
           if (blocks[k].mask != MSELF) return false;
           kself = int(k - k0);
-          pm.blk.push_back(0);
+          pm.blk.push_back(nloc << 23 | 1u << 29);
           continue;
         }
         if (blocks[k].mask != MOFF) return false;
@@ -388,7 +390,7 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
             }
         }
         if (!cnt) return false;
-        pm.blk.push_back(desc | cnt);
+        pm.blk.push_back(desc | cnt | nloc << 23);
       }
       if (kself < 0) return false;
       if (row_ptr[6 * size_t(a) + 6] - row_ptr[6 * size_t(a)] != int64_t(13 * nb + 1)) return false;
@@ -397,6 +399,7 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
       pm.own_blk_ptr.push_back(uint32_t(pm.blk.size()));
       for (uint32_t r = 0; r < d; ++r) pm.pairs.push_back(el[r]);
       slot += d;
+      nloc++;
     }
     pm.patch_node_ptr.push_back(uint32_t(pm.own_node.size()));
     pm.patch_pair_ptr.push_back(uint32_t(pm.pairs.size()));
@@ -405,12 +408,19 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
     cur_pairs = 0;
     return true;
   };
+  const uint32_t max_pairs = uint32_t(pm.pairs_per_patch), max_nodes = max_pairs / 4;
+  uint32_t cur_blocks = 0;
   for (auto& o : order) {
     uint32_t a = o.second;
-    if (cur.size() == size_t(femdev::PF_MAX_NODES) || cur_pairs + deg[a] > uint32_t(femdev::PF_MAX_PAIRS))
+    uint32_t nb = uint32_t(bptr[a + 1] - bptr[a]);
+    if (nb > 16) return false;
+    if (cur.size() == size_t(max_nodes) || cur_pairs + deg[a] > max_pairs || cur_blocks + nb > 16 * max_nodes) {
       if (!flush()) return false;
+      cur_blocks = 0;
+    }
     cur.push_back(a);
     cur_pairs += deg[a];
+    cur_blocks += nb;
   }
   return flush();
 }
@@ -526,7 +536,8 @@ std::string build_pattern(fem_model* m) {
   col.shrink_to_fit();
   {
     femdev::PlateMaps pm;
-    if (build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
+    pm.pairs_per_patch = m->pf_pairs;
+    if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
       e = femdev::upload_plate_maps(m->dev, pm);
       if (!e.empty()) return e;
       m->path = "plate_fused";
@@ -894,5 +905,21 @@ int fem_csr_to_host(fem_model* m, int64_t* rp, int32_t* ci, double* v) {
 double fem_last_assembly_ms(fem_model* m) { return m && m->dev ? femdev::last_ms(m->dev) : -1.0; }
 
 const char* fem_assembly_path(fem_model* m) { return m ? m->path.c_str() : ""; }
+
+int fem_set_option(fem_model* m, const char* key, int64_t value) {
+  if (!m || !key) return FEM_ERR_ARGUMENT;
+  std::string k(key);
+  if (k == "plate_patch_pairs") {
+    if (value != 64 && value != 128 && value != 256)
+      return fail(m, FEM_ERR_ARGUMENT, "plate_patch_pairs must be 64, 128 or 256");
+    m->pf_pairs = int(value);
+  } else if (k == "force_generic") {
+    m->force_generic = value != 0;
+  } else {
+    return fail(m, FEM_ERR_ARGUMENT, "unknown option " + k);
+  }
+  m->pattern_valid = false;
+  return FEM_OK;
+}
 
 }  // extern "C"

@@ -107,6 +107,12 @@ int fem_csr_to_host(fem_model* m, int64_t* row_ptr, int32_t* col_idx, double* va
  * HIP events around the value kernels only (ms); -1 when unavailable. */
 double fem_last_assembly_ms(fem_model* m);
 
+/* Tuning options (no counterpart in the crate; they change speed, never values):
+ *   "plate_patch_pairs"  threads per workgroup of the fused plate kernel: 64, 128 or 256
+ *   "force_generic"      1 = always use the generic value path
+ * Setting an option invalidates the pattern (rebuilt by the next assembly). */
+int fem_set_option(fem_model* m, const char* key, int64_t value);
+
 /* Name of the value path used by the last assembly ("plate_fused" or "generic"). */
 const char* fem_assembly_path(fem_model* m);
 
