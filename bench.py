@@ -69,7 +69,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="plate_512", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--patch-pairs", type=int, default=0, help="fused plate workgroup size (0 = default)")
+    ap.add_argument("--patch-nodes", type=int, default=0, help="fused plate: owned nodes per patch (0 = default)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -83,8 +83,8 @@ def main():
     gen, kw = WORKLOADS[args.workload]
     mesh = getattr(meshgen, gen)(**kw)
     model = FEM.from_mesh(mesh)
-    if args.patch_pairs:
-        model.set_option("plate_patch_pairs", args.patch_pairs)
+    if args.patch_nodes:
+        model.set_option("plate_patch_nodes", args.patch_nodes)
     t0 = time.perf_counter()
     model.build_pattern()
     setup_s = time.perf_counter() - t0

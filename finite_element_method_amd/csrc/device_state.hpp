@@ -33,9 +33,9 @@ struct Device {
   int64_t* slot_ptr = nullptr;
   uint32_t* slot_src = nullptr;
   // plate fused maps
-  int n_patches = 0, pf_pairs_per_patch = 128;
+  int n_patches = 0;
   uint32_t *pf_node_ptr = nullptr, *pf_node = nullptr, *pf_info = nullptr, *pf_blk_ptr = nullptr, *pf_blk = nullptr,
-           *pf_pair_ptr = nullptr, *pf_pairs = nullptr;
+           *pf_elem_ptr = nullptr, *pf_elems = nullptr, *pf_row_ptr = nullptr, *pf_rows = nullptr;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;

@@ -297,89 +297,108 @@ __host__ __device__ inline void plate_row_add_ip(const PlateIP& g, const PlateMa
 // and, because the MITC4 shear rows factor as B_r[a] = alpha_a(r,s) * v_a and
 // B_s[a] = beta_a(r,s) * u_a with point-independent v_a, u_a (plate.rs:452-508),
 //   A = sum w alpha_p alpha_q,  B = sum w beta_p beta_q.
-// The 13 entries of block q (layout of plate_row_add_ip) follow from these
-// with the constitutive constants.  Same values as the per-point sums up to
-// rounding.  xl = {x1, y1, x2, y2, x4, y4} relative to node 3.  Block q
-// starts at out[BS * q].
-template <int BS = 13>
-__host__ __device__ inline void plate_row_folded(const double xl[6], const PlateMat& m, int p, double* out) {
+// The 13 entries of block q follow from these with the constitutive
+// constants.  Same values as the per-point sums up to rounding.
+// xl = {x1, y1, x2, y2, x4, y4}: local coordinates relative to node 3.
+
+// per-point data: w = det J (alpha = 1), dh/dx, dh/dy (quad:613-653), alpha, beta
+struct PlatePoint {
+  double w, dx[4], dy[4], al[4], be[4];
+};
+constexpr int PLATE_POINT_DOUBLES = 17;
+
+// integration point ip in the crate's order (g,g), (-g,g), (-g,-g), (g,-g)
+__host__ __device__ inline void plate_point(const double xl[6], int ip, PlatePoint& o) {
   const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double g = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? g : -g;
+  const double s = (ip < 2) ? g : -g;
   const double q4 = 0.25;
   const double x1q = x1 * q4, x2q = x2 * q4, x4q = x4 * q4, y1q = y1 * q4, y2q = y2 * q4, y4q = y4 * q4;
+  // Jacobian (quad:252-338 with x3 = y3 = 0), determinant and inverse (quad:380-503)
+  const double j00 = ((((x1q + x1q * s) - x2q) - x2q * s) + x4q) - x4q * s;
+  const double j01 = ((((y1q + y1q * s) - y2q) - y2q * s) + y4q) - y4q * s;
+  const double j10 = ((((x1q + x1q * r) + x2q) - x2q * r) - x4q) - x4q * r;
+  const double j11 = ((((y1q + y1q * r) + y2q) - y2q * r) - y4q) - y4q * r;
+  const double det = j00 * j11 - j01 * j10;
+  const double rdet = 1.0 / det;
+  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+  // shape-function derivatives (quad:505-583)
+  const double dr[4] = {q4 + q4 * s, -(q4 + q4 * s), -(q4 - q4 * s), q4 - q4 * s};
+  const double ds[4] = {q4 + q4 * r, q4 - q4 * r, -(q4 - q4 * r), -(q4 + q4 * r)};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    o.dx[a] = i00 * dr[a] + i01 * ds[a];
+    o.dy[a] = i10 * dr[a] + i11 * ds[a];
+  }
+  // MITC4 multipliers gamma_rz, gamma_sz (plate.rs:424-447)
   const double ax = x1 - x2 + x4, bx = x1 - x2 - x4, cx = x1 + x2 - x4;
   const double ay = y1 - y2 + y4, by = y1 - y2 - y4, cy = y1 + y2 - y4;
-  double Mxx[4], Mxy[4], Myx[4], Myy[4], A[4], B[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) Mxx[q] = Mxy[q] = Myx[q] = Myy[q] = A[q] = B[q] = 0.0;
-  const double g = sqrt(double(ONE_THIRD_F32));
-#pragma unroll
-  for (int ip = 0; ip < 4; ++ip) {
-    const double r = (ip == 0 || ip == 3) ? g : -g;
-    const double s = (ip < 2) ? g : -g;
-    const double j00 = ((((x1q + x1q * s) - x2q) - x2q * s) + x4q) - x4q * s;
-    const double j01 = ((((y1q + y1q * s) - y2q) - y2q * s) + y4q) - y4q * s;
-    const double j10 = ((((x1q + x1q * r) + x2q) - x2q * r) - x4q) - x4q * r;
-    const double j11 = ((((y1q + y1q * r) + y2q) - y2q * r) - y4q) - y4q * r;
-    const double det = j00 * j11 - j01 * j10;
-    const double rdet = 1.0 / det;
-    const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
-    const double dr[4] = {q4 + q4 * s, -(q4 + q4 * s), -(q4 - q4 * s), q4 - q4 * s};
-    const double ds[4] = {q4 + q4 * r, q4 - q4 * r, -(q4 - q4 * r), -(q4 + q4 * r)};
-    double dx[4], dy[4];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      dx[a] = i00 * dr[a] + i01 * ds[a];
-      dy[a] = i10 * dr[a] + i11 * ds[a];
-    }
-    const double r8 = 0.125 * rdet;
-    const double mr = sqrt((cx + r * bx) * (cx + r * bx) + (cy + r * by) * (cy + r * by)) * r8;
-    const double ms = sqrt((ax + s * bx) * (ax + s * bx) + (ay + s * by) * (ay + s * by)) * r8;
-    const double al[4] = {(1.0 + s) * mr, (1.0 + s) * mr, (1.0 - s) * mr, (1.0 - s) * mr};
-    const double be[4] = {(1.0 + r) * ms, (1.0 - r) * ms, (1.0 - r) * ms, (1.0 + r) * ms};
-    const double w = det;
-    const double wdx = w * sel4(dx[0], dx[1], dx[2], dx[3], p);
-    const double wdy = w * sel4(dy[0], dy[1], dy[2], dy[3], p);
-    const double wal = w * sel4(al[0], al[1], al[2], al[3], p);
-    const double wbe = w * sel4(be[0], be[1], be[2], be[3], p);
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      Mxx[q] += wdx * dx[q];
-      Mxy[q] += wdx * dy[q];
-      Myx[q] += wdy * dx[q];
-      Myy[q] += wdy * dy[q];
-      A[q] += wal * al[q];
-      B[q] += wbe * be[q];
-    }
-  }
-  // point-independent shear vectors (w, theta_x, theta_y components)
+  const double r8 = 0.125 * rdet;
+  const double mr = sqrt((cx + r * bx) * (cx + r * bx) + (cy + r * by) * (cy + r * by)) * r8;
+  const double ms = sqrt((ax + s * bx) * (ax + s * bx) + (ay + s * by) * (ay + s * by)) * r8;
+  o.al[0] = (1.0 + s) * mr; o.al[1] = (1.0 + s) * mr; o.al[2] = (1.0 - s) * mr; o.al[3] = (1.0 - s) * mr;
+  o.be[0] = (1.0 + r) * ms; o.be[1] = (1.0 - r) * ms; o.be[2] = (1.0 - r) * ms; o.be[3] = (1.0 + r) * ms;
+  o.w = det;
+}
+
+// point-independent shear vectors (w, theta_x, theta_y components) of the
+// gamma_rz rows (v) and gamma_sz rows (u), plate.rs:452-508
+struct PlateShear {
+  double v[4][3], u[4][3];
+};
+__host__ __device__ inline void plate_shear_vectors(const double xl[6], PlateShear& o) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
   const double e12x = -(y1 - y2) * 0.25, e12y = (x1 - x2) * 0.25;
   const double e43x = -(y4) * 0.25, e43y = (x4) * 0.25;
   const double e14x = -(y1 - y4) * 0.25, e14y = (x1 - x4) * 0.25;
   const double e23x = -(y2) * 0.25, e23y = (x2) * 0.25;
   const double v[4][3] = {{0.5, e12x, e12y}, {-0.5, e12x, e12y}, {-0.5, e43x, e43y}, {0.5, e43x, e43y}};
   const double u[4][3] = {{0.5, e14x, e14y}, {0.5, e23x, e23y}, {-0.5, e23x, e23y}, {-0.5, e14x, e14y}};
-  double vp[3], up[3];
 #pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    vp[i] = m.cs * sel4(v[0][i], v[1][i], v[2][i], v[3][i], p);
-    up[i] = m.cs * sel4(u[0][i], u[1][i], u[2][i], u[3][i], p);
-  }
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) { o.v[a][i] = v[a][i]; o.u[a][i] = u[a][i]; }
+}
+
+// acc[24] = Mxx[4] Mxy[4] Myx[4] Myy[4] A[4] B[4] of row p; wdx = w dx_p, wdy = w dy_p,
+// wal = w alpha_p, wbe = w beta_p of the same point
+__host__ __device__ inline void plate_row_accumulate(const PlatePoint& g, double wdx, double wdy, double wal, double wbe,
+                                                     double acc[24]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
+    acc[q] += wdx * g.dx[q];
+    acc[4 + q] += wdx * g.dy[q];
+    acc[8 + q] += wdy * g.dx[q];
+    acc[12 + q] += wdy * g.dy[q];
+    acc[16 + q] += wal * g.al[q];
+    acc[20 + q] += wbe * g.be[q];
+  }
+}
+
+// the 13 entries of each block q of row p (block q starts at out[BS * q]);
+// vp, up: shear vectors of node p scaled by the shear constant cs
+template <int BS>
+__host__ __device__ inline void plate_row_finish(const double acc[24], const PlateMat& m, const double vp[3],
+                                                 const double up[3], const PlateShear& sv, double* out) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const double Mxx = acc[q], Mxy = acc[4 + q], Myx = acc[8 + q], Myy = acc[12 + q], A = acc[16 + q],
+                 B = acc[20 + q];
     double* o = out + BS * q;
-    o[0] = m.cm00 * Mxx[q] + m.cm22 * Myy[q];
-    o[1] = m.cm01 * Mxy[q] + m.cm22 * Myx[q];
-    o[2] = m.cm01 * Myx[q] + m.cm22 * Mxy[q];
-    o[3] = m.cm00 * Myy[q] + m.cm22 * Mxx[q];
-    const double bxx = m.cb00 * Myy[q] + m.cb22 * Mxx[q];
-    const double bxy = -(m.cb01 * Myx[q] + m.cb22 * Mxy[q]);
-    const double byx = -(m.cb01 * Mxy[q] + m.cb22 * Myx[q]);
-    const double byy = m.cb00 * Mxx[q] + m.cb22 * Myy[q];
+    o[0] = m.cm00 * Mxx + m.cm22 * Myy;
+    o[1] = m.cm01 * Mxy + m.cm22 * Myx;
+    o[2] = m.cm01 * Myx + m.cm22 * Mxy;
+    o[3] = m.cm00 * Myy + m.cm22 * Mxx;
+    const double bxx = m.cb00 * Myy + m.cb22 * Mxx;
+    const double bxy = -(m.cb01 * Myx + m.cb22 * Mxy);
+    const double byx = -(m.cb01 * Mxy + m.cb22 * Myx);
+    const double byy = m.cb00 * Mxx + m.cb22 * Myy;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
       for (int j = 0; j < 3; ++j) {
-        double sh = (A[q] * vp[i]) * v[q][j] + (B[q] * up[i]) * u[q][j];
+        double sh = (A * vp[i]) * sv.v[q][j] + (B * up[i]) * sv.u[q][j];
         double b = 0.0;
         if (i == 1 && j == 1) b = bxx;
         else if (i == 1 && j == 2) b = bxy;
@@ -388,6 +407,33 @@ __host__ __device__ inline void plate_row_folded(const double xl[6], const Plate
         o[4 + 3 * i + j] = b + sh;
       }
   }
+}
+
+// single-lane version: row block of node p (used by the generic path)
+template <int BS = 13>
+__host__ __device__ inline void plate_row_folded(const double xl[6], const PlateMat& m, int p, double* out) {
+  double acc[24];
+#pragma unroll
+  for (int k = 0; k < 24; ++k) acc[k] = 0.0;
+#pragma unroll 1
+  for (int ip = 0; ip < 4; ++ip) {
+    PlatePoint g;
+    plate_point(xl, ip, g);
+    const double wdx = g.w * sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p);
+    const double wdy = g.w * sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
+    const double wal = g.w * sel4(g.al[0], g.al[1], g.al[2], g.al[3], p);
+    const double wbe = g.w * sel4(g.be[0], g.be[1], g.be[2], g.be[3], p);
+    plate_row_accumulate(g, wdx, wdy, wal, wbe, acc);
+  }
+  PlateShear sv;
+  plate_shear_vectors(xl, sv);
+  double vp[3], up[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vp[i] = m.cs * sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], p);
+    up[i] = m.cs * sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], p);
+  }
+  plate_row_finish<BS>(acc, m, vp, up, sv, out);
 }
 
 // Gauss abscissa of the 2x2 rule in the crate's point order (g,g), (-g,g), (-g,-g), (g,-g)

@@ -52,23 +52,25 @@ struct GenericMaps {
 std::string upload_generic(Device* d, const GenericMaps& g);
 
 // Plate fused path (flat plates whose frame is exactly the identity): one
-// workgroup per patch of owned nodes.  For every owned node, its elements
-// (insertion order = rank r) and its row blocks; see plate_fused.hip.
-// Patch shapes (threads per workgroup = max pairs; owned nodes = pairs / 4;
-// node-pair blocks <= 16 per owned node): 64, 128 or 256 pairs.
+// workgroup per patch of owned nodes; see plate_fused.hip.  Limits per patch:
+constexpr int PF_MAX_ELEMS = 64;   // elements touching the owned nodes
+constexpr int PF_MAX_ROWS = 128;   // owned (node, element) rows
+constexpr int PF_MAX_NODES = 64;   // owned nodes
 constexpr int PF_MAX_DEG = 8;      // elements per node
 constexpr int PF_MAX_CONTRIB = 4;  // elements sharing one off-diagonal node pair
+constexpr int PF_MAX_OUT = 7424;   // CSR values of the owned rows (sum of 13 nb + 1)
 struct PlateMaps {
   int n_patches = 0;
-  int pairs_per_patch = 128;
   std::vector<uint32_t> patch_node_ptr;  // n_patches + 1, into own_*
   std::vector<uint32_t> own_node;        // owned node index
-  std::vector<uint32_t> own_info;        // deg | nb << 8 | kself << 16 | slot_base << 24
+  std::vector<uint32_t> own_info;        // deg | nb << 8 | kself << 16 | first row << 24
   std::vector<uint32_t> own_blk_ptr;     // n_own + 1, into blk
   std::vector<uint32_t> blk;             // per node-pair block: count | (r:3, q:2) << (3 + 5 m) |
                                          //   owned-node slot << 23 | diagonal flag << 29
-  std::vector<uint32_t> patch_pair_ptr;  // n_patches + 1, into pairs
-  std::vector<uint32_t> pairs;           // element | p << 30 (p = local index of the owned node)
+  std::vector<uint32_t> patch_elem_ptr;  // n_patches + 1, into elems
+  std::vector<uint32_t> elems;           // elements touching the patch (insertion order)
+  std::vector<uint32_t> patch_row_ptr;   // n_patches + 1, into rows
+  std::vector<uint32_t> rows;            // per owned row: patch-local element | p << 8
 };
 std::string upload_plate_maps(Device* d, const PlateMaps& pm);
 

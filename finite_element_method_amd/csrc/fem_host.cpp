@@ -94,7 +94,7 @@ struct fem_model {
   std::vector<uint16_t> rzero[3];
   int64_t nnz = 0;
   std::string path = "none";
-  int pf_pairs = 128;  // fused plate patch size (64, 128 or 256), see fem_set_option
+  int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
   bool force_generic = false;
 };
 
@@ -353,22 +353,26 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
     order.push_back({code, uint32_t(a)});
   }
   std::sort(order.begin(), order.end());
-  // greedy patches
+  // greedy patches over the Morton order, within the kernel's limits
+  const uint32_t max_nodes = uint32_t(std::min(m->pf_nodes, femdev::PF_MAX_NODES));
   pm.patch_node_ptr.push_back(0);
-  pm.patch_pair_ptr.push_back(0);
+  pm.patch_elem_ptr.push_back(0);
+  pm.patch_row_ptr.push_back(0);
   pm.own_blk_ptr.push_back(0);
-  std::vector<uint32_t> cur;
-  uint32_t cur_pairs = 0;
+  std::vector<uint32_t> cur, cur_elems;
+  uint32_t cur_rows = 0, cur_blocks = 0, cur_out = 0;
+  std::vector<int32_t> elem_local(E, -1);
   auto flush = [&]() -> bool {
     if (cur.empty()) return true;
     std::sort(cur.begin(), cur.end());
+    std::sort(cur_elems.begin(), cur_elems.end());  // insertion order
+    for (size_t i = 0; i < cur_elems.size(); ++i) elem_local[cur_elems[i]] = int32_t(i);
     uint32_t slot = 0, nloc = 0;
     for (uint32_t a : cur) {
       const uint32_t* el = &nel[nptr[a]];
       uint32_t d = deg[a];
       uint64_t k0 = bptr[a], k1 = bptr[a + 1];
       uint32_t nb = uint32_t(k1 - k0);
-      if (nb > 255) return false;
       int kself = -1;
       for (uint64_t k = k0; k < k1; ++k) {
         uint32_t b = blocks[k].b;
@@ -397,31 +401,50 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
       pm.own_node.push_back(a);
       pm.own_info.push_back(d | nb << 8 | uint32_t(kself) << 16 | slot << 24);
       pm.own_blk_ptr.push_back(uint32_t(pm.blk.size()));
-      for (uint32_t r = 0; r < d; ++r) pm.pairs.push_back(el[r]);
+      for (uint32_t r = 0; r < d; ++r) {
+        uint32_t e = el[r] & 0x3FFFFFFFu, p = el[r] >> 30;
+        pm.rows.push_back(uint32_t(elem_local[e]) | p << 8);
+      }
       slot += d;
       nloc++;
     }
+    for (uint32_t e : cur_elems) {
+      pm.elems.push_back(e);
+      elem_local[e] = -1;
+    }
     pm.patch_node_ptr.push_back(uint32_t(pm.own_node.size()));
-    pm.patch_pair_ptr.push_back(uint32_t(pm.pairs.size()));
+    pm.patch_elem_ptr.push_back(uint32_t(pm.elems.size()));
+    pm.patch_row_ptr.push_back(uint32_t(pm.rows.size()));
     pm.n_patches++;
     cur.clear();
-    cur_pairs = 0;
+    cur_elems.clear();
+    cur_rows = cur_blocks = cur_out = 0;
     return true;
   };
-  const uint32_t max_pairs = uint32_t(pm.pairs_per_patch), max_nodes = max_pairs / 4;
-  uint32_t cur_blocks = 0;
+  std::vector<uint8_t> in_patch(E, 0);
   for (auto& o : order) {
     uint32_t a = o.second;
     uint32_t nb = uint32_t(bptr[a + 1] - bptr[a]);
     if (nb > 16) return false;
-    if (cur.size() == size_t(max_nodes) || cur_pairs + deg[a] > max_pairs || cur_blocks + nb > 16 * max_nodes) {
+    uint32_t new_elems = 0;
+    for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_patch[nel[nptr[a] + r] & 0x3FFFFFFFu];
+    if (cur.size() == max_nodes || cur_rows + deg[a] > uint32_t(femdev::PF_MAX_ROWS) ||
+        cur_elems.size() + new_elems > size_t(femdev::PF_MAX_ELEMS) || cur_blocks + nb > 16u * max_nodes ||
+        cur_out + 13 * nb + 1 > uint32_t(femdev::PF_MAX_OUT)) {
+      for (uint32_t e : cur_elems) in_patch[e] = 0;
       if (!flush()) return false;
-      cur_blocks = 0;
+      new_elems = deg[a];
     }
     cur.push_back(a);
-    cur_pairs += deg[a];
+    cur_rows += deg[a];
     cur_blocks += nb;
+    cur_out += 13 * nb + 1;
+    for (uint32_t r = 0; r < deg[a]; ++r) {
+      uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
+      if (!in_patch[e]) { in_patch[e] = 1; cur_elems.push_back(e); }
+    }
   }
+  for (uint32_t e : cur_elems) in_patch[e] = 0;
   return flush();
 }
 
@@ -536,7 +559,6 @@ std::string build_pattern(fem_model* m) {
   col.shrink_to_fit();
   {
     femdev::PlateMaps pm;
-    pm.pairs_per_patch = m->pf_pairs;
     if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
       e = femdev::upload_plate_maps(m->dev, pm);
       if (!e.empty()) return e;
@@ -909,10 +931,10 @@ const char* fem_assembly_path(fem_model* m) { return m ? m->path.c_str() : ""; }
 int fem_set_option(fem_model* m, const char* key, int64_t value) {
   if (!m || !key) return FEM_ERR_ARGUMENT;
   std::string k(key);
-  if (k == "plate_patch_pairs") {
-    if (value != 64 && value != 128 && value != 256)
-      return fail(m, FEM_ERR_ARGUMENT, "plate_patch_pairs must be 64, 128 or 256");
-    m->pf_pairs = int(value);
+  if (k == "plate_patch_nodes") {
+    if (value < 4 || value > femdev::PF_MAX_NODES)
+      return fail(m, FEM_ERR_ARGUMENT, "plate_patch_nodes must be in [4, 64]");
+    m->pf_nodes = int(value);
   } else if (k == "force_generic") {
     m->force_generic = value != 0;
   } else {

@@ -108,7 +108,7 @@ int fem_csr_to_host(fem_model* m, int64_t* row_ptr, int32_t* col_idx, double* va
 double fem_last_assembly_ms(fem_model* m);
 
 /* Tuning options (no counterpart in the crate; they change speed, never values):
- *   "plate_patch_pairs"  threads per workgroup of the fused plate kernel: 64, 128 or 256
+ *   "plate_patch_nodes"  owned nodes per workgroup of the fused plate kernel: 4..64
  *   "force_generic"      1 = always use the generic value path
  * Setting an option invalidates the pattern (rebuilt by the next assembly). */
 int fem_set_option(fem_model* m, const char* key, int64_t value);
