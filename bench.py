@@ -70,6 +70,7 @@ def main():
     ap.add_argument("--workload", default="plate_512", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--patch-nodes", type=int, default=0, help="fused plate: owned nodes per patch (0 = default)")
+    ap.add_argument("--wg-per-cu", type=int, default=-1, help="fused plate: resident workgroups per CU (0 = one per patch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -85,6 +86,8 @@ def main():
     model = FEM.from_mesh(mesh)
     if args.patch_nodes:
         model.set_option("plate_patch_nodes", args.patch_nodes)
+    if args.wg_per_cu >= 0:
+        model.set_option("plate_wg_per_cu", args.wg_per_cu)
     t0 = time.perf_counter()
     model.build_pattern()
     setup_s = time.perf_counter() - t0

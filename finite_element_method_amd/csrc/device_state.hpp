@@ -33,9 +33,10 @@ struct Device {
   int64_t* slot_ptr = nullptr;
   uint32_t* slot_src = nullptr;
   // plate fused maps
-  int n_patches = 0;
-  uint32_t *pf_node_ptr = nullptr, *pf_node = nullptr, *pf_info = nullptr, *pf_blk_ptr = nullptr, *pf_blk = nullptr,
-           *pf_elem_ptr = nullptr, *pf_elems = nullptr, *pf_row_ptr = nullptr, *pf_rows = nullptr;
+  int n_patches = 0, pf_grid = 0;
+  uint32_t *pf_hdr = nullptr, *pf_conn = nullptr, *pf_eid = nullptr, *pf_rows = nullptr, *pf_oinfo = nullptr,
+           *pf_oboff = nullptr, *pf_oooff = nullptr, *pf_blk = nullptr;
+  int64_t* pf_obase = nullptr;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;

@@ -376,6 +376,33 @@ __host__ __device__ inline void plate_row_accumulate(const PlatePoint& g, double
   }
 }
 
+// the 13 entries of block q of row p from its six integrals; vp, up: shear
+// vectors of node p scaled by the shear constant cs; vq, uq: those of node q
+__host__ __device__ inline void plate_block_entries(double Mxx, double Mxy, double Myx, double Myy, double A, double B,
+                                                    const PlateMat& m, const double vp[3], const double up[3],
+                                                    const double vq[3], const double uq[3], double o[13]) {
+  o[0] = m.cm00 * Mxx + m.cm22 * Myy;
+  o[1] = m.cm01 * Mxy + m.cm22 * Myx;
+  o[2] = m.cm01 * Myx + m.cm22 * Mxy;
+  o[3] = m.cm00 * Myy + m.cm22 * Mxx;
+  const double bxx = m.cb00 * Myy + m.cb22 * Mxx;
+  const double bxy = -(m.cb01 * Myx + m.cb22 * Mxy);
+  const double byx = -(m.cb01 * Mxy + m.cb22 * Myx);
+  const double byy = m.cb00 * Mxx + m.cb22 * Myy;
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      double sh = (A * vp[i]) * vq[j] + (B * up[i]) * uq[j];
+      double b = 0.0;
+      if (i == 1 && j == 1) b = bxx;
+      else if (i == 1 && j == 2) b = bxy;
+      else if (i == 2 && j == 1) b = byx;
+      else if (i == 2 && j == 2) b = byy;
+      o[4 + 3 * i + j] = b + sh;
+    }
+}
+
 // the 13 entries of each block q of row p (block q starts at out[BS * q]);
 // vp, up: shear vectors of node p scaled by the shear constant cs
 template <int BS>
@@ -383,29 +410,8 @@ __host__ __device__ inline void plate_row_finish(const double acc[24], const Pla
                                                  const double up[3], const PlateShear& sv, double* out) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    const double Mxx = acc[q], Mxy = acc[4 + q], Myx = acc[8 + q], Myy = acc[12 + q], A = acc[16 + q],
-                 B = acc[20 + q];
-    double* o = out + BS * q;
-    o[0] = m.cm00 * Mxx + m.cm22 * Myy;
-    o[1] = m.cm01 * Mxy + m.cm22 * Myx;
-    o[2] = m.cm01 * Myx + m.cm22 * Mxy;
-    o[3] = m.cm00 * Myy + m.cm22 * Mxx;
-    const double bxx = m.cb00 * Myy + m.cb22 * Mxx;
-    const double bxy = -(m.cb01 * Myx + m.cb22 * Mxy);
-    const double byx = -(m.cb01 * Mxy + m.cb22 * Myx);
-    const double byy = m.cb00 * Mxx + m.cb22 * Myy;
-#pragma unroll
-    for (int i = 0; i < 3; ++i)
-#pragma unroll
-      for (int j = 0; j < 3; ++j) {
-        double sh = (A * vp[i]) * sv.v[q][j] + (B * up[i]) * sv.u[q][j];
-        double b = 0.0;
-        if (i == 1 && j == 1) b = bxx;
-        else if (i == 1 && j == 2) b = bxy;
-        else if (i == 2 && j == 1) b = byx;
-        else if (i == 2 && j == 2) b = byy;
-        o[4 + 3 * i + j] = b + sh;
-      }
+    plate_block_entries(acc[q], acc[4 + q], acc[8 + q], acc[12 + q], acc[16 + q], acc[20 + q], m, vp, up, sv.v[q],
+                        sv.u[q], out + BS * q);
   }
 }
 

@@ -51,8 +51,8 @@ void destroy_device(Device* d) {
   f(d->xyz);
   for (int t = 0; t < 3; ++t) { f(d->conn[t]); f(d->params[t]); f(d->R[t]); f(d->rzero[t]); f(d->eclass[t]); f(d->ebase[t]); }
   f(d->derived); f(d->row_ptr); f(d->col_idx); f(d->val); f(d->class_mask); f(d->class_off); f(d->contrib);
-  f(d->slot_ptr); f(d->slot_src); f(d->pf_node_ptr); f(d->pf_node); f(d->pf_info); f(d->pf_blk_ptr); f(d->pf_blk);
-  f(d->pf_elem_ptr); f(d->pf_elems); f(d->pf_row_ptr); f(d->pf_rows);
+  f(d->slot_ptr); f(d->slot_src); f(d->pf_hdr); f(d->pf_conn); f(d->pf_eid); f(d->pf_rows); f(d->pf_oinfo);
+  f(d->pf_oboff); f(d->pf_oooff); f(d->pf_blk); f(d->pf_obase);
   if (d->ev0) hipEventDestroy(d->ev0);
   if (d->ev1) hipEventDestroy(d->ev1);
   delete d;

@@ -58,21 +58,26 @@ constexpr int PF_MAX_ROWS = 128;   // owned (node, element) rows
 constexpr int PF_MAX_NODES = 64;   // owned nodes
 constexpr int PF_MAX_DEG = 8;      // elements per node
 constexpr int PF_MAX_CONTRIB = 4;  // elements sharing one off-diagonal node pair
-constexpr int PF_MAX_OUT = 7424;   // CSR values of the owned rows (sum of 13 nb + 1)
+constexpr int PF_MAX_OUT = 4096;   // CSR values of the owned rows (sum of 13 nb + 1)
+constexpr int PF_MAX_BLOCKS = 512; // owned node-pair blocks
 struct PlateMaps {
   int n_patches = 0;
-  std::vector<uint32_t> patch_node_ptr;  // n_patches + 1, into own_*
-  std::vector<uint32_t> own_node;        // owned node index
+  // per patch: {elem_off, node_off, row_off, blk_off}, {n_elems, n_owned, n_rows, n_blocks},
+  //            {n_values, 0, 0, 0}
+  std::vector<uint32_t> hdr;             // 12 per patch
+  std::vector<uint32_t> pconn;           // 4 per patch element: node indices
+  std::vector<uint32_t> peid;            // per patch element: element id (insertion order within the patch)
+  std::vector<uint32_t> rows;            // per owned row: patch-local element | p << 8
+  std::vector<int64_t> own_base;         // per owned node: CSR position of its first row
   std::vector<uint32_t> own_info;        // deg | nb << 8 | kself << 16 | first row << 24
-  std::vector<uint32_t> own_blk_ptr;     // n_own + 1, into blk
+  std::vector<uint32_t> own_boff;        // first block of the owned node within the patch
+  std::vector<uint32_t> own_ooff;        // first CSR value of the owned node within the patch
   std::vector<uint32_t> blk;             // per node-pair block: count | (r:3, q:2) << (3 + 5 m) |
                                          //   owned-node slot << 23 | diagonal flag << 29
-  std::vector<uint32_t> patch_elem_ptr;  // n_patches + 1, into elems
-  std::vector<uint32_t> elems;           // elements touching the patch (insertion order)
-  std::vector<uint32_t> patch_row_ptr;   // n_patches + 1, into rows
-  std::vector<uint32_t> rows;            // per owned row: patch-local element | p << 8
 };
-std::string upload_plate_maps(Device* d, const PlateMaps& pm);
+// wg_per_cu: resident workgroups per CU that loop over the patches (0 = one
+// workgroup per patch)
+std::string upload_plate_maps(Device* d, const PlateMaps& pm, int wg_per_cu);
 
 // Value kernels.  `val` is nnz doubles of device memory, `stream` a hipStream_t.
 std::string assemble_generic(Device* d, double* val, void* stream);

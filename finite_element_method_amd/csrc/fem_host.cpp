@@ -95,6 +95,7 @@ struct fem_model {
   int64_t nnz = 0;
   std::string path = "none";
   int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
+  int pf_wg_per_cu = 2;  // fused plate: resident workgroups per CU looping over patches (0 = one per patch)
   bool force_generic = false;
 };
 
@@ -317,6 +318,7 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
                       const std::vector<int64_t>& row_ptr, femdev::PlateMaps& pm) {
   const auto& S = m->set[PLATE];
   if (m->set[TRUSS].size() || m->set[BEAM].size() || S.size() == 0 || S.size() >= (1u << 30)) return false;
+  if (row_ptr.back() * 8 >= (int64_t(1) << 32)) return false;  // 32-bit byte offsets of the buffer stores
   for (uint16_t rz : m->rzero[PLATE])
     if (!(rz & 0x4000)) return false;
   const size_t N = m->nodes_number, E = S.size();
@@ -355,10 +357,6 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
   std::sort(order.begin(), order.end());
   // greedy patches over the Morton order, within the kernel's limits
   const uint32_t max_nodes = uint32_t(std::min(m->pf_nodes, femdev::PF_MAX_NODES));
-  pm.patch_node_ptr.push_back(0);
-  pm.patch_elem_ptr.push_back(0);
-  pm.patch_row_ptr.push_back(0);
-  pm.own_blk_ptr.push_back(0);
   std::vector<uint32_t> cur, cur_elems;
   uint32_t cur_rows = 0, cur_blocks = 0, cur_out = 0;
   std::vector<int32_t> elem_local(E, -1);
@@ -367,13 +365,16 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
     std::sort(cur.begin(), cur.end());
     std::sort(cur_elems.begin(), cur_elems.end());  // insertion order
     for (size_t i = 0; i < cur_elems.size(); ++i) elem_local[cur_elems[i]] = int32_t(i);
-    uint32_t slot = 0, nloc = 0;
+    const uint32_t elem_off = uint32_t(pm.peid.size()), node_off = uint32_t(pm.own_info.size()),
+                   row_off = uint32_t(pm.rows.size()), blk_off = uint32_t(pm.blk.size());
+    uint32_t slot = 0, nloc = 0, ooff = 0;
     for (uint32_t a : cur) {
       const uint32_t* el = &nel[nptr[a]];
       uint32_t d = deg[a];
       uint64_t k0 = bptr[a], k1 = bptr[a + 1];
       uint32_t nb = uint32_t(k1 - k0);
       int kself = -1;
+      const uint32_t boff = uint32_t(pm.blk.size()) - blk_off;
       for (uint64_t k = k0; k < k1; ++k) {
         uint32_t b = blocks[k].b;
         if (b == a) {
@@ -398,9 +399,11 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
       }
       if (kself < 0) return false;
       if (row_ptr[6 * size_t(a) + 6] - row_ptr[6 * size_t(a)] != int64_t(13 * nb + 1)) return false;
-      pm.own_node.push_back(a);
+      pm.own_base.push_back(row_ptr[6 * size_t(a)]);
       pm.own_info.push_back(d | nb << 8 | uint32_t(kself) << 16 | slot << 24);
-      pm.own_blk_ptr.push_back(uint32_t(pm.blk.size()));
+      pm.own_boff.push_back(boff);
+      pm.own_ooff.push_back(ooff);
+      ooff += 13 * nb + 1;
       for (uint32_t r = 0; r < d; ++r) {
         uint32_t e = el[r] & 0x3FFFFFFFu, p = el[r] >> 30;
         pm.rows.push_back(uint32_t(elem_local[e]) | p << 8);
@@ -409,12 +412,13 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
       nloc++;
     }
     for (uint32_t e : cur_elems) {
-      pm.elems.push_back(e);
+      pm.peid.push_back(e);
+      for (int q = 0; q < 4; ++q) pm.pconn.push_back(S.conn[4 * size_t(e) + q]);
       elem_local[e] = -1;
     }
-    pm.patch_node_ptr.push_back(uint32_t(pm.own_node.size()));
-    pm.patch_elem_ptr.push_back(uint32_t(pm.elems.size()));
-    pm.patch_row_ptr.push_back(uint32_t(pm.rows.size()));
+    const uint32_t h[12] = {elem_off, node_off, row_off, blk_off, uint32_t(cur_elems.size()), uint32_t(cur.size()),
+                            uint32_t(pm.rows.size()) - row_off, uint32_t(pm.blk.size()) - blk_off, ooff, 0, 0, 0};
+    pm.hdr.insert(pm.hdr.end(), h, h + 12);
     pm.n_patches++;
     cur.clear();
     cur_elems.clear();
@@ -429,11 +433,10 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
     uint32_t new_elems = 0;
     for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_patch[nel[nptr[a] + r] & 0x3FFFFFFFu];
     if (cur.size() == max_nodes || cur_rows + deg[a] > uint32_t(femdev::PF_MAX_ROWS) ||
-        cur_elems.size() + new_elems > size_t(femdev::PF_MAX_ELEMS) || cur_blocks + nb > 16u * max_nodes ||
+        cur_elems.size() + new_elems > size_t(femdev::PF_MAX_ELEMS) || cur_blocks + nb > uint32_t(femdev::PF_MAX_BLOCKS) ||
         cur_out + 13 * nb + 1 > uint32_t(femdev::PF_MAX_OUT)) {
       for (uint32_t e : cur_elems) in_patch[e] = 0;
       if (!flush()) return false;
-      new_elems = deg[a];
     }
     cur.push_back(a);
     cur_rows += deg[a];
@@ -560,7 +563,7 @@ std::string build_pattern(fem_model* m) {
   {
     femdev::PlateMaps pm;
     if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
-      e = femdev::upload_plate_maps(m->dev, pm);
+      e = femdev::upload_plate_maps(m->dev, pm, m->pf_wg_per_cu);
       if (!e.empty()) return e;
       m->path = "plate_fused";
       m->pattern_valid = true;
@@ -935,6 +938,9 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
     if (value < 4 || value > femdev::PF_MAX_NODES)
       return fail(m, FEM_ERR_ARGUMENT, "plate_patch_nodes must be in [4, 64]");
     m->pf_nodes = int(value);
+  } else if (k == "plate_wg_per_cu") {
+    if (value < 0 || value > 64) return fail(m, FEM_ERR_ARGUMENT, "plate_wg_per_cu must be in [0, 64]");
+    m->pf_wg_per_cu = int(value);
   } else if (k == "force_generic") {
     m->force_generic = value != 0;
   } else {

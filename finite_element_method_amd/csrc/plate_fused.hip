@@ -17,68 +17,134 @@
 // owned node is the sum of KROT6 over its elements.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device_state.hpp"
 #include "element_math.hpp"
 #include "fem_device.hpp"
 
 namespace femdev {
 
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
 constexpr int PF_THREADS = 256;
 constexpr int PF_ELEMS = 64;                 // elements touching a patch
 constexpr int PF_ROWS = 128;                 // owned (node, element) rows
 constexpr int PF_NODES = 64;                 // owned nodes
-constexpr int PF_OUT = PF_MAX_OUT;           // staged CSR values of a patch (13 nb + 1 per node)
+constexpr int PF_BLOCKS = 2 * PF_THREADS;    // owned node-pair blocks
 // LDS strides are odd multiples of 16 bytes so that 16-byte accesses of
 // neighbouring lanes fall into different banks.
 constexpr int PF_GEO = 18;                   // doubles per (element, point): PlatePoint (17) + pad
 constexpr int PF_EDAT = 34;                  // doubles per element: 7 constants, pad, v[12], u[12], pad
 constexpr int PF_BS = 14;                    // doubles per staged block (13 used)
 constexpr int PF_SLOT = 4 * PF_BS + 2;       // doubles per staged row
-constexpr int PF_STEP1 = PF_ELEMS * (4 * PF_GEO + PF_EDAT);
 constexpr int pf_max(int a, int b) { return a > b ? a : b; }
-constexpr int PF_LDS = pf_max(pf_max(PF_ROWS * PF_SLOT, PF_STEP1), PF_OUT);
+// LDS: element constants | one region reused for point data, staged rows, staged output
+constexpr int PF_REGION = pf_max(pf_max(PF_ROWS * PF_SLOT, PF_ELEMS * 4 * PF_GEO), PF_MAX_OUT);
 static_assert(PF_THREADS == 4 * PF_ELEMS, "one lane per (element, point)");
-static_assert(PF_ELEMS == PF_MAX_ELEMS && PF_ROWS == PF_MAX_ROWS && PF_NODES == PF_MAX_NODES, "host limits");
+static_assert(PF_ELEMS == PF_MAX_ELEMS && PF_ROWS == PF_MAX_ROWS && PF_NODES == PF_MAX_NODES &&
+                  PF_BLOCKS == PF_MAX_BLOCKS,
+              "host limits");
 
-__global__ __launch_bounds__(PF_THREADS) void plate_fused_kernel(
-    const double* __restrict__ xyz, const uint32_t* __restrict__ conn, const double* __restrict__ params,
-    const uint32_t* __restrict__ node_ptr, const uint32_t* __restrict__ own_node,
-    const uint32_t* __restrict__ own_info, const uint32_t* __restrict__ blk_ptr, const uint32_t* __restrict__ blk,
-    const uint32_t* __restrict__ elem_ptr, const uint32_t* __restrict__ elems, const uint32_t* __restrict__ row_ptr_,
-    const uint32_t* __restrict__ rows, const int64_t* __restrict__ csr_row_ptr, double* __restrict__ val) {
-  __shared__ double2 lds2[PF_LDS / 2];
-  __shared__ int stage_p[PF_ROWS];
-  __shared__ int64_t s_base[PF_NODES];
-  __shared__ uint32_t s_info[PF_NODES];
-  __shared__ uint32_t s_boff[PF_NODES];
-  __shared__ uint32_t s_ooff[PF_NODES + 1];
-  double* lds = reinterpret_cast<double*>(lds2);
-  double2* geo2 = lds2;                                   // [elem][point][PF_GEO / 2]
-  double* edat = lds + PF_ELEMS * 4 * PF_GEO;             // [elem][PF_EDAT]
-  const int patch = blockIdx.x;
-  const int t = threadIdx.x;
-  const uint32_t n0 = node_ptr[patch], nown = node_ptr[patch + 1] - n0;
-  const uint32_t e0 = elem_ptr[patch], nel = elem_ptr[patch + 1] - e0;
-  const uint32_t r0 = row_ptr_[patch], nrows = row_ptr_[patch + 1] - r0;
-  const uint32_t b0 = blk_ptr[n0], nblk = blk_ptr[n0 + nown] - b0;
+// Inputs of one patch held in registers, loaded one patch ahead.  Every load
+// is unconditional (indices clamped into the patch, which is never empty) so
+// that the loads of the next patch stay in flight while this one computes.
+struct PatchRegs {
+  uint32_t elem_off, node_off, row_off, blk_off, nel, nown, nrows, nblk, total;
+  uint4 c;             // lane's element (t / 4): node indices
+  uint32_t eid;        // lane's element id
+  double X[4][2];      // lane's element: node coordinates (x, y)
+  double4 prm;         // lane's element: E, nu, t, k
+  uint32_t row;        // lane's owned row
+  int64_t base;        // lane's owned node: CSR position of its row 6a
+  uint32_t info, boff, ooff;
+  uint32_t blk[2];     // lane's node-pair blocks t and t + 256
+};
 
-  if (t < int(nown)) {
-    const uint32_t a = own_node[n0 + t];
-    s_base[t] = csr_row_ptr[6 * int64_t(a)];
-    s_info[t] = own_info[n0 + t];
-    s_boff[t] = blk_ptr[n0 + t] - b0;
+struct PatchArgs {
+  const double* __restrict__ xyz;
+  const double* __restrict__ params;
+  const uint4* __restrict__ hdr;
+  const uint4* __restrict__ pconn;
+  const uint32_t* __restrict__ peid;
+  const uint32_t* __restrict__ rows;
+  const int64_t* __restrict__ obase;
+  const uint32_t* __restrict__ oinfo;
+  const uint32_t* __restrict__ oboff;
+  const uint32_t* __restrict__ oooff;
+  const uint32_t* __restrict__ blk;
+  int n_patches;
+  double* __restrict__ val;
+  uint32_t val_bytes;
+};
+
+struct PatchLds {
+  double2 region[PF_REGION / 2];    // point data | staged rows | staged output
+  double edat[PF_ELEMS * PF_EDAT];  // element constants
+  int stage_p[PF_ROWS];
+  int64_t base[PF_NODES];
+  uint32_t info[PF_NODES];
+  uint32_t boff[PF_NODES];
+  uint32_t ooff[PF_NODES + 1];
+};
+// the position -> owned node table of step 5 sits in the region above the staged output
+static_assert(PF_MAX_OUT * 8 + PF_MAX_OUT * 4 <= PF_REGION * 8, "region too small");
+
+__device__ __forceinline__ void pf_load_header(PatchRegs& P, const PatchArgs& A, int patch) {
+  const uint4 h0 = A.hdr[3 * patch], h1 = A.hdr[3 * patch + 1], h2 = A.hdr[3 * patch + 2];
+  P.elem_off = h0.x; P.node_off = h0.y; P.row_off = h0.z; P.blk_off = h0.w;
+  P.nel = h1.x; P.nown = h1.y; P.nrows = h1.z; P.nblk = h1.w;
+  P.total = h2.x;
+}
+__device__ __forceinline__ void pf_load_maps(PatchRegs& P, const PatchArgs& A, int t) {
+  const uint32_t ut = uint32_t(t);
+  const uint32_t le = min(ut >> 2, P.nel - 1);
+  P.c = A.pconn[P.elem_off + le];
+  P.eid = A.peid[P.elem_off + le];
+  P.row = A.rows[P.row_off + min(ut, P.nrows - 1)];
+  const uint32_t o = P.node_off + min(ut, P.nown - 1);
+  P.base = A.obase[o];
+  P.info = A.oinfo[o];
+  P.boff = A.oboff[o];
+  P.ooff = A.oooff[o];
+  P.blk[0] = A.blk[P.blk_off + min(ut, P.nblk - 1)];
+  P.blk[1] = A.blk[P.blk_off + min(ut + PF_THREADS, P.nblk - 1)];
+}
+__device__ __forceinline__ void pf_load_element(PatchRegs& P, const PatchArgs& A) {
+  const uint32_t n[4] = {P.c.x, P.c.y, P.c.z, P.c.w};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double2 v = *reinterpret_cast<const double2*>(A.xyz + 3 * size_t(n[a]));
+    P.X[a][0] = v.x; P.X[a][1] = v.y;
   }
+  P.prm = *reinterpret_cast<const double4*>(A.params + 4 * size_t(P.eid));
+}
+
+// One patch: compute `cur`, prefetch `nxt` (the patch gridDim.x further on).
+__device__ __forceinline__ bool pf_process(PatchRegs& cur, PatchRegs& nxt, int patch, const PatchArgs& A,
+                                           PatchLds& L) {
+  const int t = threadIdx.x;
+  const int next = patch + int(gridDim.x);
+  const bool has_next = next < A.n_patches;
+  pf_load_header(nxt, A, has_next ? next : patch);
+  double* lds = reinterpret_cast<double*>(L.region);
+  double2* geo2 = L.region;  // [elem][point][PF_GEO / 2]
+  uint32_t* pos_off = reinterpret_cast<uint32_t*>(lds + PF_MAX_OUT);  // byte offset of each staged value
+
+  __syncthreads();  // the previous patch's LDS reads are complete
+  if (t < int(cur.nown)) { L.base[t] = cur.base; L.info[t] = cur.info; L.boff[t] = cur.boff; L.ooff[t] = cur.ooff; }
+  if (t == 0) L.ooff[cur.nown] = cur.total;
 
   // ---- step 1: point kinematics of every patch element
   const int le = t >> 2, ip = t & 3;
-  if (le < int(nel)) {
-    const uint32_t e = elems[e0 + le];
-    const uint4 c = *reinterpret_cast<const uint4*>(conn + 4 * size_t(e));
-    const double x3 = xyz[3 * size_t(c.z)], y3 = xyz[3 * size_t(c.z) + 1];
+  if (le < int(cur.nel)) {
     double xl[6];
-    xl[0] = xyz[3 * size_t(c.x)] - x3; xl[1] = xyz[3 * size_t(c.x) + 1] - y3;
-    xl[2] = xyz[3 * size_t(c.y)] - x3; xl[3] = xyz[3 * size_t(c.y) + 1] - y3;
-    xl[4] = xyz[3 * size_t(c.w)] - x3; xl[5] = xyz[3 * size_t(c.w) + 1] - y3;
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      const int a = m < 2 ? m : 3;
+      xl[2 * m] = cur.X[a][0] - cur.X[2][0];
+      xl[2 * m + 1] = cur.X[a][1] - cur.X[2][1];
+    }
     fem::PlatePoint g;
     fem::plate_point(xl, ip, g);
     double2* go = geo2 + (le * 4 + ip) * (PF_GEO / 2);
@@ -91,38 +157,36 @@ __global__ __launch_bounds__(PF_THREADS) void plate_fused_kernel(
     go[6] = make_double2(g.al[3], g.be[0]);
     go[7] = make_double2(g.be[1], g.be[2]);
     go[8] = make_double2(g.be[3], 0.0);
+    double* ed = L.edat + le * PF_EDAT;
     if (ip == 0) {
-      const double4 prm = *reinterpret_cast<const double4*>(params + 4 * size_t(e));
-      const fem::PlateMat m = fem::plate_material(prm.x, prm.y, prm.z, prm.w);
+      const fem::PlateMat m = fem::plate_material(cur.prm.x, cur.prm.y, cur.prm.z, cur.prm.w);
+      ed[0] = m.cm00; ed[1] = m.cm01; ed[2] = m.cm22; ed[3] = m.cb00; ed[4] = m.cb01; ed[5] = m.cb22; ed[6] = m.cs;
+    } else {
       fem::PlateShear sv;
       fem::plate_shear_vectors(xl, sv);
-      double* ed = edat + le * PF_EDAT;
-      ed[0] = m.cm00; ed[1] = m.cm01; ed[2] = m.cm22; ed[3] = m.cb00; ed[4] = m.cb01; ed[5] = m.cb22; ed[6] = m.cs;
+      // lane 1: v of nodes 0-3 (first half), lane 2: v (second half) and u (first), lane 3: u (rest)
 #pragma unroll
       for (int a = 0; a < 4; ++a)
 #pragma unroll
-        for (int i = 0; i < 3; ++i) { ed[8 + 3 * a + i] = sv.v[a][i]; ed[20 + 3 * a + i] = sv.u[a][i]; }
+        for (int i = 0; i < 3; ++i) {
+          const int kv = 3 * a + i;
+          if ((ip == 1 && kv < 8) || (ip == 2 && kv >= 8)) ed[8 + kv] = sv.v[a][i];
+          if ((ip == 2 && kv < 4) || (ip == 3 && kv >= 4)) ed[20 + kv] = sv.u[a][i];
+        }
     }
-  }
-  if (t == 0) {
-    uint32_t off = 0;
-    for (uint32_t o = 0; o < nown; ++o) {
-      s_ooff[o] = off;
-      off += 13 * ((s_info[o] >> 8) & 255u) + 1;
-    }
-    s_ooff[nown] = off;
   }
   __syncthreads();
+  pf_load_maps(nxt, A, t);
 
-  // ---- step 2: owned rows folded over the four points
-  double row[PF_SLOT - 2];
-  if (t < int(nrows)) {
-    const uint32_t ref = rows[r0 + t];
-    const int re = int(ref & 255u), p = int(ref >> 8);
-    double acc[24];
+  // ---- step 2: owned rows folded over the four points (24 integrals per row)
+  double acc[24];
+  int p = 0, re = 0;
+  if (t < int(cur.nrows)) {
+    re = int(cur.row & 255u);
+    p = int(cur.row >> 8);
 #pragma unroll
     for (int k = 0; k < 24; ++k) acc[k] = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int q = 0; q < 4; ++q) {
       const double2* gi = geo2 + (re * 4 + q) * (PF_GEO / 2);
       double gv[18];
@@ -138,120 +202,160 @@ __global__ __launch_bounds__(PF_THREADS) void plate_fused_kernel(
       const double wbe = g.w * fem::sel4(g.be[0], g.be[1], g.be[2], g.be[3], p);
       fem::plate_row_accumulate(g, wdx, wdy, wal, wbe, acc);
     }
-    const double* ed = edat + re * PF_EDAT;
+    L.stage_p[t] = p;
+  }
+  __syncthreads();
+
+  // ---- step 3: row blocks from the integrals, staged over the point data
+  if (t < int(cur.nrows)) {
+    const double* ed = L.edat + re * PF_EDAT;
     fem::PlateMat m;
     m.cm00 = ed[0]; m.cm01 = ed[1]; m.cm22 = ed[2]; m.cb00 = ed[3]; m.cb01 = ed[4]; m.cb22 = ed[5]; m.cs = ed[6];
-    fem::PlateShear sv;
-#pragma unroll
-    for (int a = 0; a < 4; ++a)
-#pragma unroll
-      for (int i = 0; i < 3; ++i) { sv.v[a][i] = ed[8 + 3 * a + i]; sv.u[a][i] = ed[20 + 3 * a + i]; }
     double vp[3], up[3];
 #pragma unroll
     for (int i = 0; i < 3; ++i) { vp[i] = m.cs * ed[8 + 3 * p + i]; up[i] = m.cs * ed[20 + 3 * p + i]; }
-    fem::plate_row_finish<PF_BS>(acc, m, vp, up, sv, row);
-    stage_p[t] = p;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      double vq[3], uq[3], o[14];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) { vq[i] = ed[8 + 3 * q + i]; uq[i] = ed[20 + 3 * q + i]; }
+      fem::plate_block_entries(acc[q], acc[4 + q], acc[8 + q], acc[12 + q], acc[16 + q], acc[20 + q], m, vp, up,
+                               vq, uq, o);
+      o[13] = 0.0;
+      double2* dst = L.region + (t * PF_SLOT + PF_BS * q) / 2;
+#pragma unroll
+      for (int h = 0; h < 7; ++h) dst[h] = make_double2(o[2 * h], o[2 * h + 1]);
+    }
   }
   __syncthreads();
+  pf_load_element(nxt, A);
 
-  // ---- step 3: stage the rows (over the step-1 data)
-  if (t < int(nrows)) {
+  // ---- step 4: one lane per owned node-pair block; contributors summed in
+  //      element-insertion order, then placed in CSR order
+  double bsum[2][13];
 #pragma unroll
-    for (int k = 0; k < (PF_SLOT - 2) / 2; ++k) lds2[t * (PF_SLOT / 2) + k] = make_double2(row[2 * k], row[2 * k + 1]);
-  }
-  __syncthreads();
-
-  // ---- step 4: one lane per owned node-pair block: contributors summed in
-  //      element-insertion order (registers), then placed in CSR order (LDS)
-  constexpr int MAXB = (PF_NODES * 16 + PF_THREADS - 1) / PF_THREADS;
-  double bsum[MAXB][13];
-  int bpos[MAXB];
-#pragma unroll
-  for (int it = 0; it < MAXB; ++it) {
-    const uint32_t bi = uint32_t(t + it * PF_THREADS);
-    bpos[it] = -1;
-    if (bi >= nblk) continue;
-    const uint32_t d = blk[b0 + bi];
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t bi = uint32_t(t + j * PF_THREADS);
+    if (bi >= cur.nblk) continue;
+    const uint32_t d = cur.blk[j];
     const int n = int((d >> 23) & 63u);
-    const uint32_t info = s_info[n];
+    const uint32_t info = L.info[n];
     const int deg = int(info & 255u), sb = int(info >> 24);
     const bool self = (d >> 29) & 1u;
     const int cnt = self ? deg : int(d & 7u);
     for (int m = 0; m < cnt; ++m) {
       int r, q;
-      if (self) { r = m; q = stage_p[sb + m]; }
+      if (self) { r = m; q = L.stage_p[sb + m]; }
       else { r = int((d >> (3 + 5 * m)) & 7u); q = int((d >> (6 + 5 * m)) & 3u); }
-      const double2* src = lds2 + ((sb + r) * PF_SLOT + PF_BS * q) / 2;
+      const double2* src = L.region + ((sb + r) * PF_SLOT + PF_BS * q) / 2;
       double v[14];
 #pragma unroll
       for (int h = 0; h < 7; ++h) { const double2 w = src[h]; v[2 * h] = w.x; v[2 * h + 1] = w.y; }
 #pragma unroll
-      for (int j = 0; j < 13; ++j) bsum[it][j] = (m == 0) ? v[j] : bsum[it][j] + v[j];
+      for (int i = 0; i < 13; ++i) bsum[j][i] = (m == 0) ? v[i] : bsum[j][i] + v[i];
     }
-    bpos[it] = int(bi);
   }
   __syncthreads();
 #pragma unroll
-  for (int it = 0; it < MAXB; ++it) {
-    if (bpos[it] < 0) continue;
-    const uint32_t bi = uint32_t(bpos[it]);
-    const uint32_t d = blk[b0 + bi];
+  for (int j = 0; j < 2; ++j) {
+    const uint32_t bi = uint32_t(t + j * PF_THREADS);
+    if (bi >= cur.nblk) continue;
+    const uint32_t d = cur.blk[j];
     const int n = int((d >> 23) & 63u);
-    const uint32_t info = s_info[n];
+    const uint32_t info = L.info[n];
     const int deg = int(info & 255u), nb = int((info >> 8) & 255u);
-    const int k = int(bi - s_boff[n]);
-    double* o = lds + s_ooff[n];
-    o[2 * k] = bsum[it][0];
-    o[2 * k + 1] = bsum[it][1];
-    o[2 * nb + 2 * k] = bsum[it][2];
-    o[2 * nb + 2 * k + 1] = bsum[it][3];
+    const int k = int(bi - L.boff[n]);
+    const int o0 = int(L.ooff[n]);
+    const uint32_t g0 = uint32_t(L.base[n]) * 8u;  // 32-bit byte offsets (checked when the maps are built)
+    const int pos[13] = {2 * k, 2 * k + 1, 2 * nb + 2 * k, 2 * nb + 2 * k + 1,
+                         4 * nb + 3 * k, 4 * nb + 3 * k + 1, 4 * nb + 3 * k + 2,
+                         7 * nb + 3 * k, 7 * nb + 3 * k + 1, 7 * nb + 3 * k + 2,
+                         10 * nb + 3 * k, 10 * nb + 3 * k + 1, 10 * nb + 3 * k + 2};
 #pragma unroll
-    for (int j = 0; j < 3; ++j) {
-      o[4 * nb + 3 * k + j] = bsum[it][4 + j];
-      o[7 * nb + 3 * k + j] = bsum[it][7 + j];
-      o[10 * nb + 3 * k + j] = bsum[it][10 + j];
+    for (int i = 0; i < 13; ++i) {
+      lds[o0 + pos[i]] = bsum[j][i];
+      pos_off[o0 + pos[i]] = g0 + 8u * uint32_t(pos[i]);
     }
     if ((d >> 29) & 1u) {
       // drilling row: KROT6 of every element, in insertion order
       double dr = fem::KROT6;
       for (int r = 1; r < deg; ++r) dr += fem::KROT6;
-      o[13 * nb] = dr;
+      lds[o0 + 13 * nb] = dr;
+      pos_off[o0 + 13 * nb] = g0 + 8u * uint32_t(13 * nb);
     }
   }
   __syncthreads();
 
-  // ---- step 5: coalesced write-out of the owned CSR segments
-  const int lane = t & 63, wave = t >> 6;
-  for (int o = wave; o < int(nown); o += PF_THREADS / 64) {
-    const int len = int(s_ooff[o + 1] - s_ooff[o]);
-    const double* src = lds + s_ooff[o];
-    double* dst = val + s_base[o];
-    for (int i = lane; i < len; i += 64) dst[i] = src[i];
+  // ---- step 5: the patch's CSR values, 16 buffer stores per lane (lanes past
+  //      the end store to an out-of-range offset, which the hardware drops)
+  const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(A.val, 0, int(A.val_bytes), 0x00020000);
+  const uint32_t total = L.ooff[cur.nown];
+#pragma unroll
+  for (int k = 0; k < PF_MAX_OUT / PF_THREADS; ++k) {
+    const uint32_t pos = uint32_t(t + k * PF_THREADS);
+    const bool valid = pos < total;
+    const uint32_t pc = valid ? pos : 0u;
+    const uint32_t off = valid ? pos_off[pc] : 0xFFFFFFFFu;
+    const u32x2 v = __builtin_bit_cast(u32x2, lds[pc]);
+    __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, off, 0, 0);
+  }
+  return has_next;
+}
+
+__global__ __launch_bounds__(PF_THREADS, 2) void plate_fused_kernel(PatchArgs A) {
+  __shared__ PatchLds L;
+  int patch = blockIdx.x;
+  if (patch >= A.n_patches) return;
+  PatchRegs P0, P1;
+  pf_load_header(P0, A, patch);
+  pf_load_maps(P0, A, threadIdx.x);
+  pf_load_element(P0, A);
+  while (true) {
+    if (!pf_process(P0, P1, patch, A, L)) break;
+    patch += int(gridDim.x);
+    if (!pf_process(P1, P0, patch, A, L)) break;
+    patch += int(gridDim.x);
   }
 }
 
-std::string upload_plate_maps(Device* d, const PlateMaps& pm) {
+std::string upload_plate_maps(Device* d, const PlateMaps& pm, int wg_per_cu) {
   d->n_patches = pm.n_patches;
   std::string e;
-  if (!(e = dupload(d->pf_node_ptr, pm.patch_node_ptr.data(), pm.patch_node_ptr.size())).empty()) return e;
-  if (!(e = dupload(d->pf_node, pm.own_node.data(), pm.own_node.size())).empty()) return e;
-  if (!(e = dupload(d->pf_info, pm.own_info.data(), pm.own_info.size())).empty()) return e;
-  if (!(e = dupload(d->pf_blk_ptr, pm.own_blk_ptr.data(), pm.own_blk_ptr.size())).empty()) return e;
+  if (!(e = dupload(d->pf_hdr, pm.hdr.data(), pm.hdr.size())).empty()) return e;
+  if (!(e = dupload(d->pf_conn, pm.pconn.data(), pm.pconn.size())).empty()) return e;
+  if (!(e = dupload(d->pf_eid, pm.peid.data(), pm.peid.size())).empty()) return e;
+  if (!(e = dupload(d->pf_rows, pm.rows.data(), pm.rows.size())).empty()) return e;
+  if (!(e = dupload(d->pf_obase, pm.own_base.data(), pm.own_base.size())).empty()) return e;
+  if (!(e = dupload(d->pf_oinfo, pm.own_info.data(), pm.own_info.size())).empty()) return e;
+  if (!(e = dupload(d->pf_oboff, pm.own_boff.data(), pm.own_boff.size())).empty()) return e;
+  if (!(e = dupload(d->pf_oooff, pm.own_ooff.data(), pm.own_ooff.size())).empty()) return e;
   if (!(e = dupload(d->pf_blk, pm.blk.data(), pm.blk.size())).empty()) return e;
-  if (!(e = dupload(d->pf_elem_ptr, pm.patch_elem_ptr.data(), pm.patch_elem_ptr.size())).empty()) return e;
-  if (!(e = dupload(d->pf_elems, pm.elems.data(), pm.elems.size())).empty()) return e;
-  if (!(e = dupload(d->pf_row_ptr, pm.patch_row_ptr.data(), pm.patch_row_ptr.size())).empty()) return e;
-  return dupload(d->pf_rows, pm.rows.data(), pm.rows.size());
+  int cus = 0;
+  HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->dev));
+  d->pf_grid = wg_per_cu <= 0 ? pm.n_patches : std::max(1, std::min(pm.n_patches, cus * wg_per_cu));
+  return "";
 }
 
 std::string assemble_plate_fused(Device* d, double* val, void* stream) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipEventRecord(d->ev0, s));
   if (d->n_patches > 0) {
-    plate_fused_kernel<<<d->n_patches, PF_THREADS, 0, s>>>(d->xyz, d->conn[PLATE], d->params[PLATE], d->pf_node_ptr,
-                                                         d->pf_node, d->pf_info, d->pf_blk_ptr, d->pf_blk,
-                                                         d->pf_elem_ptr, d->pf_elems, d->pf_row_ptr, d->pf_rows,
-                                                         d->row_ptr, val);
+    PatchArgs a;
+    a.xyz = d->xyz;
+    a.params = d->params[PLATE];
+    a.hdr = reinterpret_cast<const uint4*>(d->pf_hdr);
+    a.pconn = reinterpret_cast<const uint4*>(d->pf_conn);
+    a.peid = d->pf_eid;
+    a.rows = d->pf_rows;
+    a.obase = d->pf_obase;
+    a.oinfo = d->pf_oinfo;
+    a.oboff = d->pf_oboff;
+    a.oooff = d->pf_oooff;
+    a.blk = d->pf_blk;
+    a.n_patches = d->n_patches;
+    a.val = val;
+    a.val_bytes = uint32_t(d->nnz * 8);
+    plate_fused_kernel<<<d->pf_grid, PF_THREADS, 0, s>>>(a);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(d->ev1, s));
