@@ -22,6 +22,7 @@ import numpy as np
 __all__ = ["FEM", "FemError", "DeviceError", "TRUSS", "BEAM", "PLATE", "lib"]
 
 TRUSS, BEAM, PLATE = 0, 1, 2
+PARAMS_PER = {TRUSS: 3, BEAM: 11, PLATE: 4}
 FEM_OK, FEM_ERR_VALIDATION, FEM_ERR_STATE, FEM_ERR_DEVICE, FEM_ERR_ARGUMENT = 0, 1, 2, 3, 4
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -66,6 +67,12 @@ def lib():
         "fem_last_assembly_ms": (dbl, [vp]),
         "fem_assembly_path": (ctypes.c_char_p, [vp]),
         "fem_set_option": (ci, [vp, ctypes.c_char_p, i64]),
+        "fem_model_size": (ci, [vp, psz, psz, psz, psz]),
+        "fem_backward": (ci, [vp, vp, vp, vp, vp, vp, vp]),
+        "fem_compliance_sensitivity": (ci, [vp, vp, vp, vp, vp, vp, vp]),
+        "fem_last_backward_ms": (dbl, [vp]),
+        "fem_update_params": (ci, [vp, ci, vp, vp]),
+        "fem_update_nodes": (ci, [vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -213,6 +220,10 @@ class FEM:
 
     def assemble_into(self, val, stream=None):
         """Assembles into a caller-owned float64 CUDA tensor of nnz values."""
+        self.build_pattern()
+        nnz = self.csr_info()[1]
+        if val.numel() != nnz or not val.is_cuda or val.dtype.itemsize != 8 or not val.is_contiguous():
+            raise ValueError("expected a contiguous float64 CUDA tensor of %d values" % nnz)
         self._call(lib().fem_assemble_into(self._h, ctypes.c_void_p(val.data_ptr()), _stream_handle(stream)))
 
     def csr_info(self):
@@ -246,6 +257,72 @@ class FEM:
 
     def assembly_path(self):
         return lib().fem_assembly_path(self._h).decode()
+
+    # ----------------------------------------------------------- differentiation
+    def model_size(self):
+        """(n_nodes, n_truss, n_beam, n_plate) committed so far."""
+        v = [ctypes.c_size_t() for _ in range(4)]
+        self._call(lib().fem_model_size(self._h, *[ctypes.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def _adjoint(self, fn, g, params, xyz, stream, expect):
+        import torch
+
+        n_nodes, *counts = self.model_size()
+        if g.numel() != expect:
+            raise ValueError("expected %d values, got %d" % (expect, g.numel()))
+        dev = g.device
+        grads = {}
+        ptrs = []
+        for t, n in zip((TRUSS, BEAM, PLATE), counts):
+            if params and n:
+                grads[t] = torch.empty((n, PARAMS_PER[t]), dtype=torch.float64, device=dev)
+                ptrs.append(ctypes.c_void_p(grads[t].data_ptr()))
+            else:
+                ptrs.append(None)
+        dxyz = torch.empty((n_nodes, 3), dtype=torch.float64, device=dev) if xyz else None
+        if g.dtype != torch.float64 or not g.is_cuda or not g.is_contiguous():
+            raise ValueError("expected a contiguous float64 CUDA tensor")
+        if stream is None:
+            stream = torch.cuda.current_stream(dev)
+        self._call(fn(self._h, ctypes.c_void_p(g.data_ptr()), *ptrs,
+                      None if dxyz is None else ctypes.c_void_p(dxyz.data_ptr()), _stream_handle(stream)))
+        return grads, dxyz
+
+    def backward(self, dl_dk, params=True, xyz=True, stream=None):
+        """Gradients of a scalar L(K) from dL/dK (float64 CUDA tensor of nnz
+        values in CSR order): ({TRUSS: (n, 3), BEAM: (n, 11), PLATE: (n, 4)}
+        parameter gradients in the add_* layouts, (N, 3) node-coordinate
+        gradient or None).  See fem_backward in include/fem_amd.h."""
+        self.build_pattern()
+        return self._adjoint(lib().fem_backward, dl_dk, params, xyz, stream, self.csr_info()[1])
+
+    def compliance_sensitivity(self, u, params=True, xyz=True, stream=None):
+        """-u_e^T (dK_e/dp) u_e summed per parameter, for nodal displacements
+        u (float64 CUDA tensor of 6N values, dof order of K)."""
+        return self._adjoint(lib().fem_compliance_sensitivity, u, params, xyz, stream, 6 * self.model_size()[0])
+
+    def last_backward_ms(self):
+        return lib().fem_last_backward_ms(self._h)
+
+    def update_params(self, etype, params, stream=None):
+        """Replaces one element type's parameters (float64 CUDA tensor, add_* layout)."""
+        params = params.detach().contiguous()
+        n = self.model_size()[1 + int(etype)]
+        if params.numel() != n * PARAMS_PER[etype] or not params.is_cuda or params.dtype.itemsize != 8:
+            raise ValueError("expected a float64 CUDA tensor of %d x %d" % (n, PARAMS_PER[etype]))
+        self._call(lib().fem_update_params(self._h, int(etype), ctypes.c_void_p(params.data_ptr()),
+                                           _stream_handle(stream)))
+        self._version = getattr(self, "_version", 0) + 1
+
+    def update_nodes(self, xyz, stream=None):
+        """Replaces all node coordinates (float64 CUDA tensor, N x 3)."""
+        xyz = xyz.detach().contiguous()
+        n = self.model_size()[0]
+        if xyz.numel() != 3 * n or not xyz.is_cuda or xyz.dtype.itemsize != 8:
+            raise ValueError("expected a float64 CUDA tensor of %d x 3" % n)
+        self._call(lib().fem_update_nodes(self._h, ctypes.c_void_p(xyz.data_ptr()), _stream_handle(stream)))
+        self._version = getattr(self, "_version", 0) + 1
 
     # ----------------------------------------------------------- convenience
     def load_mesh(self, mesh):

@@ -1,0 +1,403 @@
+// adjoint.hip -- backward (adjoint) kernels of the assembly: derivatives of a
+// scalar L(K) with respect to element parameters and node coordinates, given
+// dL/dK on the CSR pattern, and the compliance sensitivities -u_e^T (dK_e/dp) u_e
+// given nodal displacements u.  Element stiffness is recomputed, not stored.
+//
+// Element gradient blocks are gathered through the pattern's block structure:
+// element node pair (p, q) -> global node-pair block k (bid), whose CSR entries
+// in row i start at row_ptr[6 a + i] + boff[6 k + i] and follow the block's
+// union mask bmask[k].  All pattern entries of the block are read (not only the
+// element's own structural entries), so coordinate derivatives that create
+// entries the element lacks at its current geometry are still seen.
+//
+// Determinism: per-element results are written once; the four lanes of a plate
+// element combine with a fixed butterfly; nodal coordinate gradients are
+// per-(element, node) rows summed per node in element-insertion order by
+// node_gather_kernel.  No atomics.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "bar_adjoint.hpp"
+#include "device_state.hpp"
+#include "element_math.hpp"
+#include "fem_device.hpp"
+#include "plate_adjoint.hpp"
+
+namespace femdev {
+
+namespace {
+
+constexpr int MODE_DLDK = 0;  // g = dL/dK values (nnz)
+constexpr int MODE_U = 1;     // g = nodal displacements (6 N); L = -u^T K u per element
+
+struct AdjArgs {
+  const double* xyz;
+  const uint32_t* conn;
+  const double* params;
+  const double* R;
+  const double* derived;
+  size_t n;
+  const double* g;
+  const uint32_t* bid;
+  const uint64_t* bmask;
+  const uint16_t* boff;
+  const int64_t* row_ptr;
+  double* dparam;  // n x params_per, may be null
+  double* xs;      // n x nodes_per x 3, may be null
+  double rel_tol;
+};
+
+// logical block of hardware block b: the hardware deals blocks to the eight
+// XCDs round robin (b % 8); each XCD gets a contiguous range of element blocks
+// so that neighbouring elements share an L2
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
+  const unsigned x = b & 7u, j = b >> 3, q = nb >> 3, r = nb & 7u;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
+}
+
+// global-frame 6x6 gradient block of node pair block k, rows of node a
+__device__ inline void gather_block(const AdjArgs& A, uint32_t k, uint32_t a, double G[36]) {
+  const uint64_t um = A.bmask[k];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    const uint32_t mi = uint32_t(um >> (6 * i)) & 63u;
+    const int64_t rs = A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i];
+#pragma unroll
+    for (int j = 0; j < 6; ++j)
+      G[i * 6 + j] = ((mi >> j) & 1u) ? A.g[rs + __builtin_popcount(mi & ((1u << j) - 1u))] : 0.0;
+  }
+}
+
+__device__ inline bool is_identity(const double R[9]) {
+  return R[0] == 1.0 && R[4] == 1.0 && R[8] == 1.0 && R[1] == 0.0 && R[2] == 0.0 && R[3] == 0.0 && R[5] == 0.0 &&
+         R[6] == 0.0 && R[7] == 0.0;
+}
+
+// Gl = R6 G R6^T in place (sub-blocks R G_XY R^T)
+__device__ inline void to_local(double G[36], const double R[9]) {
+#pragma unroll
+  for (int bi = 0; bi < 2; ++bi)
+#pragma unroll
+    for (int bj = 0; bj < 2; ++bj) {
+      double T[9];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          T[i * 3 + j] = (R[i * 3 + 0] * G[(3 * bi + 0) * 6 + 3 * bj + j] + R[i * 3 + 1] * G[(3 * bi + 1) * 6 + 3 * bj + j]) +
+                         R[i * 3 + 2] * G[(3 * bi + 2) * 6 + 3 * bj + j];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          G[(3 * bi + i) * 6 + 3 * bj + j] = (T[i * 3 + 0] * R[j * 3 + 0] + T[i * 3 + 1] * R[j * 3 + 1]) + T[i * 3 + 2] * R[j * 3 + 2];
+    }
+}
+
+// local-frame displacement of node a: (R u_t, R u_r)
+__device__ inline void local_u(const double* __restrict__ u, uint32_t a, const double R[9], double o[6]) {
+  double v[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) v[i] = u[6 * size_t(a) + i];
+#pragma unroll
+  for (int h = 0; h < 2; ++h)
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      o[3 * h + i] = (R[i * 3 + 0] * v[3 * h + 0] + R[i * 3 + 1] * v[3 * h + 1]) + R[i * 3 + 2] * v[3 * h + 2];
+}
+
+__device__ inline double quad_sum(double v) {
+  v += __shfl_xor(v, 1, 4);
+  v += __shfl_xor(v, 2, 4);
+  return v;
+}
+
+// ---------------------------------------------------------------- plates
+// Four lanes per element (lane p = row block of node p), 64 elements per block.
+template <int MODE>
+__global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const size_t e0 = size_t(b) * 64 + (threadIdx.x >> 2);
+  const int p = int(threadIdx.x & 3u);
+  const bool live = e0 < A.n;
+  const size_t e = live ? e0 : A.n - 1;  // dead lanes still take part in the lane sums
+  uint32_t nd[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) nd[a] = A.conn[4 * e + a];
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = A.R[9 * e + k];
+  const bool ident = is_identity(R);
+  double loc[4][3];
+  {
+    const double x3 = A.xyz[3 * size_t(nd[2])], y3 = A.xyz[3 * size_t(nd[2]) + 1], z3 = A.xyz[3 * size_t(nd[2]) + 2];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double dd[3] = {A.xyz[3 * size_t(nd[a])] - x3, A.xyz[3 * size_t(nd[a]) + 1] - y3,
+                            A.xyz[3 * size_t(nd[a]) + 2] - z3};
+#pragma unroll
+      for (int i = 0; i < 3; ++i) loc[a][i] = (R[i * 3 + 0] * dd[0] + R[i * 3 + 1] * dd[1]) + R[i * 3 + 2] * dd[2];
+    }
+  }
+  const double xl[6] = {loc[0][0], loc[0][1], loc[1][0], loc[1][1], loc[3][0], loc[3][1]};
+  const double E = A.params[4 * e], nu = A.params[4 * e + 1], t = A.params[4 * e + 2], kk = A.params[4 * e + 3];
+  const fem::PlateMat mat = fem::plate_material(E, nu, t, kk);
+  fem::PlateShear sv;
+  fem::plate_shear_vectors(xl, sv);
+  // pass 1: integrals of row p
+  double acc[24];
+#pragma unroll
+  for (int i = 0; i < 24; ++i) acc[i] = 0.0;
+#pragma unroll 1
+  for (int ip = 0; ip < 4; ++ip) {
+    fem::PlatePoint g;
+    fem::plate_point(xl, ip, g);
+    const double wdx = g.w * fem::sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p);
+    const double wdy = g.w * fem::sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
+    const double wal = g.w * fem::sel4(g.al[0], g.al[1], g.al[2], g.al[3], p);
+    const double wbe = g.w * fem::sel4(g.be[0], g.be[1], g.be[2], g.be[3], p);
+    fem::plate_row_accumulate(g, wdx, wdy, wal, wbe, acc);
+  }
+  // pass 2: gradient blocks of row p
+  double P[7], c[24], gxl[6], hx = 0.0, hy = 0.0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) P[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
+  const uint32_t ap = p == 0 ? nd[0] : (p == 1 ? nd[1] : (p == 2 ? nd[2] : nd[3]));
+  double up[6];
+  if (MODE == MODE_U) local_u(A.g, ap, R, up);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    double gl[36];
+    if (MODE == MODE_DLDK) {
+      gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
+      if (!ident) to_local(gl, R);
+    } else {
+      double uq[6];
+      local_u(A.g, nd[q], R, uq);
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) gl[i * 6 + j] = -(up[i] * uq[j]);
+    }
+    fem::plate_block_adjoint(acc, mat, sv, p, q, gl, P, c, gxl, hx, hy);
+  }
+  // pass 3: reverse sweep over the integration points
+#pragma unroll 1
+  for (int ip = 0; ip < 4; ++ip) fem::plate_point_adjoint(xl, ip, p, c, gxl);
+  // element sums over the four lanes
+#pragma unroll
+  for (int i = 0; i < 7; ++i) P[i] = quad_sum(P[i]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gxl[i] = quad_sum(gxl[i]);
+  hx = quad_sum(hx);
+  hy = quad_sum(hy);
+  if (!live) return;
+  double gp[4], gX[4][3];
+  fem::plate_adjoint_finish(P, gxl, hx, hy, loc, R, E, nu, t, kk, gp, gX);
+  if (A.dparam) A.dparam[4 * e + p] = fem::sel4(gp[0], gp[1], gp[2], gp[3], p);
+  if (A.xs) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) A.xs[(4 * e + p) * 3 + i] = fem::sel4(gX[0][i], gX[1][i], gX[2][i], gX[3][i], p);
+  }
+}
+
+// ---------------------------------------------------------------- trusses
+template <int MODE>
+__global__ __launch_bounds__(256) void truss_adjoint_kernel(AdjArgs A) {
+  const size_t e = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+  if (e >= A.n) return;
+  const uint32_t nd[2] = {A.conn[2 * e], A.conn[2 * e + 1]};
+  double d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = A.xyz[3 * size_t(nd[1]) + i] - A.xyz[3 * size_t(nd[0]) + i];
+  const double r0[3] = {A.R[9 * e], A.R[9 * e + 1], A.R[9 * e + 2]};
+  double H[9];
+  if (MODE == MODE_DLDK) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) H[i] = 0.0;
+#pragma unroll 1
+    for (int pq = 0; pq < 4; ++pq) {
+      const int p = pq >> 1, q = pq & 1;
+      const uint32_t k = A.bid[4 * e + pq], a = nd[p];
+      const uint64_t um = A.bmask[k];
+      const double s = p == q ? 1.0 : -1.0;
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        const uint32_t mi = uint32_t(um >> (6 * i)) & 63u;
+        const int64_t rs = A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+          if ((mi >> j) & 1u) H[i * 3 + j] += s * A.g[rs + __builtin_popcount(mi & ((1u << j) - 1u))];
+      }
+    }
+  } else {
+    double du[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) du[i] = A.g[6 * size_t(nd[1]) + i] - A.g[6 * size_t(nd[0]) + i];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int j = 0; j < 3; ++j) H[i * 3 + j] = -(du[i] * du[j]);
+  }
+  double gp[3], gd[3];
+  fem::truss_adjoint(d, r0, A.params[3 * e], A.params[3 * e + 1], A.params[3 * e + 2], H, gp, gd);
+  if (A.dparam) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) A.dparam[3 * e + i] = gp[i];
+  }
+  if (A.xs) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      A.xs[(2 * e) * 3 + i] = -gd[i];
+      A.xs[(2 * e + 1) * 3 + i] = gd[i];
+    }
+  }
+}
+
+// ---------------------------------------------------------------- beams
+template <int MODE>
+__global__ __launch_bounds__(256) void beam_adjoint_kernel(AdjArgs A) {
+  const size_t e = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
+  if (e >= A.n) return;
+  const uint32_t nd[2] = {A.conn[2 * e], A.conn[2 * e + 1]};
+  double d[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) d[i] = A.xyz[3 * size_t(nd[1]) + i] - A.xyz[3 * size_t(nd[0]) + i];
+  const double L = fem::norm3(d[0], d[1], d[2]);
+  const double J = fem::bar_jacobian(L);
+  const double inv_j = 1.0 / J;
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = A.R[9 * e + k];
+  const double* prm = A.params + 11 * e;
+  double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int pq = 0; pq < 4; ++pq) {
+    const int p = pq >> 1, q = pq & 1;
+    double gl[36];
+    if (MODE == MODE_DLDK) {
+      gather_block(A, A.bid[4 * e + pq], nd[p], gl);
+      to_local(gl, R);
+    } else {
+      double up[6], uq[6];
+      local_u(A.g, nd[p], R, up);
+      local_u(A.g, nd[q], R, uq);
+#pragma unroll
+      for (int i = 0; i < 6; ++i)
+#pragma unroll
+        for (int j = 0; j < 6; ++j) gl[i * 6 + j] = -(up[i] * uq[j]);
+    }
+    fem::beam_term_sums(gl, (p == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j, (q == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j, S);
+  }
+  double i11p, i22p, angle;
+  fem::beam_principal(prm[3], prm[4], prm[5], A.rel_tol, i11p, i22p, angle);
+  double gp[11];
+  fem::beam_adjoint_params(S, J * 2.0, prm[0], prm[1], prm[2], prm[3], prm[4], prm[5], prm[6], prm[7],
+                           A.derived[2 * e], A.derived[2 * e + 1], angle, gp);
+  (void)i11p;
+  (void)i22p;
+  if (A.dparam) {
+#pragma unroll
+    for (int i = 0; i < 11; ++i) A.dparam[11 * e + i] = gp[i];
+  }
+}
+
+// nodal coordinate gradient = sum of the node's (element, node) rows in
+// element-insertion order
+__global__ __launch_bounds__(256) void node_gather_kernel(const uint32_t* __restrict__ inc_ptr,
+                                                          const uint32_t* __restrict__ inc,
+                                                          const double* __restrict__ xs, size_t n_nodes,
+                                                          double* __restrict__ dxyz) {
+  const size_t a = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (a >= n_nodes) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (uint32_t k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
+    const size_t r = inc[k];
+    s0 += xs[3 * r];
+    s1 += xs[3 * r + 1];
+    s2 += xs[3 * r + 2];
+  }
+  dxyz[3 * a] = s0;
+  dxyz[3 * a + 1] = s1;
+  dxyz[3 * a + 2] = s2;
+}
+
+}  // namespace
+
+std::string upload_adjoint_maps(Device* d, const AdjointMaps& am) {
+  for (int t = 0; t < 3; ++t) {
+    auto e = dupload(d->adj_bid[t], am.bid[t].data(), am.bid[t].size());
+    if (!e.empty()) return e;
+  }
+  auto e = dupload(d->adj_bmask, am.bmask.data(), am.bmask.size());
+  if (!e.empty()) return e;
+  e = dupload(d->adj_boff, am.boff.data(), am.boff.size());
+  if (!e.empty()) return e;
+  e = dupload(d->adj_inc_ptr, am.inc_ptr.data(), am.inc_ptr.size());
+  if (!e.empty()) return e;
+  e = dupload(d->adj_inc, am.inc.data(), am.inc.size());
+  if (!e.empty()) return e;
+  for (int t = 0; t < 3; ++t) d->adj_xoff[t] = am.xoff[t];
+  d->adj_nxs = am.n_xs_rows;
+  return dalloc(d->adj_xs, size_t(3 * am.n_xs_rows));
+}
+
+std::string run_adjoint(Device* d, int mode, const double* g, double* const dparam[3], double* dxyz, void* stream) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipEventRecord(d->ev2, s));
+  for (int t = 0; t < 3; ++t) {
+    const size_t n = d->n_elem[t];
+    if (!n) continue;
+    AdjArgs A{};
+    A.xyz = d->xyz;
+    A.conn = d->conn[t];
+    A.params = d->params[t];
+    A.R = d->R[t];
+    A.derived = d->derived;
+    A.n = n;
+    A.g = g;
+    A.bid = d->adj_bid[t];
+    A.bmask = d->adj_bmask;
+    A.boff = d->adj_boff;
+    A.row_ptr = d->row_ptr;
+    A.dparam = dparam[t];
+    A.xs = (dxyz && t != BEAM) ? d->adj_xs + 3 * d->adj_xoff[t] : nullptr;
+    A.rel_tol = d->rel_tol;
+    if (t == PLATE) {
+      const unsigned blocks = unsigned((n + 63) / 64);
+      if (mode == MODE_DLDK) plate_adjoint_kernel<MODE_DLDK><<<blocks, 256, 0, s>>>(A);
+      else plate_adjoint_kernel<MODE_U><<<blocks, 256, 0, s>>>(A);
+    } else {
+      const unsigned blocks = unsigned((n + 255) / 256);
+      if (t == TRUSS) {
+        if (mode == MODE_DLDK) truss_adjoint_kernel<MODE_DLDK><<<blocks, 256, 0, s>>>(A);
+        else truss_adjoint_kernel<MODE_U><<<blocks, 256, 0, s>>>(A);
+      } else {
+        if (mode == MODE_DLDK) beam_adjoint_kernel<MODE_DLDK><<<blocks, 256, 0, s>>>(A);
+        else beam_adjoint_kernel<MODE_U><<<blocks, 256, 0, s>>>(A);
+      }
+    }
+    HIP_TRY(hipGetLastError());
+  }
+  if (dxyz && d->n_nodes) {
+    const unsigned blocks = unsigned((d->n_nodes + 255) / 256);
+    node_gather_kernel<<<blocks, 256, 0, s>>>(d->adj_inc_ptr, d->adj_inc, d->adj_xs, d->n_nodes, dxyz);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(d->ev3, s));
+  d->timed_bwd = true;
+  return "";
+}
+
+double last_backward_ms(Device* d) {
+  if (!d->timed_bwd) return -1.0;
+  if (hipEventSynchronize(d->ev3) != hipSuccess) return -1.0;
+  float ms = 0.f;
+  if (hipEventElapsedTime(&ms, d->ev2, d->ev3) != hipSuccess) return -1.0;
+  return double(ms);
+}
+
+}  // namespace femdev

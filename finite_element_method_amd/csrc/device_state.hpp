@@ -37,9 +37,18 @@ struct Device {
   uint32_t *pf_hdr = nullptr, *pf_conn = nullptr, *pf_eid = nullptr, *pf_rows = nullptr, *pf_oinfo = nullptr,
            *pf_oboff = nullptr, *pf_oooff = nullptr, *pf_blk = nullptr;
   int64_t* pf_obase = nullptr;
+  // adjoint maps
+  uint32_t* adj_bid[3] = {nullptr, nullptr, nullptr};
+  uint64_t* adj_bmask = nullptr;
+  uint16_t* adj_boff = nullptr;
+  uint32_t *adj_inc_ptr = nullptr, *adj_inc = nullptr;
+  double* adj_xs = nullptr;
+  int64_t adj_xoff[3] = {0, 0, 0};
+  int64_t adj_nxs = 0;
+  double rel_tol = 0.0, abs_tol = 0.0;
   // timing
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  bool timed = false, timed_bwd = false;
 };
 
 #define HIP_TRY(expr)                                                                 \

@@ -37,7 +37,8 @@ Device* create_device(std::string& err) {
   }
   auto* d = new Device();
   hipGetDevice(&d->dev);
-  if (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess) {
+  if (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess ||
+      hipEventCreate(&d->ev2) != hipSuccess || hipEventCreate(&d->ev3) != hipSuccess) {
     err = "hipEventCreate failed";
     delete d;
     return nullptr;
@@ -53,8 +54,10 @@ void destroy_device(Device* d) {
   f(d->derived); f(d->row_ptr); f(d->col_idx); f(d->val); f(d->class_mask); f(d->class_off); f(d->contrib);
   f(d->slot_ptr); f(d->slot_src); f(d->pf_hdr); f(d->pf_conn); f(d->pf_eid); f(d->pf_rows); f(d->pf_oinfo);
   f(d->pf_oboff); f(d->pf_oooff); f(d->pf_blk); f(d->pf_obase);
-  if (d->ev0) hipEventDestroy(d->ev0);
-  if (d->ev1) hipEventDestroy(d->ev1);
+  for (int t = 0; t < 3; ++t) f(d->adj_bid[t]);
+  f(d->adj_bmask); f(d->adj_boff); f(d->adj_inc_ptr); f(d->adj_inc); f(d->adj_xs);
+  for (hipEvent_t ev : {d->ev0, d->ev1, d->ev2, d->ev3})
+    if (ev) hipEventDestroy(ev);
   delete d;
 }
 
@@ -132,6 +135,8 @@ __global__ void plate_setup_kernel(const uint32_t* __restrict__ conn, const doub
 }
 
 std::string run_setup(Device* d, int type, size_t n, double rel_tol, double abs_tol, std::vector<uint16_t>& rzero) {
+  d->rel_tol = rel_tol;
+  d->abs_tol = abs_tol;
   rzero.assign(n, 0);
   if (n == 0) return "";
   auto e = dalloc(d->R[type], 9 * n);
@@ -151,6 +156,26 @@ std::string run_setup(Device* d, int type, size_t n, double rel_tol, double abs_
   }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpy(rzero.data(), d->rzero[type], n * sizeof(uint16_t), hipMemcpyDeviceToHost));
+  return "";
+}
+
+std::string replace_params(Device* d, int type, const double* src, double* host_copy, void* stream) {
+  const size_t n = d->n_elem[type] * PARAMS_PER[type];
+  if (!n) return "";
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemcpyAsync(d->params[type], src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipMemcpy(host_copy, d->params[type], n * sizeof(double), hipMemcpyDeviceToHost));
+  return "";
+}
+
+std::string replace_nodes(Device* d, const double* src, double* host_copy, void* stream) {
+  const size_t n = 3 * d->n_nodes;
+  if (!n) return "";
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  HIP_TRY(hipMemcpyAsync(d->xyz, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  HIP_TRY(hipMemcpy(host_copy, d->xyz, n * sizeof(double), hipMemcpyDeviceToHost));
   return "";
 }
 

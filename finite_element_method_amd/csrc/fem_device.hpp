@@ -84,6 +84,29 @@ std::string assemble_generic(Device* d, double* val, void* stream);
 std::string assemble_plate_fused(Device* d, double* val, void* stream);
 double last_ms(Device* d);
 
+// Adjoint maps (adjoint.hip)
+struct AdjointMaps {
+  std::vector<uint32_t> bid[3];   // per element node pair (p * nodes_per + q): global node-pair block
+  std::vector<uint64_t> bmask;    // per block: union structural mask (bit 6 i + j)
+  std::vector<uint16_t> boff;     // per block and row i: first entry of the block within the CSR row
+  std::vector<uint32_t> inc_ptr;  // N + 1
+  std::vector<uint32_t> inc;      // per node: its (element, node) gradient rows, element-insertion order
+  int64_t xoff[3] = {0, 0, 0};    // first gradient row of each element type (beams have none)
+  int64_t n_xs_rows = 0;
+};
+std::string upload_adjoint_maps(Device* d, const AdjointMaps& am);
+// mode 0: g = dL/dK (nnz values, CSR order); mode 1: g = nodal displacements
+// (6 N) and the outputs are -u_e^T (dK_e/dp) u_e.  dparam[t]: n_t x params_per
+// (or null); dxyz: N x 3 (or null; trusses and plates only).
+std::string run_adjoint(Device* d, int mode, const double* g, double* const dparam[3], double* dxyz, void* stream);
+double last_backward_ms(Device* d);
+
+// Device-to-device replacement of one element type's parameters or of the node
+// coordinates (then synchronous); the host mirrors are refreshed from the
+// device.  The caller reruns run_setup for the frames.
+std::string replace_params(Device* d, int type, const double* src, double* host_copy, void* stream);
+std::string replace_nodes(Device* d, const double* src, double* host_copy, void* stream);
+
 double* values(Device* d);
 const int64_t* row_ptr(Device* d);
 const int32_t* col_idx(Device* d);

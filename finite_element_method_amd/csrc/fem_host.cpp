@@ -97,6 +97,13 @@ struct fem_model {
   int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
   int pf_wg_per_cu = 2;  // fused plate: resident workgroups per CU looping over patches (0 = one per patch)
   bool force_generic = false;
+  // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
+  // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
+  std::vector<uint64_t> pat_bptr;
+  std::vector<uint32_t> pat_bnode;
+  std::vector<uint64_t> pat_bmask;
+  std::vector<uint16_t> pat_boff;
+  bool adj_valid = false;
 };
 
 namespace {
@@ -119,6 +126,7 @@ void clear_model(fem_model* m, uint32_t nodes_number) {
   m->seq = 0;
   m->dirty = true;
   m->pattern_valid = false;
+  m->adj_valid = false;
   m->nnz = 0;
   for (auto& r : m->rzero) r.clear();
 }
@@ -558,6 +566,15 @@ std::string build_pattern(fem_model* m) {
   e = femdev::upload_pattern(m->dev, n_rows, row_ptr.data(), nnz, col.data());
   if (!e.empty()) return e;
   m->nnz = nnz;
+  m->adj_valid = false;
+  m->pat_bptr = bptr;
+  m->pat_bnode.resize(blocks.size());
+  m->pat_bmask.resize(blocks.size());
+  for (size_t k = 0; k < blocks.size(); ++k) {
+    m->pat_bnode[k] = blocks[k].b;
+    m->pat_bmask[k] = blocks[k].mask;
+  }
+  m->pat_boff = boff;
   col.clear();
   col.shrink_to_fit();
   {
@@ -632,6 +649,99 @@ int ensure_ready(fem_model* m) {
   if (!m->pattern_valid) {
     auto e = build_pattern(m);
     if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  }
+  return FEM_OK;
+}
+
+// adjoint maps from the pattern's node-pair blocks, built on first use
+int ensure_adjoint(fem_model* m) {
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  if (m->adj_valid) return FEM_OK;
+  femdev::AdjointMaps am;
+  const size_t N = m->nodes_number;
+  for (int t = 0; t < 3; ++t) {
+    const int nn = femdev::NODES_PER[t];
+    const auto& S = m->set[t];
+    am.bid[t].resize(S.size() * nn * nn);
+    for (size_t e = 0; e < S.size(); ++e)
+      for (int p = 0; p < nn; ++p)
+        for (int q = 0; q < nn; ++q) {
+          const uint32_t a = S.conn[e * nn + p], b = S.conn[e * nn + q];
+          const uint32_t* k0 = m->pat_bnode.data() + m->pat_bptr[a];
+          const uint32_t* k1 = m->pat_bnode.data() + m->pat_bptr[a + 1];
+          const uint32_t* kb = std::lower_bound(k0, k1, b);
+          if (kb == k1 || *kb != b) return fail(m, FEM_ERR_STATE, "element node pair missing from the pattern");
+          am.bid[t][(e * nn + p) * nn + q] = uint32_t(kb - m->pat_bnode.data());
+        }
+  }
+  if (m->pat_bnode.size() >= (size_t(1) << 32)) return fail(m, FEM_ERR_ARGUMENT, "too many node-pair blocks");
+  am.bmask = m->pat_bmask;
+  am.boff = m->pat_boff;
+  // per node: (element, node) gradient rows of trusses and plates in insertion order
+  am.xoff[TRUSS] = 0;
+  am.xoff[BEAM] = int64_t(m->set[TRUSS].size()) * 2;
+  am.xoff[PLATE] = am.xoff[BEAM];
+  am.n_xs_rows = am.xoff[PLATE] + int64_t(m->set[PLATE].size()) * 4;
+  if (am.n_xs_rows >= (int64_t(1) << 32)) return fail(m, FEM_ERR_ARGUMENT, "too many element nodes");
+  struct ERef {
+    uint64_t seq;
+    int t;
+    uint32_t pos;
+  };
+  std::vector<ERef> order;
+  for (int t : {TRUSS, PLATE})
+    for (uint32_t i = 0; i < m->set[t].size(); ++i) order.push_back({m->set[t].seq[i], t, i});
+  std::sort(order.begin(), order.end(), [](const ERef& x, const ERef& y) { return x.seq < y.seq; });
+  am.inc_ptr.assign(N + 1, 0);
+  for (auto& r : order) {
+    const int nn = femdev::NODES_PER[r.t];
+    for (int p = 0; p < nn; ++p) am.inc_ptr[m->set[r.t].conn[size_t(r.pos) * nn + p] + 1]++;
+  }
+  for (size_t a = 0; a < N; ++a) am.inc_ptr[a + 1] += am.inc_ptr[a];
+  am.inc.resize(am.inc_ptr[N]);
+  {
+    std::vector<uint32_t> f(am.inc_ptr.begin(), am.inc_ptr.end() - 1);
+    for (auto& r : order) {
+      const int nn = femdev::NODES_PER[r.t];
+      for (int p = 0; p < nn; ++p)
+        am.inc[f[m->set[r.t].conn[size_t(r.pos) * nn + p]]++] = uint32_t(am.xoff[r.t] + int64_t(r.pos) * nn + p);
+    }
+  }
+  auto e = femdev::upload_adjoint_maps(m->dev, am);
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  m->adj_valid = true;
+  return FEM_OK;
+}
+
+int run_backward(fem_model* m, int mode, const double* g, double* dp_truss, double* dp_beam, double* dp_plate,
+                 double* dxyz, void* stream) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!g) return fail(m, FEM_ERR_ARGUMENT, mode == 0 ? "dL/dK values are required" : "displacements are required");
+  if (dxyz && m->set[BEAM].size())
+    return fail(m, FEM_ERR_ARGUMENT,
+                "node-coordinate gradients are not available for models with beam elements: the section roll "
+                "angle (beam.rs:185-202, acos of a sign-blind cosine) is not differentiable at its usual values 0 and pi");
+  int rc = ensure_adjoint(m);
+  if (rc) return rc;
+  double* dp[3] = {dp_truss, dp_beam, dp_plate};
+  auto e = femdev::run_adjoint(m->dev, mode, g, dp, dxyz, stream);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+// reruns the setup kernels after a parameter or coordinate change; a changed
+// frame zero pattern invalidates the sparsity pattern
+int rerun_setup(fem_model* m, bool all_types, int type) {
+  for (int t = 0; t < 3; ++t) {
+    if (!all_types && t != type) continue;
+    std::vector<uint16_t> rz;
+    auto e = femdev::run_setup(m->dev, t, m->set[t].size(), m->rel_tol, m->abs_tol, rz);
+    if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+    if (rz != m->rzero[t]) {
+      m->rzero[t] = rz;
+      m->pattern_valid = false;
+      m->adj_valid = false;
+    }
   }
   return FEM_OK;
 }
@@ -928,6 +1038,48 @@ int fem_csr_to_host(fem_model* m, int64_t* rp, int32_t* ci, double* v) {
 }
 
 double fem_last_assembly_ms(fem_model* m) { return m && m->dev ? femdev::last_ms(m->dev) : -1.0; }
+
+int fem_backward(fem_model* m, const double* dl_dk, double* dl_dparam_truss, double* dl_dparam_beam,
+                 double* dl_dparam_plate, double* dl_dxyz, void* stream) {
+  return run_backward(m, 0, dl_dk, dl_dparam_truss, dl_dparam_beam, dl_dparam_plate, dl_dxyz, stream);
+}
+
+int fem_compliance_sensitivity(fem_model* m, const double* u, double* s_truss, double* s_beam, double* s_plate,
+                               double* s_xyz, void* stream) {
+  return run_backward(m, 1, u, s_truss, s_beam, s_plate, s_xyz, stream);
+}
+
+int fem_model_size(const fem_model* m, size_t* n_nodes, size_t* n_truss, size_t* n_beam, size_t* n_plate) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (n_nodes) *n_nodes = m->node_number.size();
+  if (n_truss) *n_truss = m->set[TRUSS].size();
+  if (n_beam) *n_beam = m->set[BEAM].size();
+  if (n_plate) *n_plate = m->set[PLATE].size();
+  return FEM_OK;
+}
+
+double fem_last_backward_ms(fem_model* m) { return m && m->dev ? femdev::last_backward_ms(m->dev) : -1.0; }
+
+int fem_update_params(fem_model* m, int type, const double* params_device, void* stream) {
+  if (!m || type < 0 || type > 2 || (!params_device && m->set[type].size())) return FEM_ERR_ARGUMENT;
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  auto e = femdev::replace_params(m->dev, type, params_device, m->set[type].params.data(), stream);
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  return type == BEAM ? rerun_setup(m, false, BEAM) : FEM_OK;
+}
+
+int fem_update_nodes(fem_model* m, const double* xyz_device, void* stream) {
+  if (!m || (!xyz_device && m->node_number.size())) return FEM_ERR_ARGUMENT;
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  auto e = femdev::replace_nodes(m->dev, xyz_device, m->xyz.data(), stream);
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  m->coord_hash.clear();
+  for (size_t i = 0; i < m->node_number.size(); ++i)
+    m->coord_hash[coord_key(m->xyz[3 * i], m->xyz[3 * i + 1], m->xyz[3 * i + 2])].push_back(uint32_t(i));
+  return rerun_setup(m, true, 0);
+}
 
 const char* fem_assembly_path(fem_model* m) { return m ? m->path.c_str() : ""; }
 

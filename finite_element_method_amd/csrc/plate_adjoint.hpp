@@ -1,0 +1,288 @@
+// plate_adjoint.hpp -- reverse-mode derivative of a plate's contribution to a
+// scalar L(K) (the adjoint of plate.rs:160-724 as folded in element_math.hpp).
+//
+// Given the local-frame gradient Gl = T (dL/dK_e) T^T of one element, lane p
+// (row block of node p) produces its additive share of
+//   * P[7]   dL/d(cm00, cm01, cm22, cb00, cb01, cb22, cs): the constitutive
+//            constants, from which E, nu, t and k derivatives follow;
+//   * gxl[6] dL/d(x1, y1, x2, y2, x4, y4): local nodal coordinates at fixed frame;
+//   * hx, hy dL/d(omega_x, omega_y): an infinitesimal rotation of the element
+//            frame, dR = [omega]x R.  K_e does not depend on the in-plane gauge
+//            of the frame, so omega_z never contributes.
+// plate_adjoint_finish turns the sums of the four lanes into parameter and
+// world-frame nodal-coordinate gradients.  No K_e is stored: the forward
+// integrals are recomputed.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "element_math.hpp"
+
+namespace fem {
+
+// Reverse sweep of one integration point.  Adds to gxl the derivative with
+// respect to xl of
+//   l = sum_q w [dx_p (cxx_q dx_q + cxy_q dy_q) + dy_p (cyx_q dx_q + cyy_q dy_q)
+//               + al_p cA_q al_q + be_p cB_q be_q]
+// with c = {cxx[4], cxy[4], cyx[4], cyy[4], cA[4], cB[4]} held fixed, where
+// w, dx, dy, al, be are the per-point data of plate_point (w = det J).
+__host__ __device__ inline void plate_point_adjoint(const double xl[6], int ip, int p, const double c[24],
+                                                    double gxl[6]) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double g = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? g : -g;
+  const double s = (ip < 2) ? g : -g;
+  const double q4 = 0.25;
+  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
+  // Jacobian rows: (j00, j01) = us/4, (j10, j11) = ur/4 with
+  //   us = x1 (1+s) - x2 (1+s) + x4 (1-s),  ur = x1 (1+r) + x2 (1-r) - x4 (1+r)
+  const double usx = (x1 * ps - x2 * ps) + x4 * ms_, usy = (y1 * ps - y2 * ps) + y4 * ms_;
+  const double urx = (x1 * pr + x2 * mr_) - x4 * pr, ury = (y1 * pr + y2 * mr_) - y4 * pr;
+  const double j00 = usx * q4, j01 = usy * q4, j10 = urx * q4, j11 = ury * q4;
+  const double det = j00 * j11 - j01 * j10;
+  const double rdet = 1.0 / det;
+  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+  const double dr[4] = {q4 + q4 * s, -(q4 + q4 * s), -(q4 - q4 * s), q4 - q4 * s};
+  const double ds[4] = {q4 + q4 * r, q4 - q4 * r, -(q4 - q4 * r), -(q4 + q4 * r)};
+  double dx[4], dy[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    dx[a] = i00 * dr[a] + i01 * ds[a];
+    dy[a] = i10 * dr[a] + i11 * ds[a];
+  }
+  // MITC4 multipliers: mr = |ur| / (8 det), ms = |us| / (8 det) (plate.rs:424-447)
+  const double sqr = sqrt(urx * urx + ury * ury), sqs = sqrt(usx * usx + usy * usy);
+  const double m_r = sqr * 0.125 * rdet, m_s = sqs * 0.125 * rdet;
+  const double fa[4] = {ps, ps, ms_, ms_}, fb[4] = {pr, mr_, mr_, pr};
+  double al[4], be[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    al[a] = fa[a] * m_r;
+    be[a] = fb[a] * m_s;
+  }
+  const double dxp = sel4(dx[0], dx[1], dx[2], dx[3], p), dyp = sel4(dy[0], dy[1], dy[2], dy[3], p);
+  const double alp = sel4(al[0], al[1], al[2], al[3], p), bep = sel4(be[0], be[1], be[2], be[3], p);
+  double SX = 0.0, SY = 0.0, SA = 0.0, SB = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    SX += c[q] * dx[q] + c[4 + q] * dy[q];
+    SY += c[8 + q] * dx[q] + c[12 + q] * dy[q];
+    SA += c[16 + q] * al[q];
+    SB += c[20 + q] * be[q];
+  }
+  // adjoints of the point data
+  double g_det = ((dxp * SX + dyp * SY) + alp * SA) + bep * SB;
+  double g_i00 = 0.0, g_i01 = 0.0, g_i10 = 0.0, g_i11 = 0.0, g_mr = 0.0, g_ms = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const bool self = a == p;
+    const double gdx = det * ((dxp * c[a] + dyp * c[8 + a]) + (self ? SX : 0.0));
+    const double gdy = det * ((dxp * c[4 + a] + dyp * c[12 + a]) + (self ? SY : 0.0));
+    const double gal = det * (alp * c[16 + a] + (self ? SA : 0.0));
+    const double gbe = det * (bep * c[20 + a] + (self ? SB : 0.0));
+    g_i00 += gdx * dr[a];
+    g_i01 += gdx * ds[a];
+    g_i10 += gdy * dr[a];
+    g_i11 += gdy * ds[a];
+    g_mr += gal * fa[a];
+    g_ms += gbe * fb[a];
+  }
+  // m_r = sqr / (8 det), m_s = sqs / (8 det)
+  g_det -= (g_mr * m_r + g_ms * m_s) * rdet;
+  const double g_sqr = g_mr * 0.125 * rdet, g_sqs = g_ms * 0.125 * rdet;
+  // inverse Jacobian
+  double g_j11 = g_i00 * rdet, g_j01 = -g_i01 * rdet, g_j10 = -g_i10 * rdet, g_j00 = g_i11 * rdet;
+  g_det -= (((g_i00 * i00 + g_i01 * i01) + g_i10 * i10) + g_i11 * i11) * rdet;
+  // det = j00 j11 - j01 j10
+  g_j00 += g_det * j11;
+  g_j11 += g_det * j00;
+  g_j01 -= g_det * j10;
+  g_j10 -= g_det * j01;
+  // back to the linear forms us, ur (sqrt terms: d|v| = v/|v|)
+  const double kux = g_j00 * q4 + (sqs > 0.0 ? g_sqs * usx / sqs : 0.0);
+  const double kuy = g_j01 * q4 + (sqs > 0.0 ? g_sqs * usy / sqs : 0.0);
+  const double krx = g_j10 * q4 + (sqr > 0.0 ? g_sqr * urx / sqr : 0.0);
+  const double kry = g_j11 * q4 + (sqr > 0.0 ? g_sqr * ury / sqr : 0.0);
+  gxl[0] += kux * ps + krx * pr;
+  gxl[1] += kuy * ps + kry * pr;
+  gxl[2] += -kux * ps + krx * mr_;
+  gxl[3] += -kuy * ps + kry * mr_;
+  gxl[4] += kux * ms_ - krx * pr;
+  gxl[5] += kuy * ms_ - kry * pr;
+}
+
+// adjoint of the shear vectors of node a (components 1 and 2; component 0 is
+// the constant +-1/2): gv = dL/dv_a, gu = dL/du_a, mapped onto xl through
+//   e12 = (-(y1-y2), x1-x2)/4, e43 = (-y4, x4)/4, e14 = (-(y1-y4), x1-x4)/4, e23 = (-y2, x2)/4
+__host__ __device__ inline void plate_shear_vector_adjoint(int a, double gv1, double gv2, double gu1, double gu2,
+                                                           double gxl[6]) {
+  const double q = 0.25;
+  if (a <= 1) {  // v_a carries e12
+    gxl[1] -= gv1 * q; gxl[3] += gv1 * q;
+    gxl[0] += gv2 * q; gxl[2] -= gv2 * q;
+  } else {       // e43
+    gxl[5] -= gv1 * q;
+    gxl[4] += gv2 * q;
+  }
+  if (a == 0 || a == 3) {  // u_a carries e14
+    gxl[1] -= gu1 * q; gxl[5] += gu1 * q;
+    gxl[0] += gu2 * q; gxl[4] -= gu2 * q;
+  } else {                 // e23
+    gxl[3] -= gu1 * q;
+    gxl[2] += gu2 * q;
+  }
+}
+
+// h += antisymmetric parts (omega_x, omega_y) of K G^T + K^T G for one pair of
+// 3x3 sub-blocks K = k[r0.., c0..], G = gl[r0.., c0..] of 6x6 row-major blocks
+__host__ __device__ inline void plate_frame_adjoint_sub(const double* k, const double* gl, int r0, int c0, double& hx,
+                                                        double& hy) {
+#pragma unroll
+  for (int cc = 0; cc < 3; ++cc) {
+    const double k0c = k[(r0 + 0) * 6 + c0 + cc], k1c = k[(r0 + 1) * 6 + c0 + cc], k2c = k[(r0 + 2) * 6 + c0 + cc];
+    const double g0c = gl[(r0 + 0) * 6 + c0 + cc], g1c = gl[(r0 + 1) * 6 + c0 + cc], g2c = gl[(r0 + 2) * 6 + c0 + cc];
+    const double kc0 = k[(r0 + cc) * 6 + c0 + 0], kc1 = k[(r0 + cc) * 6 + c0 + 1], kc2 = k[(r0 + cc) * 6 + c0 + 2];
+    const double gc0 = gl[(r0 + cc) * 6 + c0 + 0], gc1 = gl[(r0 + cc) * 6 + c0 + 1], gc2 = gl[(r0 + cc) * 6 + c0 + 2];
+    hx += (k2c * g1c - k1c * g2c) + (kc2 * gc1 - kc1 * gc2);
+    hy += (k0c * g2c - k2c * g0c) + (kc0 * gc2 - kc2 * gc0);
+  }
+}
+
+// Per-lane share of the adjoint sums for block q of row p.  acc: the row's
+// integrals (plate_row_accumulate), sv: shear vectors, gl: local-frame
+// gradient block (p, q), 6x6 row-major.  c receives the integral coefficients
+// of column q (layout of plate_point_adjoint).
+__host__ __device__ inline void plate_block_adjoint(const double acc[24], const PlateMat& m, const PlateShear& sv,
+                                                    int p, int q, const double gl[36], double P[7], double c[24],
+                                                    double gxl[6], double& hx, double& hy) {
+  const double Mxx = acc[q], Mxy = acc[4 + q], Myx = acc[8 + q], Myy = acc[12 + q], A = acc[16 + q], B = acc[20 + q];
+  const double Guu = gl[0], Guv = gl[1], Gvu = gl[6], Gvv = gl[7];
+  const double Gxx = gl[21], Gxy = gl[22], Gyx = gl[27], Gyy = gl[28];
+  // constitutive partials
+  P[0] += Guu * Mxx + Gvv * Myy;
+  P[1] += Guv * Mxy + Gvu * Myx;
+  P[2] += ((Guu * Myy + Guv * Myx) + Gvu * Mxy) + Gvv * Mxx;
+  P[3] += Gxx * Myy + Gyy * Mxx;
+  P[4] -= Gxy * Myx + Gyx * Mxy;
+  P[5] += ((Gxx * Mxx - Gxy * Mxy) - Gyx * Myx) + Gyy * Myy;
+  // shear: Gs = gl[2..4][2..4]; v_p^T Gs v_q, u_p^T Gs u_q and the one-sided products
+  double vp[3], up[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vp[i] = sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], p);
+    up[i] = sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], p);
+  }
+  double Gv_q[3], Gu_q[3], vG[3], uG[3];  // Gs v_q, Gs u_q, v_p^T Gs, u_p^T Gs
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Gv_q[i] = 0.0; Gu_q[i] = 0.0; vG[i] = 0.0; uG[i] = 0.0;
+  }
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      const double gs = gl[(2 + i) * 6 + 2 + j];
+      Gv_q[i] += gs * sv.v[q][j];
+      Gu_q[i] += gs * sv.u[q][j];
+      vG[j] += vp[i] * gs;
+      uG[j] += up[i] * gs;
+    }
+  const double sv_ = (vp[0] * Gv_q[0] + vp[1] * Gv_q[1]) + vp[2] * Gv_q[2];
+  const double su_ = (up[0] * Gu_q[0] + up[1] * Gu_q[1]) + up[2] * Gu_q[2];
+  P[6] += A * sv_ + B * su_;
+  // integral coefficients of column q
+  c[q] = ((m.cm00 * Guu + m.cm22 * Gvv) + m.cb22 * Gxx) + m.cb00 * Gyy;
+  c[4 + q] = ((m.cm01 * Guv + m.cm22 * Gvu) - m.cb22 * Gxy) - m.cb01 * Gyx;
+  c[8 + q] = ((m.cm22 * Guv + m.cm01 * Gvu) - m.cb01 * Gxy) - m.cb22 * Gyx;
+  c[12 + q] = ((m.cm22 * Guu + m.cm00 * Gvv) + m.cb00 * Gxx) + m.cb22 * Gyy;
+  c[16 + q] = m.cs * sv_;
+  c[20 + q] = m.cs * su_;
+  // shear vectors: d/dv_p = cs A Gs v_q, d/dv_q = cs A v_p^T Gs (same with u, B)
+  const double sa = m.cs * A, sb = m.cs * B;
+  plate_shear_vector_adjoint(p, sa * Gv_q[1], sa * Gv_q[2], sb * Gu_q[1], sb * Gu_q[2], gxl);
+  plate_shear_vector_adjoint(q, sa * vG[1], sa * vG[2], sb * uG[1], sb * uG[2], gxl);
+  // frame rotation: needs the local block values
+  double kb[36];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) kb[i] = 0.0;
+  double o[13];
+  double vps[3], ups[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vps[i] = m.cs * vp[i];
+    ups[i] = m.cs * up[i];
+  }
+  plate_block_entries(Mxx, Mxy, Myx, Myy, A, B, m, vps, ups, sv.v[q], sv.u[q], o);
+  kb[0] = o[0]; kb[1] = o[1]; kb[6] = o[2]; kb[7] = o[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) kb[(2 + i) * 6 + 2 + j] = o[4 + 3 * i + j];
+  if (p == q) kb[35] = KROT6;
+  plate_frame_adjoint_sub(kb, gl, 0, 0, hx, hy);
+  plate_frame_adjoint_sub(kb, gl, 0, 3, hx, hy);
+  plate_frame_adjoint_sub(kb, gl, 3, 0, hx, hy);
+  plate_frame_adjoint_sub(kb, gl, 3, 3, hx, hy);
+}
+
+// Element-level finish from the four lanes' sums.
+//   loc[a][k]: local coordinates R (X_a - X_3) of the crate's nodes 1..4 (a = 0..3)
+//   grad_param = dL/d(E, nu, t, k); gX[a][k] = dL/dX_a (world frame)
+__host__ __device__ inline void plate_adjoint_finish(const double P[7], const double gxl_in[6], double hx, double hy,
+                                                     const double loc[4][3], const double R[9], double E, double nu,
+                                                     double t, double k, double grad_param[4], double gX[4][3]) {
+  // parameters: cm = E t/(1-nu^2), cb = E t^3/(12 (1-nu^2)), cs = E t k/(2 (1+nu))
+  const double cm = E * t / (1.0 - nu * nu);
+  const double cb = E * (t * t * t) / (12.0 * (1.0 - nu * nu));
+  const double cs = E * t * k / (2.0 * (1.0 + nu));
+  const double h = (1.0 - nu) / 2.0;
+  const double Lm = cm * ((P[0] + nu * P[1]) + h * P[2]);
+  const double Lb = cb * ((P[3] + nu * P[4]) + h * P[5]);
+  const double Ls = cs * P[6];
+  const double dcm = cm * 2.0 * nu / (1.0 - nu * nu), dcb = cb * 2.0 * nu / (1.0 - nu * nu);
+  grad_param[0] = ((Lm + Lb) + Ls) / E;
+  grad_param[1] = (((P[0] * dcm + P[1] * (cm + nu * dcm)) + P[2] * (h * dcm - 0.5 * cm)) +
+                   ((P[3] * dcb + P[4] * (cb + nu * dcb)) + P[5] * (h * dcb - 0.5 * cb))) -
+                  P[6] * cs / (1.0 + nu);
+  grad_param[2] = ((Lm + 3.0 * Lb) + Ls) / t;
+  grad_param[3] = Ls / k;
+  // frame rotation: local z of nodes 1, 2, 4 feed back through dxl = (omega x p)_{x,y}
+  double gxl[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gxl[i] = gxl_in[i];
+  const int nidx[3] = {0, 1, 3};
+  double gwx = hx, gwy = hy;
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const double z = loc[nidx[m]][2];
+    gwx -= gxl[2 * m + 1] * z;
+    gwy += gxl[2 * m] * z;
+  }
+  // omega from the normal n = (X4 - X3) x (X2 - X3): omega_x = n~_y/|n|, omega_y = -n~_x/|n|
+  // with n~ = R dn = da41 x p2 + p4 x da21 (local frame)
+  const double* p2 = loc[1];
+  const double* p4 = loc[3];
+  const double nx = p4[1] * p2[2] - p4[2] * p2[1], ny = p4[2] * p2[0] - p4[0] * p2[2], nz = p4[0] * p2[1] - p4[1] * p2[0];
+  const double nn = sqrt((nx * nx + ny * ny) + nz * nz);
+  const double gnx = -gwy / nn, gny = gwx / nn;
+  double ga[4][3];
+  ga[0][0] = gxl[0]; ga[0][1] = gxl[1]; ga[0][2] = 0.0;
+  ga[1][0] = gxl[2]; ga[1][1] = gxl[3]; ga[1][2] = 0.0;
+  ga[3][0] = gxl[4]; ga[3][1] = gxl[5]; ga[3][2] = 0.0;
+  // d n~_x = da41_y z2 - da41_z y2 + y4 da21_z - z4 da21_y
+  // d n~_y = da41_z x2 - da41_x z2 + z4 da21_x - x4 da21_z
+  const double g41[3] = {-gny * p2[2], gnx * p2[2], -gnx * p2[1] + gny * p2[0]};
+  const double g21[3] = {gny * p4[2], -gnx * p4[2], gnx * p4[1] - gny * p4[0]};
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    ga[3][i] += g41[i];
+    ga[1][i] += g21[i];
+    ga[2][i] = -((ga[0][i] + ga[1][i]) + ga[3][i]);
+  }
+  // world frame: dL/dX = R^T dL/da
+#pragma unroll
+  for (int a = 0; a < 4; ++a)
+#pragma unroll
+    for (int i = 0; i < 3; ++i) gX[a][i] = (R[0 * 3 + i] * ga[a][0] + R[1 * 3 + i] * ga[a][1]) + R[2 * 3 + i] * ga[a][2];
+}
+
+}  // namespace fem

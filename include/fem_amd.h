@@ -73,6 +73,10 @@ int fem_add_beams(fem_model* m, size_t n, const uint32_t* numbers, const uint32_
 int fem_add_plates(fem_model* m, size_t n, const uint32_t* numbers, const uint32_t* conn, const double* props,
                    size_t* n_ok);
 
+/* Counts of committed nodes and elements (the crate keeps them in its HashMaps,
+ * fem.rs:15-32; any pointer may be NULL). */
+int fem_model_size(const fem_model* m, size_t* n_nodes, size_t* n_truss, size_t* n_beam, size_t* n_plate);
+
 /* FEM::get_{truss,beam,plate}_rotation_matrix_elements(number) -> [V; 9]
  *                                                        src/fem/fem.rs:171-202
  * Computed on the device; row-major. */
@@ -107,8 +111,45 @@ int fem_csr_to_host(fem_model* m, int64_t* row_ptr, int32_t* col_idx, double* va
  * HIP events around the value kernels only (ms); -1 when unavailable. */
 double fem_last_assembly_ms(fem_model* m);
 
+/* ---- differentiation (no counterpart in the crate) ---------------------
+ *
+ * Both entries recompute each element's stiffness on the device (nothing is
+ * stored by the forward) and are deterministic (no atomics).  All pointers are
+ * device memory; every output pointer may be NULL.  Asynchronous on `stream`.
+ *   dl_dparam_truss: n_truss x 3 = dL/d(E, A, A2)   (A2 slot 0 when A2 is None)
+ *   dl_dparam_beam:  n_beam x 11 = dL/d(E, nu, A, I11, I22, I12, It, k, v1x, v1y, v1z);
+ *                    I11/I22 are given when I12 == 0 and I11 != I22, NaN otherwise;
+ *                    I12 and v1 (they rotate the section) are NaN
+ *   dl_dparam_plate: n_plate x 4 = dL/d(E, nu, thickness, shear_factor)
+ *   dl_dxyz:         N x 3 node coordinates, node insertion order; FEM_ERR_ARGUMENT
+ *                    for models with beams (the crate's beam roll angle, beam.rs:185-202,
+ *                    is acos of a sign-blind cosine: not differentiable at roll 0 or pi)
+ * Element stiffness definitions: truss.rs:95-189, beam.rs:260-605, plate.rs:160-724,
+ * frames truss.rs:66-93, beam.rs:162-258, quadrilateral_4n_element_functions.rs:131-171. */
+
+/* Backward of the assembly: dl_dk is dL/dK on the CSR pattern (nnz doubles). */
+int fem_backward(fem_model* m, const double* dl_dk, double* dl_dparam_truss, double* dl_dparam_beam,
+                 double* dl_dparam_plate, double* dl_dxyz, void* stream);
+
+/* Compliance sensitivities: with nodal displacements u (6N doubles, dof order of
+ * the global matrix) every output receives sum_e -u_e^T (dK_e/dp) u_e. */
+int fem_compliance_sensitivity(fem_model* m, const double* u, double* s_truss, double* s_beam, double* s_plate,
+                               double* s_xyz, void* stream);
+
+/* Timing of the last backward / sensitivity call (HIP events, ms); -1 when unavailable. */
+double fem_last_backward_ms(fem_model* m);
+
+/* Replace one element type's parameters (device memory, same layout as the
+ * fem_add_* props) or all node coordinates (device memory, N x 3).  Values are
+ * not re-validated.  Frames are recomputed; if a frame's zero pattern changes
+ * the sparsity pattern is rebuilt by the next assembly. */
+int fem_update_params(fem_model* m, int type, const double* params_device, void* stream);
+int fem_update_nodes(fem_model* m, const double* xyz_device, void* stream);
+
 /* Tuning options (no counterpart in the crate; they change speed, never values):
  *   "plate_patch_nodes"  owned nodes per workgroup of the fused plate kernel: 4..64
+ *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernel,
+ *                        looping over the patches: 0..64 (0 = one workgroup per patch)
  *   "force_generic"      1 = always use the generic value path
  * Setting an option invalidates the pattern (rebuilt by the next assembly). */
 int fem_set_option(fem_model* m, const char* key, int64_t value);

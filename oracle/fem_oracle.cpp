@@ -440,8 +440,9 @@ bool beam_rotation(const Vec3<V>& d, const Vec3<V>& v1, V angle, V rel_tol, V ab
 }
 
 // src/fem/structs/beam.rs:260-605 (one integration point r = 0, alpha = 2)
+// parts (test hook): bit t keeps the rank-one term t (u, v, w, tu, tv, tw)
 template <class V>
-std::vector<V> beam_local_k(const Beam<V>& e, V length) {
+std::vector<V> beam_local_k(const Beam<V>& e, V length, unsigned parts = 63) {
   const int n = 12;
   V r = V(0), alpha = V(2);
   V inv_j = V(1) / bar_jacobian(length, r);
@@ -467,6 +468,8 @@ std::vector<V> beam_local_k(const Beam<V>& e, V length) {
   std::vector<V> btw = row({{5, bar_dh1(r)}, {11, bar_dh2(r)}}, {}, true);
   V G = e.E / (V(2) * (V(1) + e.nu));
   V cs[6] = {e.A * e.E, G * e.A * e.k, G * e.A * e.k, G * e.it, e.E * e.i22p, e.E * e.i11p};
+  for (int t = 0; t < 6; ++t)
+    if (!((parts >> t) & 1u)) cs[t] = V(0);
   const std::vector<V>* Bs[6] = {&bu, &bv, &bw, &btu, &btv, &btw};
   std::vector<V> kip(n * n, V(0));
   for (int t = 0; t < 6; ++t) {
@@ -689,8 +692,10 @@ void quad_jacobian(const QuadLocal<V>& q, V r, V s, V J[4]) {
 
 // ---------------------------------------------------------------- plate
 // src/fem/structs/plate.rs:160-724
+// parts (test hook for reference gradients): bit 0 membrane, 1 bending,
+// 2 shear, 3 drilling; the default sums all four exactly as the crate does
 template <class V>
-std::vector<V> plate_local_k(const Model<V>& m, const Plate<V>& e) {
+std::vector<V> plate_local_k(const Model<V>& m, const Plate<V>& e, unsigned parts = 15) {
   const int n = 24;
   QuadLocal<V> q = quad_local(m, e);
   V ips[4][2] = {{e.ip_rs[0], e.ip_rs[0]}, {e.ip_rs[1], e.ip_rs[0]}, {e.ip_rs[1], e.ip_rs[1]}, {e.ip_rs[0], e.ip_rs[1]}};
@@ -772,11 +777,14 @@ std::vector<V> plate_local_k(const Model<V>& m, const Plate<V>& e) {
     auto km = term(bm, cm, 3), kb = term(bb, cb, 3), ks = term(bs, cs, 2);
     V w = det * alpha;
     for (int i = 0; i < n * n; ++i) {
-      V kip = (km[i] * w + kb[i] * w) + ks[i] * w;
+      V kip = parts == 15 ? (km[i] * w + kb[i] * w) + ks[i] * w
+                          : ((parts & 1 ? km[i] * w : V(0)) + (parts & 2 ? kb[i] * w : V(0))) +
+                                (parts & 4 ? ks[i] * w : V(0));
       k[i] = k[i] + kip;
     }
   }
-  for (int i = 0; i < 4; ++i) k[(6 * i + 5) * n + 6 * i + 5] += V(1.0f);  // KROT6, plate.rs:716-721
+  if (parts & 8)
+    for (int i = 0; i < 4; ++i) k[(6 * i + 5) * n + 6 * i + 5] += V(1.0f);  // KROT6, plate.rs:716-721
   return k;
 }
 std::vector<uint8_t> plate_local_pattern() {
@@ -1117,6 +1125,51 @@ int om_element_matrix(AnyModel* am, int type, uint32_t number, int frame, double
     return n;
   };
   return am->f32 ? get(*am->f) : get(*am->d);
+}
+
+// Global-frame K_e of one element from explicit node coordinates (nodes_per x 3)
+// and parameters (the fem_add_* layouts), computed as add_* does after its
+// validation (no validation here).  `parts` selects stiffness terms (see
+// plate_local_k / beam_local_k; ignored for trusses).  Writes n*n doubles and
+// returns n (6, 12, 24), or -1 when no frame can be formed.  Used by the tests'
+// reference gradients (exact part splits and finite differences).
+int om_eval_element(int type, const double* xyz, const double* params, double rel_tol, double abs_tol, unsigned parts,
+                    double* out) {
+  const int nn = type == 2 ? 4 : 2;
+  Model<double> m{rel_tol, abs_tol, uint32_t(nn)};
+  for (int a = 0; a < nn; ++a)
+    if (!add_node<double>(m, uint32_t(a + 1), xyz[3 * a], xyz[3 * a + 1], xyz[3 * a + 2])) return -1;
+  std::vector<double> K;
+  std::string err;
+  int n;
+  if (type == 0) {
+    Truss<double> e{1, 2, params[0], params[1], !std::isnan(params[2]), params[2], {}};
+    auto d = element_vector(m, 1, 2);
+    double L = norm3(d);
+    if (!rotation_to_align(d, Vec3<double>{L, 0.0, 0.0}, abs_tol, e.R, err)) return -1;
+    n = 6;
+    K = rotate_tkt(truss_local_k(e, L), e.R, n);
+  } else if (type == 1) {
+    const double* p = params;
+    Beam<double> e{1, 2, p[0], p[1], p[2], 0.0, 0.0, p[6], p[7], {}};
+    double angle;
+    principal_moments(p[3], p[4], p[5], rel_tol, e.i11p, e.i22p, angle);
+    auto d = element_vector(m, 1, 2);
+    if (!beam_rotation(d, Vec3<double>{p[8], p[9], p[10]}, angle, rel_tol, abs_tol, e.R, err)) return -1;
+    n = 12;
+    K = rotate_tkt(beam_local_k(e, norm3(d), parts), e.R, n);
+  } else {
+    Plate<double> e;
+    for (int i = 0; i < 4; ++i) e.n[i] = uint32_t(i + 1);
+    e.E = params[0]; e.nu = params[1]; e.t = params[2]; e.k = params[3];
+    if (!quad_rotation(coords(m, 2), coords(m, 3), coords(m, 4), abs_tol, e.R, err)) return -1;
+    double g = std::sqrt(double(1.0f / 3.0f)) * 1.0;
+    e.ip_rs = {g, std::sqrt(double(1.0f / 3.0f)) * -1.0, 0.0, 0.0};
+    n = 24;
+    K = rotate_tkt(plate_local_k(m, e, parts), e.R, n);
+  }
+  for (int i = 0; i < n * n; ++i) out[i] = K[i];
+  return n;
 }
 
 // principal moments of inertia (beam.rs:119-160): out = [i11_p, i22_p, angle]

@@ -59,6 +59,8 @@ def lib():
         L.om_structural_violations.argtypes = [vp]
         L.om_nonstructural_max_rel.restype = ctypes.c_double
         L.om_nonstructural_max_rel.argtypes = [vp]
+        L.om_eval_element.restype = ctypes.c_int
+        L.om_eval_element.argtypes = [ctypes.c_int, vp, vp, dbl, dbl, ctypes.c_uint, vp]
         _lib = L
     return _lib
 
@@ -71,6 +73,26 @@ def rust_debug(x, f32=False):
     buf = ctypes.create_string_buffer(64)
     lib().om_rust_debug(float(x), int(f32), buf, 64)
     return buf.value.decode()
+
+
+PLATE_MEMBRANE, PLATE_BENDING, PLATE_SHEAR, PLATE_DRILLING = 1, 2, 4, 8
+
+
+def eval_element(etype, xyz, params, rel_tol=1e-4, abs_tol=1e-12, parts=None):
+    """Global-frame K_e (n x n) of one element from explicit node coordinates
+    (nodes_per x 3) and parameters, as add_* computes it (no validation).
+
+    parts: plates, a mask of PLATE_*; beams, bit t keeps rank-one term t
+    (u, v, w, theta_u, theta_v, theta_w); None = every term."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    params = np.ascontiguousarray(params, dtype=np.float64)
+    if parts is None:
+        parts = 15 if etype == 2 else 63
+    out = np.zeros(24 * 24)
+    n = lib().om_eval_element(int(etype), _ptr(xyz), _ptr(params), float(rel_tol), float(abs_tol), int(parts), _ptr(out))
+    if n < 0:
+        raise OracleError("no element frame for these coordinates")
+    return out[: n * n].reshape(n, n)
 
 
 def principal_moments(i11, i22, i12, rel_tol):
