@@ -21,6 +21,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 elements assembled/s + fwd+bwd sensitivities/s; % of HBM roofline vs CPU"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per launch of the forward value kernel, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE counter runs of this script (scaled by a 1 GiB copy kernel that reads
+# FETCH_SIZE = 0.5 x its bytes and WRITE_SIZE = 1.0 x)
+MEASURED_TRAFFIC = {("plate_512", "plate_fused"): 333.1e6}
 
 WORKLOADS = {
     "plate_512": ("plate_grid", dict(nx=512, seed=0)),
@@ -36,6 +40,61 @@ def algorithmic_bytes(mesh, nnz):
     nn = mesh.conn.shape[1]
     n_params = {"truss": 3, "beam": 11, "plate": 4}[mesh.kind]
     return 8 * nnz + 24 * mesh.n_nodes + 4 * nn * mesh.n_elements + 8 * n_params * mesh.n_elements
+
+
+def backward_bytes(mesh, nnz, compliance=False):
+    """Compulsory HBM bytes of one backward: dL/dK (or the 6N displacements)
+    read once, geometry and parameters read once, parameter and node-coordinate
+    gradients written once."""
+    nn = mesh.conn.shape[1]
+    n_params = {"truss": 3, "beam": 11, "plate": 4}[mesh.kind]
+    src = 48 * mesh.n_nodes if compliance else 8 * nnz
+    return src + 24 * mesh.n_nodes + 4 * nn * mesh.n_elements + 16 * n_params * mesh.n_elements + 24 * mesh.n_nodes
+
+
+def time_steps(fn, steps, warmup, stream):
+    import torch
+
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    for _ in range(steps):
+        fn()
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    return ev0.elapsed_time(ev1) / steps
+
+
+def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream):
+    """Backward from dL/dK ~ N(0, 1) on the pattern, compliance sensitivities
+    with u ~ N(0, 1e-3), and forward + backward, all on the same mesh."""
+    import torch
+
+    gen = torch.Generator(device="cuda").manual_seed(0)
+    g = torch.randn(nnz, dtype=torch.float64, device="cuda", generator=gen)
+    u = 1e-3 * torch.randn(6 * mesh.n_nodes, dtype=torch.float64, device="cuda", generator=gen)
+    xyz = mesh.kind != "beam"
+    bwd_ms = time_steps(lambda: model.backward(g, xyz=xyz, stream=stream), steps, warmup, stream)
+    comp_ms = time_steps(lambda: model.compliance_sensitivity(u, xyz=xyz, stream=stream), steps, warmup, stream)
+
+    def fwd_bwd():
+        model.assemble_into(val, stream)
+        model.backward(g, xyz=xyz, stream=stream)
+
+    both_ms = time_steps(fwd_bwd, steps, warmup, stream)
+    bb = backward_bytes(mesh, nnz)
+    cb = backward_bytes(mesh, nnz, compliance=True)
+    return {"bwd_ms": bwd_ms, "compliance_ms": comp_ms, "fwd_plus_bwd_ms": both_ms,
+            "fwd_plus_bwd_over_fwd": both_ms / fwd_ms,
+            "sensitivities_per_s": mesh.n_elements / (both_ms * 1e-3),
+            "bwd_roofline": {"bound": "hbm", "achieved": bb / (bwd_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                             "unit": "GB/s", "frac": bb / (bwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                             "algorithmic_bytes_per_launch": bb},
+            "compliance_roofline": {"bound": "hbm", "achieved": cb / (comp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                                    "unit": "GB/s", "frac": cb / (comp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                                    "algorithmic_bytes_per_launch": cb}}
 
 
 def cpu_baseline(kind):
@@ -69,6 +128,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="plate_512", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-grad", action="store_true", help="skip the backward / sensitivity timings")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="run one elementwise copy of this many MiB first (a known-traffic kernel that "
+                         "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--patch-nodes", type=int, default=0, help="fused plate: owned nodes per patch (0 = default)")
     ap.add_argument("--wg-per-cu", type=int, default=-1, help="fused plate: resident workgroups per CU (0 = one per patch)")
     args = ap.parse_args()
@@ -81,6 +144,13 @@ def main():
 
     from finite_element_method_amd import FEM, meshgen
 
+    if args.copy_probe_mib:
+        n = args.copy_probe_mib * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda:0")
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     gen, kw = WORKLOADS[args.workload]
     mesh = getattr(meshgen, gen)(**kw)
     model = FEM.from_mesh(mesh)
@@ -130,11 +200,14 @@ def main():
         "config": {"workload": args.workload, "elements": mesh.n_elements, "nodes": mesh.n_nodes,
                    "dof": int(n_rows), "nnz": int(nnz), "seed": 0, "path": model.assembly_path()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": MEASURED_TRAFFIC.get((args.workload, model.assembly_path())),
                      "algorithmic_bytes_per_launch": alg},
         "setup_s": setup_s,
         "wall_s": wall,
     }
+    if not args.no_grad:
+        out["sensitivity"] = sensitivity(model, mesh, nnz, val, ms, args.steps, args.warmup, stream)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh.kind)
     print(json.dumps(out), flush=True)
