@@ -19,7 +19,7 @@ import os
 
 import numpy as np
 
-__all__ = ["FEM", "FemError", "DeviceError", "TRUSS", "BEAM", "PLATE", "lib"]
+__all__ = ["FEM", "FEMBatch", "FemError", "DeviceError", "TRUSS", "BEAM", "PLATE", "lib"]
 
 TRUSS, BEAM, PLATE = 0, 1, 2
 PARAMS_PER = {TRUSS: 3, BEAM: 11, PLATE: 4}
@@ -73,6 +73,18 @@ def lib():
         "fem_last_backward_ms": (dbl, [vp]),
         "fem_update_params": (ci, [vp, ci, vp, vp]),
         "fem_update_nodes": (ci, [vp, vp, vp]),
+        "fem_batch_create": (ci, [vp, sz, ctypes.POINTER(vp)]),
+        "fem_batch_destroy": (None, [vp]),
+        "fem_batch_last_error": (ctypes.c_char_p, [vp]),
+        "fem_batch_size": (ci, [vp, psz, psz]),
+        "fem_batch_autotune": (ci, [vp]),
+        "fem_batch_assemble": (ci, [vp, vp]),
+        "fem_batch_last_ms": (dbl, [vp]),
+        "fem_batch_config": (ctypes.c_char_p, [vp]),
+        "fem_batch_mesh_csr": (ci, [vp, sz, ctypes.POINTER(i64), ctypes.POINTER(i64), ctypes.POINTER(vp),
+                                    ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64),
+                                    ctypes.POINTER(i64)]),
+        "fem_batch_mesh_csr_to_host": (ci, [vp, sz, vp, vp, vp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -339,3 +351,65 @@ class FEM:
     @classmethod
     def from_mesh(cls, mesh):
         return cls(mesh.rel_tol, mesh.abs_tol, mesh.n_nodes).load_mesh(mesh)
+
+
+class FEMBatch:
+    """Independent meshes assembled by one hipGraph launch (fem_batch_* in
+    include/fem_amd.h).  The models' nodes and elements are copied at
+    construction; each mesh keeps its own CSR."""
+
+    def __init__(self, models):
+        arr = (ctypes.c_void_p * len(models))(*[m._h.value for m in models])
+        h = ctypes.c_void_p()
+        rc = lib().fem_batch_create(arr, len(models), ctypes.byref(h))
+        if rc != FEM_OK:
+            raise DeviceError("fem_batch_create failed (%d)" % rc)
+        self._h = h
+        self.n_meshes = len(models)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and _lib is not None:
+            _lib.fem_batch_destroy(h)
+            self._h = None
+
+    def _call(self, rc):
+        if rc != FEM_OK:
+            raise DeviceError(lib().fem_batch_last_error(self._h).decode())
+
+    def autotune(self):
+        """Times candidate launch shapes, keeps the fastest, captures the graph."""
+        self._call(lib().fem_batch_autotune(self._h))
+        return self.config()
+
+    def assemble(self, stream=None):
+        """One graph launch assembling every mesh (asynchronous on `stream`)."""
+        self._call(lib().fem_batch_assemble(self._h, _stream_handle(stream)))
+
+    def last_ms(self):
+        return lib().fem_batch_last_ms(self._h)
+
+    def config(self):
+        return lib().fem_batch_config(self._h).decode()
+
+    def groups(self):
+        n, g = ctypes.c_size_t(), ctypes.c_size_t()
+        self._call(lib().fem_batch_size(self._h, ctypes.byref(n), ctypes.byref(g)))
+        return g.value
+
+    def mesh_info(self, i):
+        """(n_rows, nnz, val_device_ptr) of mesh i."""
+        nr, nz, vb, cb = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        rp, ci, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._call(lib().fem_batch_mesh_csr(self._h, int(i), ctypes.byref(nr), ctypes.byref(nz), ctypes.byref(rp),
+                                            ctypes.byref(ci), ctypes.byref(v), ctypes.byref(vb), ctypes.byref(cb)))
+        return nr.value, nz.value, v.value
+
+    def mesh_csr(self, i):
+        """Mesh i's CSR on the host (row_ptr int64, col_idx int32, val float64)."""
+        n_rows, nnz, _ = self.mesh_info(i)
+        rp = np.zeros(n_rows + 1, dtype=np.int64)
+        ci = np.zeros(nnz, dtype=np.int32)
+        v = np.zeros(nnz, dtype=np.float64)
+        self._call(lib().fem_batch_mesh_csr_to_host(self._h, int(i), _ptr(rp), _ptr(ci), _ptr(v)))
+        return rp, ci, v

@@ -56,17 +56,47 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
   return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + j;
 }
 
-// global-frame 6x6 gradient block of node pair block k, rows of node a
+// global-frame 6x6 gradient block of node pair block k, rows of node a.  The
+// block's entries of row i are contiguous in the CSR row: load the run, then
+// place it by the row mask.
 __device__ inline void gather_block(const AdjArgs& A, uint32_t k, uint32_t a, double G[36]) {
   const uint64_t um = A.bmask[k];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     const uint32_t mi = uint32_t(um >> (6 * i)) & 63u;
-    const int64_t rs = A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i];
+    const int n = __builtin_popcount(mi);
+    const double* src = A.g + (A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i]);
+    double v[6];
 #pragma unroll
-    for (int j = 0; j < 6; ++j)
-      G[i * 6 + j] = ((mi >> j) & 1u) ? A.g[rs + __builtin_popcount(mi & ((1u << j) - 1u))] : 0.0;
+    for (int c = 0; c < 6; ++c) v[c] = c < n ? src[c] : 0.0;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      const int c = __builtin_popcount(mi & ((1u << j) - 1u));
+      double x = v[0];
+#pragma unroll
+      for (int cc = 1; cc <= j; ++cc) x = c == cc ? v[cc] : x;
+      G[i * 6 + j] = ((mi >> j) & 1u) ? x : 0.0;
+    }
   }
+}
+
+// Flat standard blocks (frame = identity, masks (u,v)^2 + (w,tx,ty)^2 and the
+// drilling diagonal on self blocks): 13 or 14 entries at fixed places.
+__device__ inline void gather_block_flat(const AdjArgs& A, uint32_t k, uint32_t a, bool self, double G[36]) {
+  int64_t rs[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) rs[i] = A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i];
+#pragma unroll
+  for (int i = 0; i < 36; ++i) G[i] = 0.0;
+  G[0] = A.g[rs[0]];
+  G[1] = A.g[rs[0] + 1];
+  G[6] = A.g[rs[1]];
+  G[7] = A.g[rs[1] + 1];
+#pragma unroll
+  for (int i = 2; i < 5; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[i * 6 + 2 + j] = A.g[rs[i] + j];
+  if (self) G[35] = A.g[rs[5]];
 }
 
 __device__ inline bool is_identity(const double R[9]) {
@@ -115,7 +145,10 @@ __device__ inline double quad_sum(double v) {
 
 // ---------------------------------------------------------------- plates
 // Four lanes per element (lane p = row block of node p), 64 elements per block.
-template <int MODE>
+// FLAT (dL/dK mode only): every plate frame is the identity and every block
+// standard; then the frame-rotation term vanishes identically (the pattern has
+// no in-plane / out-of-plane couplings) and the blocks have fixed layouts.
+template <int MODE, bool FLAT>
 __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const size_t e0 = size_t(b) * 64 + (threadIdx.x >> 2);
@@ -126,9 +159,14 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
 #pragma unroll
   for (int a = 0; a < 4; ++a) nd[a] = A.conn[4 * e + a];
   double R[9];
+  if (FLAT) {
 #pragma unroll
-  for (int k = 0; k < 9; ++k) R[k] = A.R[9 * e + k];
-  const bool ident = is_identity(R);
+    for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) R[k] = A.R[9 * e + k];
+  }
+  const bool ident = FLAT || is_identity(R);
   double loc[4][3];
   {
     const double x3 = A.xyz[3 * size_t(nd[2])], y3 = A.xyz[3 * size_t(nd[2]) + 1], z3 = A.xyz[3 * size_t(nd[2]) + 2];
@@ -138,6 +176,7 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
                             A.xyz[3 * size_t(nd[a]) + 2] - z3};
 #pragma unroll
       for (int i = 0; i < 3; ++i) loc[a][i] = (R[i * 3 + 0] * dd[0] + R[i * 3 + 1] * dd[1]) + R[i * 3 + 2] * dd[2];
+      if (FLAT) loc[a][2] = 0.0;
     }
   }
   const double xl[6] = {loc[0][0], loc[0][1], loc[1][0], loc[1][1], loc[3][0], loc[3][1]};
@@ -171,7 +210,9 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
     double gl[36];
-    if (MODE == MODE_DLDK) {
+    if (MODE == MODE_DLDK && FLAT) {
+      gather_block_flat(A, A.bid[16 * e + 4 * p + q], ap, p == q, gl);
+    } else if (MODE == MODE_DLDK) {
       gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
       if (!ident) to_local(gl, R);
     } else {
@@ -182,7 +223,7 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
 #pragma unroll
         for (int j = 0; j < 6; ++j) gl[i * 6 + j] = -(up[i] * uq[j]);
     }
-    fem::plate_block_adjoint(acc, mat, sv, p, q, gl, P, c, gxl, hx, hy);
+    fem::plate_block_adjoint(acc, mat, sv, p, q, gl, P, c, gxl, hx, hy, !FLAT);
   }
   // pass 3: reverse sweep over the integration points
 #pragma unroll 1
@@ -341,6 +382,7 @@ std::string upload_adjoint_maps(Device* d, const AdjointMaps& am) {
   e = dupload(d->adj_inc, am.inc.data(), am.inc.size());
   if (!e.empty()) return e;
   for (int t = 0; t < 3; ++t) d->adj_xoff[t] = am.xoff[t];
+  d->adj_flat = am.flat;
   d->adj_nxs = am.n_xs_rows;
   return dalloc(d->adj_xs, size_t(3 * am.n_xs_rows));
 }
@@ -368,8 +410,9 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.rel_tol = d->rel_tol;
     if (t == PLATE) {
       const unsigned blocks = unsigned((n + 63) / 64);
-      if (mode == MODE_DLDK) plate_adjoint_kernel<MODE_DLDK><<<blocks, 256, 0, s>>>(A);
-      else plate_adjoint_kernel<MODE_U><<<blocks, 256, 0, s>>>(A);
+      if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<MODE_DLDK, true><<<blocks, 256, 0, s>>>(A);
+      else if (mode == MODE_DLDK) plate_adjoint_kernel<MODE_DLDK, false><<<blocks, 256, 0, s>>>(A);
+      else plate_adjoint_kernel<MODE_U, false><<<blocks, 256, 0, s>>>(A);
     } else {
       const unsigned blocks = unsigned((n + 255) / 256);
       if (t == TRUSS) {

@@ -33,7 +33,7 @@ struct Device {
   int64_t* slot_ptr = nullptr;
   uint32_t* slot_src = nullptr;
   // plate fused maps
-  int n_patches = 0, pf_grid = 0;
+  int n_patches = 0, pf_grid = 0, gen_block = 256;
   uint32_t *pf_hdr = nullptr, *pf_conn = nullptr, *pf_eid = nullptr, *pf_rows = nullptr, *pf_oinfo = nullptr,
            *pf_oboff = nullptr, *pf_oooff = nullptr, *pf_blk = nullptr;
   int64_t* pf_obase = nullptr;
@@ -45,6 +45,7 @@ struct Device {
   double* adj_xs = nullptr;
   int64_t adj_xoff[3] = {0, 0, 0};
   int64_t adj_nxs = 0;
+  bool adj_flat = false;
   double rel_tol = 0.0, abs_tol = 0.0;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;

@@ -330,35 +330,40 @@ std::string upload_generic(Device* d, const GenericMaps& g) {
   return dupload(d->slot_src, g.slot_src.data(), g.slot_src.size());
 }
 
-std::string assemble_generic(Device* d, double* val, void* stream) {
+void set_generic_block(Device* d, int threads) { d->gen_block = threads; }
+
+std::string assemble_generic(Device* d, double* val, void* stream, bool timed) {
   hipStream_t s = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipEventRecord(d->ev0, s));
+  const unsigned bs = unsigned(d->gen_block);
+  if (timed) HIP_TRY(hipEventRecord(d->ev0, s));
   for (int t = 0; t < 3; ++t) {
     size_t n = d->n_elem[t];
     if (!n) continue;
     size_t items = n * (t == PLATE ? 16 : 4);
-    unsigned blocks = unsigned((items + 255) / 256);
+    unsigned blocks = unsigned((items + bs - 1) / bs);
     if (t == PLATE)
-      generic_items_kernel<PLATE><<<blocks, 256, 0, s>>>(d->xyz, d->conn[t], d->params[t], d->R[t], d->derived,
-                                                          d->eclass[t], d->ebase[t], d->class_mask, d->class_off,
-                                                          items, d->contrib);
-    else if (t == BEAM)
-      generic_items_kernel<BEAM><<<blocks, 256, 0, s>>>(d->xyz, d->conn[t], d->params[t], d->R[t], d->derived,
+      generic_items_kernel<PLATE><<<blocks, bs, 0, s>>>(d->xyz, d->conn[t], d->params[t], d->R[t], d->derived,
                                                          d->eclass[t], d->ebase[t], d->class_mask, d->class_off,
                                                          items, d->contrib);
+    else if (t == BEAM)
+      generic_items_kernel<BEAM><<<blocks, bs, 0, s>>>(d->xyz, d->conn[t], d->params[t], d->R[t], d->derived,
+                                                        d->eclass[t], d->ebase[t], d->class_mask, d->class_off,
+                                                        items, d->contrib);
     else
-      generic_items_kernel<TRUSS><<<blocks, 256, 0, s>>>(d->xyz, d->conn[t], d->params[t], d->R[t], d->derived,
-                                                          d->eclass[t], d->ebase[t], d->class_mask, d->class_off,
-                                                          items, d->contrib);
+      generic_items_kernel<TRUSS><<<blocks, bs, 0, s>>>(d->xyz, d->conn[t], d->params[t], d->R[t], d->derived,
+                                                         d->eclass[t], d->ebase[t], d->class_mask, d->class_off,
+                                                         items, d->contrib);
     HIP_TRY(hipGetLastError());
   }
   if (d->nnz > 0) {
-    unsigned blocks = unsigned((d->nnz + 255) / 256);
-    slot_gather_kernel<<<blocks, 256, 0, s>>>(d->slot_ptr, d->slot_src, d->contrib, d->nnz, val);
+    unsigned blocks = unsigned((d->nnz + bs - 1) / bs);
+    slot_gather_kernel<<<blocks, bs, 0, s>>>(d->slot_ptr, d->slot_src, d->contrib, d->nnz, val);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(d->ev1, s));
-  d->timed = true;
+  if (timed) {
+    HIP_TRY(hipEventRecord(d->ev1, s));
+    d->timed = true;
+  }
   return "";
 }
 

@@ -80,9 +80,33 @@ struct PlateMaps {
 std::string upload_plate_maps(Device* d, const PlateMaps& pm, int wg_per_cu);
 
 // Value kernels.  `val` is nnz doubles of device memory, `stream` a hipStream_t.
-std::string assemble_generic(Device* d, double* val, void* stream);
-std::string assemble_plate_fused(Device* d, double* val, void* stream);
+// `timed` records the model's events around the kernels (off inside graph capture).
+std::string assemble_generic(Device* d, double* val, void* stream, bool timed = true);
+std::string assemble_plate_fused(Device* d, double* val, void* stream, bool timed = true);
 double last_ms(Device* d);
+// launch shapes (the autotuner's knobs): resident workgroups per CU of the fused
+// plate kernel (0 = one per patch); block size of the generic kernels
+std::string set_plate_grid(Device* d, int wg_per_cu);
+void set_generic_block(Device* d, int threads);
+
+// Batched assembly helpers (batch.hip)
+struct Graph;
+struct Timer;
+std::string stream_create(void*& s);
+void stream_destroy(void* s);
+std::string stream_sync(void* s);
+std::string graph_begin(void* stream);
+std::string graph_end(void* stream, Graph*& g);
+std::string graph_launch(Graph* g, void* stream);
+void graph_destroy(Graph* g);
+Timer* timer_create();
+void timer_destroy(Timer* t);
+std::string timer_start(Timer* t, void* stream);
+std::string timer_stop(Timer* t, void* stream, double& ms);  // synchronizes
+std::string timer_end(Timer* t, void* stream);  // records only
+double last_ms_between(Timer* t);                // synchronizes on the end event
+std::string copy_to_host(void* dst, const void* src_device, size_t bytes);
+int64_t* row_ptr_mut(Device* d);
 
 // Adjoint maps (adjoint.hip)
 struct AdjointMaps {
@@ -93,6 +117,7 @@ struct AdjointMaps {
   std::vector<uint32_t> inc;      // per node: its (element, node) gradient rows, element-insertion order
   int64_t xoff[3] = {0, 0, 0};    // first gradient row of each element type (beams have none)
   int64_t n_xs_rows = 0;
+  bool flat = false;              // plates only, identity frames, standard flat block masks
 };
 std::string upload_adjoint_maps(Device* d, const AdjointMaps& am);
 // mode 0: g = dL/dK (nnz values, CSR order); mode 1: g = nodal displacements

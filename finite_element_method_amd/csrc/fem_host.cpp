@@ -5,6 +5,7 @@
 // (fem_device.hip, plate_fused.hip); the host only validates and indexes.
 #include <algorithm>
 #include <array>
+#include <map>
 #include <charconv>
 #include <cmath>
 #include <cstring>
@@ -104,6 +105,30 @@ struct fem_model {
   std::vector<uint64_t> pat_bmask;
   std::vector<uint16_t> pat_boff;
   bool adj_valid = false;
+  std::vector<int64_t> row_ptr_host;  // host copy of the pattern's row_ptr
+  std::vector<uint32_t> node_part;    // batch groups: mesh of each node (major key of the patch order)
+};
+
+// A batch of independent meshes assembled together (no counterpart in the
+// crate).  Meshes are copied into per-group disjoint-union models: flat
+// plate-only meshes (fused plate kernel) and all others (generic path).  A
+// mesh's nodes are a contiguous index range of its group, so its CSR rows and
+// values are contiguous slices of the group CSR and its values are summed in
+// its own element-insertion order, exactly as when it is assembled alone.
+struct fem_batch {
+  struct Mesh {
+    int group = -1;
+    size_t node_off = 0, n_nodes = 0;
+    int64_t val_off = 0, nnz = 0;
+  };
+  std::string err;
+  std::vector<fem_model*> groups;
+  std::vector<Mesh> meshes;
+  void* stream = nullptr;
+  femdev::Graph* graph = nullptr;
+  femdev::Timer* timer = nullptr;
+  bool ready = false, timed = false;
+  std::string config = "untuned";
 };
 
 namespace {
@@ -362,7 +387,16 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
     }
     order.push_back({code, uint32_t(a)});
   }
-  std::sort(order.begin(), order.end());
+  if (m->node_part.size() == N) {
+    // batch groups: meshes one after another, Morton order within each
+    std::stable_sort(order.begin(), order.end(), [&](const std::pair<uint64_t, uint32_t>& x,
+                                                     const std::pair<uint64_t, uint32_t>& y) {
+      const uint32_t px = m->node_part[x.second], py = m->node_part[y.second];
+      return px != py ? px < py : x < y;
+    });
+  } else {
+    std::sort(order.begin(), order.end());
+  }
   // greedy patches over the Morton order, within the kernel's limits
   const uint32_t max_nodes = uint32_t(std::min(m->pf_nodes, femdev::PF_MAX_NODES));
   std::vector<uint32_t> cur, cur_elems;
@@ -566,6 +600,7 @@ std::string build_pattern(fem_model* m) {
   e = femdev::upload_pattern(m->dev, n_rows, row_ptr.data(), nnz, col.data());
   if (!e.empty()) return e;
   m->nnz = nnz;
+  m->row_ptr_host = row_ptr;
   m->adj_valid = false;
   m->pat_bptr = bptr;
   m->pat_bnode.resize(blocks.size());
@@ -676,6 +711,15 @@ int ensure_adjoint(fem_model* m) {
         }
   }
   if (m->pat_bnode.size() >= (size_t(1) << 32)) return fail(m, FEM_ERR_ARGUMENT, "too many node-pair blocks");
+  am.flat = !m->set[TRUSS].size() && !m->set[BEAM].size() && m->set[PLATE].size();
+  for (uint16_t rz : m->rzero[PLATE]) am.flat = am.flat && (rz & 0x4000);
+  if (am.flat) {
+    const Mask MOFF = plate_off_mask(), MSELF = plate_off_mask() | (1ull << 35);
+    const auto& S = m->set[PLATE];
+    for (size_t e = 0; e < S.size() && am.flat; ++e)
+      for (int pq = 0; pq < 16; ++pq)
+        if (m->pat_bmask[am.bid[PLATE][16 * e + pq]] != ((pq / 4 == pq % 4) ? MSELF : MOFF)) am.flat = false;
+  }
   am.bmask = m->pat_bmask;
   am.boff = m->pat_boff;
   // per node: (element, node) gradient rows of trusses and plates in insertion order
@@ -743,6 +787,126 @@ int rerun_setup(fem_model* m, bool all_types, int type) {
       m->adj_valid = false;
     }
   }
+  return FEM_OK;
+}
+
+// ------------------------------------------------------------------ batches
+// true when every plate frame of m is exactly the identity: the normal
+// (X4 - X3) x (X2 - X3) of quad_rotation has zero x and y components and a
+// positive z component, which the minimal rotation maps to I exactly
+bool plates_flat_identity(const fem_model* m) {
+  const auto& S = m->set[PLATE];
+  for (size_t e = 0; e < S.size(); ++e) {
+    const double* p2 = &m->xyz[3 * size_t(S.conn[4 * e + 1])];
+    const double* p3 = &m->xyz[3 * size_t(S.conn[4 * e + 2])];
+    const double* p4 = &m->xyz[3 * size_t(S.conn[4 * e + 3])];
+    const double e34[3] = {p4[0] - p3[0], p4[1] - p3[1], p4[2] - p3[2]};
+    const double e32[3] = {p2[0] - p3[0], p2[1] - p3[1], p2[2] - p3[2]};
+    const double nx = e34[1] * e32[2] - e34[2] * e32[1], ny = e34[2] * e32[0] - e34[0] * e32[2];
+    const double nz = e34[0] * e32[1] - e34[1] * e32[0];
+    if (!(nx == 0.0 && ny == 0.0 && nz > 0.0)) return false;
+  }
+  return true;
+}
+
+// appends src's nodes and elements to dst (indices shifted; element order kept)
+void append_model(fem_model* dst, const fem_model* src, uint32_t part) {
+  const uint32_t off = uint32_t(dst->node_number.size());
+  for (size_t i = 0; i < src->node_number.size(); ++i) {
+    const uint32_t idx = off + uint32_t(i);
+    dst->node_idx[idx + 1] = idx;
+    dst->node_number.push_back(idx + 1);
+    dst->node_plates.emplace_back();
+    dst->node_part.push_back(part);
+  }
+  dst->xyz.insert(dst->xyz.end(), src->xyz.begin(), src->xyz.end());
+  struct ERef {
+    uint64_t seq;
+    int t;
+    uint32_t pos;
+  };
+  std::vector<ERef> order;
+  for (int t = 0; t < 3; ++t)
+    for (uint32_t i = 0; i < src->set[t].size(); ++i) order.push_back({src->set[t].seq[i], t, i});
+  std::sort(order.begin(), order.end(), [](const ERef& x, const ERef& y) { return x.seq < y.seq; });
+  for (auto& r : order) {
+    const int nn = femdev::NODES_PER[r.t], np = femdev::PARAMS_PER[r.t];
+    auto& D = dst->set[r.t];
+    const auto& S = src->set[r.t];
+    const uint32_t num = uint32_t(D.size()) + 1;
+    D.by_number[num] = uint32_t(D.size());
+    D.number.push_back(num);
+    for (int p = 0; p < nn; ++p) D.conn.push_back(S.conn[size_t(r.pos) * nn + p] + off);
+    D.params.insert(D.params.end(), S.params.begin() + size_t(r.pos) * np, S.params.begin() + size_t(r.pos + 1) * np);
+    D.seq.push_back(dst->seq++);
+  }
+  dst->dirty = true;
+  dst->pattern_valid = false;
+}
+
+int batch_fail(fem_batch* b, int code, const std::string& msg) {
+  b->err = msg;
+  return code;
+}
+
+int batch_ready(fem_batch* b) {
+  if (b->ready) return FEM_OK;
+  for (auto* g : b->groups) {
+    int rc = ensure_ready(g);
+    if (rc) return batch_fail(b, rc, g->err);
+  }
+  for (auto& mm : b->meshes) {
+    const fem_model* g = b->groups[size_t(mm.group)];
+    const size_t r0 = 6 * mm.node_off, r1 = 6 * (mm.node_off + mm.n_nodes);
+    mm.val_off = g->row_ptr_host[r0];
+    mm.nnz = g->row_ptr_host[r1] - mm.val_off;
+  }
+  b->ready = true;
+  return FEM_OK;
+}
+
+std::string batch_launch_groups_one(fem_model* g, void* stream) {
+  return g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, femdev::values(g->dev), stream, false)
+                                  : femdev::assemble_generic(g->dev, femdev::values(g->dev), stream, false);
+}
+
+std::string batch_launch_groups(fem_batch* b, void* stream, bool timed) {
+  for (auto* g : b->groups) {
+    auto e = g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, femdev::values(g->dev), stream, timed)
+                                      : femdev::assemble_generic(g->dev, femdev::values(g->dev), stream, timed);
+    if (!e.empty()) return e;
+  }
+  return "";
+}
+
+int batch_capture(fem_batch* b) {
+  femdev::graph_destroy(b->graph);
+  b->graph = nullptr;
+  auto e = femdev::graph_begin(b->stream);
+  if (e.empty()) {
+    e = batch_launch_groups(b, b->stream, false);
+    femdev::Graph* g = nullptr;
+    auto e2 = femdev::graph_end(b->stream, g);
+    if (e.empty()) e = e2;
+    b->graph = e.empty() ? g : nullptr;
+    if (!e.empty()) femdev::graph_destroy(g);
+  }
+  return e.empty() ? FEM_OK : batch_fail(b, FEM_ERR_DEVICE, e);
+}
+
+// median of `reps` timed launches of one group on the batch stream
+int time_group(fem_batch* b, fem_model* g, int reps, double& ms) {
+  std::vector<double> t;
+  for (int r = -1; r < reps; ++r) {
+    auto e = femdev::timer_start(b->timer, b->stream);
+    if (e.empty()) e = batch_launch_groups_one(g, b->stream);
+    double x = 0;
+    if (e.empty()) e = femdev::timer_stop(b->timer, b->stream, x);
+    if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+    if (r >= 0) t.push_back(x);
+  }
+  std::sort(t.begin(), t.end());
+  ms = t[t.size() / 2];
   return FEM_OK;
 }
 
@@ -1100,6 +1264,228 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
   }
   m->pattern_valid = false;
   return FEM_OK;
+}
+
+int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out) {
+  if (!out || (!models && n)) return FEM_ERR_ARGUMENT;
+  *out = nullptr;
+  for (size_t i = 0; i < n; ++i)
+    if (!models[i] || models[i]->rel_tol != models[0]->rel_tol || models[i]->abs_tol != models[0]->abs_tol)
+      return FEM_ERR_ARGUMENT;
+  auto* b = new fem_batch();
+  // Groups: flat plate-only meshes (every frame the identity: fused plate
+  // kernel), split so that each group's values stay below the fused kernel's
+  // 32-bit byte offsets (at most 118 values per node at valence <= 8); all
+  // other meshes form one generic group.
+  const uint64_t plate_cap = (uint64_t(1) << 32) / 8 / 118;  // nodes per plate group
+  std::vector<int> grp(n, -1);
+  std::vector<uint64_t> gnodes;
+  std::vector<bool> gplate;
+  int cur_plate = -1, generic = -1;
+  for (size_t i = 0; i < n; ++i) {
+    const fem_model* m = models[i];
+    const uint64_t nn = m->node_number.size();
+    const bool flat = m->set[PLATE].size() && !m->set[TRUSS].size() && !m->set[BEAM].size() && plates_flat_identity(m);
+    if (flat && nn <= plate_cap) {
+      if (cur_plate < 0 || gnodes[size_t(cur_plate)] + nn > plate_cap) {
+        cur_plate = int(gnodes.size());
+        gnodes.push_back(0);
+        gplate.push_back(true);
+      }
+      grp[i] = cur_plate;
+    } else {
+      if (generic < 0) {
+        generic = int(gnodes.size());
+        gnodes.push_back(0);
+        gplate.push_back(false);
+      }
+      grp[i] = generic;
+    }
+    gnodes[size_t(grp[i])] += nn;
+  }
+  for (uint64_t v : gnodes)
+    if (v >= (uint64_t(1) << 31)) {
+      delete b;
+      return FEM_ERR_ARGUMENT;
+    }
+  for (size_t k = 0; k < gnodes.size(); ++k) {
+    auto* g = new fem_model();
+    g->rel_tol = models[0]->rel_tol;
+    g->abs_tol = models[0]->abs_tol;
+    g->nodes_number = uint32_t(gnodes[k]);
+    b->groups.push_back(g);
+  }
+  for (size_t i = 0; i < n; ++i) {
+    fem_model* g = b->groups[size_t(grp[i])];
+    fem_batch::Mesh mm;
+    mm.group = grp[i];
+    mm.node_off = g->node_number.size();
+    mm.n_nodes = models[i]->node_number.size();
+    b->meshes.push_back(mm);
+    append_model(g, models[i], uint32_t(i));
+  }
+  auto e = femdev::stream_create(b->stream);
+  b->timer = femdev::timer_create();
+  if (!e.empty() || !b->timer) {
+    fem_batch_destroy(b);
+    return FEM_ERR_DEVICE;
+  }
+  *out = b;
+  return FEM_OK;
+}
+
+void fem_batch_destroy(fem_batch* b) {
+  if (!b) return;
+  femdev::graph_destroy(b->graph);
+  femdev::timer_destroy(b->timer);
+  for (auto* g : b->groups) fem_destroy(g);
+  femdev::stream_destroy(b->stream);
+  delete b;
+}
+
+const char* fem_batch_last_error(const fem_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+int fem_batch_size(const fem_batch* b, size_t* n_meshes, size_t* n_groups) {
+  if (!b) return FEM_ERR_ARGUMENT;
+  if (n_meshes) *n_meshes = b->meshes.size();
+  if (n_groups) *n_groups = b->groups.size();
+  return FEM_OK;
+}
+
+int fem_batch_autotune(fem_batch* b) {
+  if (!b) return FEM_ERR_ARGUMENT;
+  // tuned shapes are cached per batch signature for the life of the process
+  static std::map<std::string, std::vector<std::array<int, 3>>> cache;
+  std::string sig;
+  for (auto* g : b->groups)
+    sig += std::to_string(g->set[0].size()) + "," + std::to_string(g->set[1].size()) + "," +
+           std::to_string(g->set[2].size()) + "," + std::to_string(g->node_number.size()) + ";";
+  sig += std::to_string(b->meshes.size());
+  std::vector<std::array<int, 3>> best;  // per group: patch nodes, workgroups per CU, generic block
+  auto it = cache.find(sig);
+  if (it != cache.end()) {
+    best = it->second;
+  } else {
+    for (auto* g : b->groups) {
+      std::array<int, 3> cfg = {g->pf_nodes, g->pf_wg_per_cu, 256};
+      double best_ms = 1e300;
+      const bool plate_only = g->set[PLATE].size() && !g->set[TRUSS].size() && !g->set[BEAM].size();
+      const int patch_opts[2] = {32, 64};
+      for (int pn : patch_opts) {
+        if (!plate_only && pn != 64) continue;
+        if (g->pf_nodes != pn) {
+          g->pf_nodes = pn;
+          g->pattern_valid = false;
+          b->ready = false;
+        }
+        int rc = batch_ready(b);
+        if (rc) return rc;
+        if (g->path == "plate_fused") {
+          for (int wg : {1, 2, 0}) {
+            auto e = femdev::set_plate_grid(g->dev, wg);
+            if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+            double ms = 0;
+            rc = time_group(b, g, 3, ms);
+            if (rc) return rc;
+            if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
+          }
+        } else {
+          for (int bs : {128, 256, 512}) {
+            femdev::set_generic_block(g->dev, bs);
+            double ms = 0;
+            rc = time_group(b, g, 3, ms);
+            if (rc) return rc;
+            if (ms < best_ms) { best_ms = ms; cfg = {pn, g->pf_wg_per_cu, bs}; }
+          }
+        }
+      }
+      best.push_back(cfg);
+    }
+    cache[sig] = best;
+  }
+  // apply
+  b->config.clear();
+  for (size_t k = 0; k < b->groups.size(); ++k) {
+    auto* g = b->groups[k];
+    if (g->pf_nodes != best[k][0]) {
+      g->pf_nodes = best[k][0];
+      g->pattern_valid = false;
+      b->ready = false;
+    }
+    g->pf_wg_per_cu = best[k][1];
+  }
+  int rc = batch_ready(b);
+  if (rc) return rc;
+  for (size_t k = 0; k < b->groups.size(); ++k) {
+    auto* g = b->groups[k];
+    if (g->path == "plate_fused") {
+      auto e = femdev::set_plate_grid(g->dev, best[k][1]);
+      if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+      b->config += "group " + std::to_string(k) + ": plate_fused patch_nodes=" + std::to_string(best[k][0]) +
+                   " wg_per_cu=" + std::to_string(best[k][1]) + "; ";
+    } else {
+      femdev::set_generic_block(g->dev, best[k][2]);
+      b->config += "group " + std::to_string(k) + ": generic block=" + std::to_string(best[k][2]) + "; ";
+    }
+  }
+  return batch_capture(b);
+}
+
+int fem_batch_assemble(fem_batch* b, void* stream) {
+  if (!b) return FEM_ERR_ARGUMENT;
+  int rc = batch_ready(b);
+  if (rc) return rc;
+  if (!b->graph) {
+    rc = batch_capture(b);
+    if (rc) return rc;
+  }
+  auto e = femdev::timer_start(b->timer, stream);
+  if (e.empty()) e = femdev::graph_launch(b->graph, stream);
+  if (e.empty()) e = femdev::timer_end(b->timer, stream);
+  if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+  b->timed = true;
+  return FEM_OK;
+}
+
+double fem_batch_last_ms(fem_batch* b) { return b && b->timed ? femdev::last_ms_between(b->timer) : -1.0; }
+
+const char* fem_batch_config(const fem_batch* b) { return b ? b->config.c_str() : ""; }
+
+int fem_batch_mesh_csr(fem_batch* b, size_t mesh, int64_t* n_rows, int64_t* nnz, const int64_t** row_ptr,
+                       const int32_t** col_idx, const double** val, int64_t* val_base, int64_t* col_base) {
+  if (!b || mesh >= b->meshes.size()) return FEM_ERR_ARGUMENT;
+  int rc = batch_ready(b);
+  if (rc) return rc;
+  const auto& mm = b->meshes[mesh];
+  auto* g = b->groups[size_t(mm.group)];
+  if (n_rows) *n_rows = int64_t(6 * mm.n_nodes);
+  if (nnz) *nnz = mm.nnz;
+  if (row_ptr) *row_ptr = femdev::row_ptr(g->dev) + 6 * mm.node_off;
+  if (col_idx) *col_idx = femdev::col_idx(g->dev) + mm.val_off;
+  if (val) *val = femdev::values(g->dev) + mm.val_off;
+  if (val_base) *val_base = mm.val_off;
+  if (col_base) *col_base = int64_t(6 * mm.node_off);
+  return FEM_OK;
+}
+
+int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* rp, int32_t* ci, double* v) {
+  const int64_t *drp = nullptr;
+  const int32_t* dci = nullptr;
+  const double* dv = nullptr;
+  int64_t nr = 0, nz = 0, vb = 0, cb = 0;
+  int rc = fem_batch_mesh_csr(b, mesh, &nr, &nz, &drp, &dci, &dv, &vb, &cb);
+  if (rc) return rc;
+  std::string e;
+  if (rp) {
+    e = femdev::copy_to_host(rp, drp, size_t(nr + 1) * sizeof(int64_t));
+    for (int64_t i = 0; i <= nr && e.empty(); ++i) rp[i] -= vb;
+  }
+  if (ci && e.empty()) {
+    e = femdev::copy_to_host(ci, dci, size_t(nz) * sizeof(int32_t));
+    for (int64_t i = 0; i < nz && e.empty(); ++i) ci[i] -= int32_t(cb);
+  }
+  if (v && e.empty()) e = femdev::copy_to_host(v, dv, size_t(nz) * sizeof(double));
+  return e.empty() ? FEM_OK : batch_fail(b, FEM_ERR_DEVICE, e);
 }
 
 }  // extern "C"

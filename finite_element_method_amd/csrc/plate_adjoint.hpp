@@ -153,7 +153,7 @@ __host__ __device__ inline void plate_frame_adjoint_sub(const double* k, const d
 // of column q (layout of plate_point_adjoint).
 __host__ __device__ inline void plate_block_adjoint(const double acc[24], const PlateMat& m, const PlateShear& sv,
                                                     int p, int q, const double gl[36], double P[7], double c[24],
-                                                    double gxl[6], double& hx, double& hy) {
+                                                    double gxl[6], double& hx, double& hy, bool frame = true) {
   const double Mxx = acc[q], Mxy = acc[4 + q], Myx = acc[8 + q], Myy = acc[12 + q], A = acc[16 + q], B = acc[20 + q];
   const double Guu = gl[0], Guv = gl[1], Gvu = gl[6], Gvv = gl[7];
   const double Gxx = gl[21], Gxy = gl[22], Gyx = gl[27], Gyy = gl[28];
@@ -200,7 +200,9 @@ __host__ __device__ inline void plate_block_adjoint(const double acc[24], const 
   const double sa = m.cs * A, sb = m.cs * B;
   plate_shear_vector_adjoint(p, sa * Gv_q[1], sa * Gv_q[2], sb * Gu_q[1], sb * Gu_q[2], gxl);
   plate_shear_vector_adjoint(q, sa * vG[1], sa * vG[2], sb * uG[1], sb * uG[2], gxl);
-  // frame rotation: needs the local block values
+  // frame rotation: needs the local block values (skipped by callers whose
+  // gradient blocks carry no in-plane / out-of-plane coupling: then it is 0)
+  if (!frame) return;
   double kb[36];
 #pragma unroll
   for (int i = 0; i < 36; ++i) kb[i] = 0.0;

@@ -330,15 +330,19 @@ std::string upload_plate_maps(Device* d, const PlateMaps& pm, int wg_per_cu) {
   if (!(e = dupload(d->pf_oboff, pm.own_boff.data(), pm.own_boff.size())).empty()) return e;
   if (!(e = dupload(d->pf_oooff, pm.own_ooff.data(), pm.own_ooff.size())).empty()) return e;
   if (!(e = dupload(d->pf_blk, pm.blk.data(), pm.blk.size())).empty()) return e;
+  return set_plate_grid(d, wg_per_cu);
+}
+
+std::string set_plate_grid(Device* d, int wg_per_cu) {
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->dev));
-  d->pf_grid = wg_per_cu <= 0 ? pm.n_patches : std::max(1, std::min(pm.n_patches, cus * wg_per_cu));
+  d->pf_grid = wg_per_cu <= 0 ? d->n_patches : std::max(1, std::min(d->n_patches, cus * wg_per_cu));
   return "";
 }
 
-std::string assemble_plate_fused(Device* d, double* val, void* stream) {
+std::string assemble_plate_fused(Device* d, double* val, void* stream, bool timed) {
   hipStream_t s = static_cast<hipStream_t>(stream);
-  HIP_TRY(hipEventRecord(d->ev0, s));
+  if (timed) HIP_TRY(hipEventRecord(d->ev0, s));
   if (d->n_patches > 0) {
     PatchArgs a;
     a.xyz = d->xyz;
@@ -358,8 +362,10 @@ std::string assemble_plate_fused(Device* d, double* val, void* stream) {
     plate_fused_kernel<<<d->pf_grid, PF_THREADS, 0, s>>>(a);
     HIP_TRY(hipGetLastError());
   }
-  HIP_TRY(hipEventRecord(d->ev1, s));
-  d->timed = true;
+  if (timed) {
+    HIP_TRY(hipEventRecord(d->ev1, s));
+    d->timed = true;
+  }
   return "";
 }
 

@@ -146,6 +146,41 @@ double fem_last_backward_ms(fem_model* m);
 int fem_update_params(fem_model* m, int type, const double* params_device, void* stream);
 int fem_update_nodes(fem_model* m, const double* xyz_device, void* stream);
 
+/* ---- batched assembly (no counterpart in the crate) ---------------------
+ *
+ * Many independent meshes assembled by one hipGraph launch.  The meshes'
+ * nodes and elements are copied (at create) into per-group disjoint-union
+ * models: flat plate-only meshes (fused plate kernel) and all others (generic
+ * path); the input handles are not modified and may be destroyed afterwards.
+ * Each mesh keeps its own CSR: its rows and values are contiguous slices of
+ * its group's CSR, and every value is summed in the mesh's own element order,
+ * so the results equal those of assembling the mesh alone.  All meshes must
+ * share rel_tol and abs_tol. */
+typedef struct fem_batch fem_batch;
+int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out);
+void fem_batch_destroy(fem_batch* b);
+const char* fem_batch_last_error(const fem_batch* b);
+int fem_batch_size(const fem_batch* b, size_t* n_meshes, size_t* n_groups);
+/* Times candidate launch shapes per group (fused plate kernel: owned nodes per
+ * patch and resident workgroups per CU; generic path: block size), keeps the
+ * fastest and captures the graph.  Results are cached per batch signature for
+ * the life of the process.  Synchronous. */
+int fem_batch_autotune(fem_batch* b);
+/* One hipGraphLaunch of every group's value kernels on `stream`. */
+int fem_batch_assemble(fem_batch* b, void* stream);
+/* HIP-event time of the last fem_batch_assemble (ms); -1 when unavailable. */
+double fem_batch_last_ms(fem_batch* b);
+/* Description of the chosen launch shapes. */
+const char* fem_batch_config(const fem_batch* b);
+/* Mesh `mesh`'s CSR on the device, borrowed: row_ptr (n_rows + 1 entries) and
+ * col_idx (nnz) hold group positions and group columns; subtract *val_base from
+ * row_ptr and *col_base from col_idx for mesh-local ones.  val points at the
+ * mesh's nnz values. */
+int fem_batch_mesh_csr(fem_batch* b, size_t mesh, int64_t* n_rows, int64_t* nnz, const int64_t** row_ptr,
+                       const int32_t** col_idx, const double** val, int64_t* val_base, int64_t* col_base);
+/* Synchronous copy of mesh `mesh`'s CSR with mesh-local positions and columns. */
+int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int32_t* col_idx, double* val);
+
 /* Tuning options (no counterpart in the crate; they change speed, never values):
  *   "plate_patch_nodes"  owned nodes per workgroup of the fused plate kernel: 4..64
  *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernel,
