@@ -1,0 +1,239 @@
+// bar_owner.hip -- one-pass assembly of truss and beam meshes.
+//
+// One workgroup per range of owned nodes (<= 42 nodes, <= 256 incident
+// (node, element) entries, cut on the host).  Phase 1: one lane per entry
+// stages the element's constants and its blocks' masks and row offsets in LDS,
+// all global loads in parallel.  Phase 2: one lane per (owned node a, CSR row
+// i) walks a's entries in element-insertion order and computes, for each, row
+// i of the element's global blocks (p, p) and (p, 1 - p) (p = a's position in
+// the element).  The diagonal block (a, a) is summed in registers over all
+// incident elements (insertion order, starting from the first contribution);
+// an off-diagonal block (a, b) has exactly one contributing element (checked
+// when the maps are built) and is written directly.  Every CSR value is
+// written once: no contribution buffer, no atomics, bitwise reproducible.
+//
+// Beam blocks use the rank-one form of beam.rs:495-605: with T = diag(R, R),
+//   row i of T^T (sum_t s_t v_t,p v_t,q^T) T = sum_t s_t (T^T v_t,p)_i (T^T v_t,q)^T.
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "device_state.hpp"
+#include "element_math.hpp"
+#include "fem_device.hpp"
+
+namespace femdev {
+
+namespace {
+
+constexpr int BO_THREADS = 256;
+constexpr int BO_NODES = 42;     // owned nodes per workgroup (6 row lanes each)
+constexpr int BO_ENTRIES = 256;  // (owned node, incident element) entries per workgroup
+constexpr int BO_DATA = 18;      // doubles per staged entry
+
+struct BarArgs {
+  const double* __restrict__ xyz;
+  const uint32_t* __restrict__ conn[2];
+  const double* __restrict__ params[2];
+  const double* __restrict__ R[2];
+  const double* __restrict__ derived;
+  const uint32_t* __restrict__ bid[2];
+  const uint64_t* __restrict__ bmask;
+  const uint16_t* __restrict__ boff;
+  const uint32_t* __restrict__ inc_ptr;
+  const uint32_t* __restrict__ inc;
+  const uint2* __restrict__ ranges;  // per workgroup: first node, node count
+  const int64_t* __restrict__ row_ptr;
+  double* __restrict__ val;
+};
+
+struct BarLds {
+  double data[BO_ENTRIES * BO_DATA];   // per entry: R (9), term scales (6), bp, bq, pad / truss: r0, kpp, kpq
+  uint64_t mask[BO_ENTRIES][2];        // masks of the entry's blocks (p, p) and (p, 1 - p)
+  uint16_t off[BO_ENTRIES][2][6];      // per-row offsets of those blocks
+  uint32_t type[BO_ENTRIES];
+  uint32_t first[BO_NODES + 1];        // entry range of each owned node
+};
+
+// T^T v_t for the beam's rank-one term t with nodal shape value b
+// (v: u = b e0; v = b e1 - e5/2; w = b e2 - e4/2; tu = b e3; tv = b e4; tw = b e5)
+__device__ inline void beam_term_vec(int t, double b, const double R[9], double o[6]) {
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    double tr = 0.0, rr = 0.0;
+    if (t == 0) tr = b * R[0 * 3 + m];
+    else if (t == 1) { tr = b * R[1 * 3 + m]; rr = -0.5 * R[2 * 3 + m]; }
+    else if (t == 2) { tr = b * R[2 * 3 + m]; rr = -0.5 * R[1 * 3 + m]; }
+    else rr = b * R[(t - 3) * 3 + m];
+    o[m] = tr;
+    o[3 + m] = rr;
+  }
+}
+
+// stage one (owned node, incident element) entry: the element constants
+// (truss.rs:132-189, beam.rs:495-605) and its two blocks' masks and offsets
+__device__ inline void stage_entry(const BarArgs& A, BarLds& L, int slot, uint32_t u, int i_row_unused) {
+  (void)i_row_unused;
+  const int type = int(u >> 31), p = int(u & 1u);
+  const size_t e = (u >> 1) & 0x3FFFFFFFu;
+  const uint32_t n0 = A.conn[type][2 * e], n1 = A.conn[type][2 * e + 1];
+  double d[3];
+#pragma unroll
+  for (int k = 0; k < 3; ++k) d[k] = A.xyz[3 * size_t(n1) + k] - A.xyz[3 * size_t(n0) + k];
+  const double Lg = fem::norm3(d[0], d[1], d[2]);
+  const double J = fem::bar_jacobian(Lg);
+  const double inv_j = 1.0 / J;
+  const double bp = (p == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j, bq = (p == 0 ? 0.0 + 0.5 : 0.0 - 0.5) * inv_j;
+  const double* Rp = A.R[type] + 9 * e;
+  double* o = L.data + slot * BO_DATA;
+  if (type == 0) {
+    const double* prm = A.params[0] + 3 * e;
+    const double E = prm[0], Ar = prm[1], A2 = prm[2];
+    const double area = (A2 == A2) ? (A2 - Ar) / 2.0 * 0.0 + Ar - (A2 - Ar) / 2.0 * -1.0 : Ar;
+    const double scale = area * E * J * 2.0;
+    o[0] = Rp[0]; o[1] = Rp[1]; o[2] = Rp[2];
+    o[3] = (bp * bp) * scale;
+    o[4] = (bp * bq) * scale;
+  } else {
+    const double* prm = A.params[1] + 11 * e;
+    const double E = prm[0], nu = prm[1], Ar = prm[2], it = prm[6], k = prm[7];
+    const double G = E / (2.0 * (1.0 + nu));
+    const double det_j = J, alpha = 2.0;
+    const double i11p = A.derived[2 * e], i22p = A.derived[2 * e + 1];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) o[m] = Rp[m];
+    o[9] = Ar * E * det_j * alpha;
+    o[10] = G * Ar * k * det_j * alpha;
+    o[11] = G * Ar * k * det_j * alpha;
+    o[12] = G * it * det_j * alpha;
+    o[13] = E * i22p * det_j * alpha;
+    o[14] = E * i11p * det_j * alpha;
+    o[15] = bp;
+    o[16] = bq;
+  }
+  const uint32_t ks = A.bid[type][4 * e + 3 * p], kn = A.bid[type][4 * e + 2 * p + (1 - p)];
+  L.mask[slot][0] = A.bmask[ks];
+  L.mask[slot][1] = A.bmask[kn];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    L.off[slot][0][i] = A.boff[6 * size_t(ks) + i];
+    L.off[slot][1][i] = A.boff[6 * size_t(kn) + i];
+  }
+  L.type[slot] = uint32_t(type);
+}
+
+// row i of an entry's blocks (p, p) -> self and (p, 1 - p) -> nb
+__device__ inline void entry_rows(const BarLds& L, int slot, int i, double self[6], double nb[6]) {
+  const double* o = L.data + slot * BO_DATA;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) { self[j] = 0.0; nb[j] = 0.0; }
+  if (L.type[slot] == 0) {
+    if (i >= 3) return;
+    const double ri = o[i];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+      self[j] = (ri * o[3]) * o[j];
+      nb[j] = (ri * o[4]) * o[j];
+    }
+    return;
+  }
+  double R[9];
+#pragma unroll
+  for (int m = 0; m < 9; ++m) R[m] = o[m];
+  const double bp = o[15], bq = o[16];
+#pragma unroll
+  for (int t = 0; t < 6; ++t) {
+    double vp[6], vq[6];
+    beam_term_vec(t, bp, R, vp);
+    beam_term_vec(t, bq, R, vq);
+    const double ai = i == 0 ? vp[0] : (i == 1 ? vp[1] : (i == 2 ? vp[2] : (i == 3 ? vp[3] : (i == 4 ? vp[4] : vp[5]))));
+    const double sa = o[9 + t] * ai;
+#pragma unroll
+    for (int j = 0; j < 6; ++j) {
+      self[j] += sa * vp[j];
+      nb[j] += sa * vq[j];
+    }
+  }
+}
+
+__device__ inline void write_row(double* val, uint64_t mask, const uint16_t* off, int i, int64_t r0, const double v[6]) {
+  const uint32_t mi = uint32_t(mask >> (6 * i)) & 63u;
+  double* dst = val + (r0 + off[i]);
+#pragma unroll
+  for (int j = 0; j < 6; ++j)
+    if ((mi >> j) & 1u) dst[__builtin_popcount(mi & ((1u << j) - 1u))] = v[j];
+}
+
+// One workgroup per range of owned nodes: stage the incident entries (one
+// lane each, all loads in parallel), then one lane per (owned node, row).
+__global__ __launch_bounds__(BO_THREADS) void bar_owner_kernel(BarArgs A) {
+  __shared__ BarLds L;
+  const int t = threadIdx.x;
+  const uint2 rg = A.ranges[blockIdx.x];
+  const uint32_t a0 = rg.x, na = rg.y;
+  const uint32_t e0 = A.inc_ptr[a0], ne = A.inc_ptr[a0 + na] - e0;
+  if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
+  if (t < int(ne)) stage_entry(A, L, t, A.inc[e0 + t], 0);
+  __syncthreads();
+  const int j = t / 6, i = t % 6;
+  if (j >= int(na)) return;
+  const size_t a = size_t(a0) + j;
+  const int64_t r0 = A.row_ptr[6 * a + i];
+  if (A.row_ptr[6 * a + i + 1] == r0) return;
+  const int s0 = int(L.first[j]), s1 = int(L.first[j + 1]);
+  double acc[6];
+  for (int sl = s0; sl < s1; ++sl) {
+    double self[6], nb[6];
+    entry_rows(L, sl, i, self, nb);
+#pragma unroll
+    for (int q = 0; q < 6; ++q) acc[q] = sl == s0 ? self[q] : acc[q] + self[q];
+    write_row(A.val, L.mask[sl][1], L.off[sl][1], i, r0, nb);
+  }
+  if (s1 > s0) write_row(A.val, L.mask[s0][0], L.off[s0][0], i, r0, acc);
+}
+
+}  // namespace
+
+std::string upload_bar_maps(Device* d, const BarMaps& bm) {
+  std::string e;
+  for (int t = 0; t < 2; ++t)
+    if (!(e = dupload(d->bo_bid[t], bm.bid[t].data(), bm.bid[t].size())).empty()) return e;
+  if (!(e = dupload(d->bo_bmask, bm.bmask.data(), bm.bmask.size())).empty()) return e;
+  if (!(e = dupload(d->bo_boff, bm.boff.data(), bm.boff.size())).empty()) return e;
+  if (!(e = dupload(d->bo_inc_ptr, bm.inc_ptr.data(), bm.inc_ptr.size())).empty()) return e;
+  if (!(e = dupload(d->bo_ranges, bm.ranges.data(), bm.ranges.size())).empty()) return e;
+  d->bo_n_ranges = bm.ranges.size() / 2;
+  return dupload(d->bo_inc, bm.inc.data(), bm.inc.size());
+}
+
+std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (timed) HIP_TRY(hipEventRecord(d->ev0, s));
+  if (d->bo_n_ranges) {
+    BarArgs A;
+    A.xyz = d->xyz;
+    for (int t = 0; t < 2; ++t) {
+      A.conn[t] = d->conn[t];
+      A.params[t] = d->params[t];
+      A.R[t] = d->R[t];
+      A.bid[t] = d->bo_bid[t];
+    }
+    A.derived = d->derived;
+    A.bmask = d->bo_bmask;
+    A.boff = d->bo_boff;
+    A.inc_ptr = d->bo_inc_ptr;
+    A.inc = d->bo_inc;
+    A.ranges = reinterpret_cast<const uint2*>(d->bo_ranges);
+    A.row_ptr = d->row_ptr;
+    A.val = val;
+    bar_owner_kernel<<<unsigned(d->bo_n_ranges), BO_THREADS, 0, s>>>(A);
+    HIP_TRY(hipGetLastError());
+  }
+  if (timed) {
+    HIP_TRY(hipEventRecord(d->ev1, s));
+    d->timed = true;
+  }
+  return "";
+}
+
+}  // namespace femdev

@@ -37,6 +37,12 @@ struct Device {
   uint32_t *pf_hdr = nullptr, *pf_conn = nullptr, *pf_eid = nullptr, *pf_rows = nullptr, *pf_oinfo = nullptr,
            *pf_oboff = nullptr, *pf_oooff = nullptr, *pf_blk = nullptr;
   int64_t* pf_obase = nullptr;
+  // bar owner maps
+  uint32_t* bo_bid[2] = {nullptr, nullptr};
+  uint64_t* bo_bmask = nullptr;
+  uint16_t* bo_boff = nullptr;
+  uint32_t *bo_inc_ptr = nullptr, *bo_inc = nullptr, *bo_ranges = nullptr;
+  size_t bo_n_ranges = 0;
   // adjoint maps
   uint32_t* adj_bid[3] = {nullptr, nullptr, nullptr};
   uint64_t* adj_bmask = nullptr;

@@ -55,6 +55,8 @@ void destroy_device(Device* d) {
   f(d->slot_ptr); f(d->slot_src); f(d->pf_hdr); f(d->pf_conn); f(d->pf_eid); f(d->pf_rows); f(d->pf_oinfo);
   f(d->pf_oboff); f(d->pf_oooff); f(d->pf_blk); f(d->pf_obase);
   for (int t = 0; t < 3; ++t) f(d->adj_bid[t]);
+  for (int t = 0; t < 2; ++t) f(d->bo_bid[t]);
+  f(d->bo_bmask); f(d->bo_boff); f(d->bo_inc_ptr); f(d->bo_inc); f(d->bo_ranges);
   f(d->adj_bmask); f(d->adj_boff); f(d->adj_inc_ptr); f(d->adj_inc); f(d->adj_xs);
   for (hipEvent_t ev : {d->ev0, d->ev1, d->ev2, d->ev3})
     if (ev) hipEventDestroy(ev);

@@ -79,6 +79,20 @@ struct PlateMaps {
 // workgroup per patch)
 std::string upload_plate_maps(Device* d, const PlateMaps& pm, int wg_per_cu);
 
+// Bar owner path (bar_owner.hip): meshes of trusses and beams in which every
+// off-diagonal node pair belongs to one element.  One thread per (node, row).
+struct BarMaps {
+  std::vector<uint32_t> bid[2];   // trusses, beams: per element node pair (2 p + q): global block
+  std::vector<uint64_t> bmask;    // per block: union structural mask
+  std::vector<uint16_t> boff;     // per block and row: first entry of the block within the row
+  std::vector<uint32_t> inc_ptr;  // N + 1
+  std::vector<uint32_t> inc;      // per node, insertion order: type << 31 | element << 1 | p
+  std::vector<uint32_t> ranges;   // per workgroup: first node, node count (<= 42 nodes, <= 256 entries)
+};
+constexpr int BAR_WG_NODES = 42, BAR_WG_ENTRIES = 256;
+std::string upload_bar_maps(Device* d, const BarMaps& bm);
+std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed = true);
+
 // Value kernels.  `val` is nnz doubles of device memory, `stream` a hipStream_t.
 // `timed` records the model's events around the kernels (off inside graph capture).
 std::string assemble_generic(Device* d, double* val, void* stream, bool timed = true);

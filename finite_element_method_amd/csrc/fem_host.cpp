@@ -560,13 +560,19 @@ std::string build_pattern(fem_model* m) {
   std::vector<uint64_t> bptr(N + 1, 0);
   std::vector<PairRef> blocks;
   blocks.reserve(pr.size() / 2 + 1);
+  bool single_pairs = true;  // every off-diagonal node pair belongs to one element
   for (size_t a = 0; a < N; ++a) {
     auto* b0 = pr.data() + nptr[a];
     auto* b1 = pr.data() + nptr[a + 1];
-    std::sort(b0, b1, [](const PairRef& x, const PairRef& y) { return x.b < y.b; });
+    std::stable_sort(b0, b1, [](const PairRef& x, const PairRef& y) { return x.b < y.b; });
     for (auto* it = b0; it != b1;) {
       PairRef acc = *it;
-      for (++it; it != b1 && it->b == acc.b; ++it) acc.mask |= it->mask;
+      int cnt = 1;
+      for (++it; it != b1 && it->b == acc.b; ++it) {
+        acc.mask |= it->mask;
+        cnt++;
+      }
+      if (cnt > 1 && acc.b != uint32_t(a)) single_pairs = false;
       if (acc.mask) blocks.push_back(acc);
     }
     bptr[a + 1] = blocks.size();
@@ -618,6 +624,61 @@ std::string build_pattern(fem_model* m) {
       e = femdev::upload_plate_maps(m->dev, pm, m->pf_wg_per_cu);
       if (!e.empty()) return e;
       m->path = "plate_fused";
+      m->pattern_valid = true;
+      return "";
+    }
+  }
+  if (!m->force_generic && single_pairs && !m->set[PLATE].size() && m->set[TRUSS].size() + m->set[BEAM].size() &&
+      m->set[TRUSS].size() < (1u << 30) && m->set[BEAM].size() < (1u << 30)) {
+    femdev::BarMaps bm;
+    bool ok = true;
+    for (int t = 0; t < 2 && ok; ++t) {
+      const auto& S = m->set[t];
+      bm.bid[t].resize(S.size() * 4);
+      for (size_t e = 0; e < S.size() && ok; ++e)
+        for (int pq = 0; pq < 4; ++pq) {
+          const uint32_t a = S.conn[2 * e + pq / 2], b = S.conn[2 * e + pq % 2];
+          auto* k0 = blocks.data() + bptr[a];
+          auto* k1 = blocks.data() + bptr[a + 1];
+          auto* kb = std::lower_bound(k0, k1, b, [](const PairRef& x, uint32_t v) { return x.b < v; });
+          if (kb == k1 || kb->b != b) { ok = false; break; }
+          bm.bid[t][4 * e + pq] = uint32_t(kb - blocks.data());
+        }
+    }
+    if (ok && blocks.size() < (size_t(1) << 32)) {
+      bm.bmask.resize(blocks.size());
+      for (size_t k = 0; k < blocks.size(); ++k) bm.bmask[k] = blocks[k].mask;
+      bm.boff = boff;
+      bm.inc_ptr.assign(N + 1, 0);
+      for (auto& r : order) {
+        const uint32_t* c = &m->set[r.t].conn[size_t(r.pos) * 2];
+        bm.inc_ptr[c[0] + 1]++;
+        bm.inc_ptr[c[1] + 1]++;
+      }
+      for (size_t a = 0; a < N; ++a) bm.inc_ptr[a + 1] += bm.inc_ptr[a];
+      bm.inc.resize(bm.inc_ptr[N]);
+      std::vector<uint32_t> f(bm.inc_ptr.begin(), bm.inc_ptr.end() - 1);
+      for (auto& r : order) {
+        const uint32_t* c = &m->set[r.t].conn[size_t(r.pos) * 2];
+        for (int p = 0; p < 2; ++p) bm.inc[f[c[p]]++] = uint32_t(r.t) << 31 | r.pos << 1 | uint32_t(p);
+      }
+      // workgroup ranges: consecutive nodes, <= 42 nodes and <= 256 entries each
+      uint32_t a0 = 0;
+      while (a0 < N && ok) {
+        uint32_t na = 0;
+        while (a0 + na < N && na < uint32_t(femdev::BAR_WG_NODES) &&
+               bm.inc_ptr[a0 + na + 1] - bm.inc_ptr[a0] <= uint32_t(femdev::BAR_WG_ENTRIES))
+          na++;
+        if (na == 0) ok = false;  // one node with more than 256 incident elements
+        bm.ranges.push_back(a0);
+        bm.ranges.push_back(na);
+        a0 += na;
+      }
+    }
+    if (ok && blocks.size() < (size_t(1) << 32)) {
+      e = femdev::upload_bar_maps(m->dev, bm);
+      if (!e.empty()) return e;
+      m->path = "bar_owner";
       m->pattern_valid = true;
       return "";
     }
@@ -865,15 +926,19 @@ int batch_ready(fem_batch* b) {
   return FEM_OK;
 }
 
+std::string launch_values(fem_model* g, double* val, void* stream, bool timed) {
+  return g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, val, stream, timed)
+         : g->path == "bar_owner" ? femdev::assemble_bar_owner(g->dev, val, stream, timed)
+                                  : femdev::assemble_generic(g->dev, val, stream, timed);
+}
+
 std::string batch_launch_groups_one(fem_model* g, void* stream) {
-  return g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, femdev::values(g->dev), stream, false)
-                                  : femdev::assemble_generic(g->dev, femdev::values(g->dev), stream, false);
+  return launch_values(g, femdev::values(g->dev), stream, false);
 }
 
 std::string batch_launch_groups(fem_batch* b, void* stream, bool timed) {
   for (auto* g : b->groups) {
-    auto e = g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, femdev::values(g->dev), stream, timed)
-                                      : femdev::assemble_generic(g->dev, femdev::values(g->dev), stream, timed);
+    auto e = launch_values(g, femdev::values(g->dev), stream, timed);
     if (!e.empty()) return e;
   }
   return "";
@@ -1166,6 +1231,7 @@ int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
   int rc = ensure_ready(m);
   if (rc) return rc;
   std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
+                  : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream)
                                            : femdev::assemble_generic(m->dev, val_device, stream);
   return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
 }
@@ -1273,33 +1339,33 @@ int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out) {
     if (!models[i] || models[i]->rel_tol != models[0]->rel_tol || models[i]->abs_tol != models[0]->abs_tol)
       return FEM_ERR_ARGUMENT;
   auto* b = new fem_batch();
-  // Groups: flat plate-only meshes (every frame the identity: fused plate
+  // Groups, so that every mesh takes the same value path as when assembled
+  // alone: flat plate-only meshes (every frame the identity: fused plate
   // kernel), split so that each group's values stay below the fused kernel's
-  // 32-bit byte offsets (at most 118 values per node at valence <= 8); all
-  // other meshes form one generic group.
+  // 32-bit byte offsets (at most 118 values per node at valence <= 8); truss
+  // and beam meshes (node-owner kernel); all other meshes (generic path).
   const uint64_t plate_cap = (uint64_t(1) << 32) / 8 / 118;  // nodes per plate group
   std::vector<int> grp(n, -1);
   std::vector<uint64_t> gnodes;
-  std::vector<bool> gplate;
-  int cur_plate = -1, generic = -1;
+  int cur_plate = -1, bars = -1, generic = -1;
+  auto open_group = [&](int& g) {
+    if (g < 0) {
+      g = int(gnodes.size());
+      gnodes.push_back(0);
+    }
+    return g;
+  };
   for (size_t i = 0; i < n; ++i) {
     const fem_model* m = models[i];
     const uint64_t nn = m->node_number.size();
-    const bool flat = m->set[PLATE].size() && !m->set[TRUSS].size() && !m->set[BEAM].size() && plates_flat_identity(m);
-    if (flat && nn <= plate_cap) {
-      if (cur_plate < 0 || gnodes[size_t(cur_plate)] + nn > plate_cap) {
-        cur_plate = int(gnodes.size());
-        gnodes.push_back(0);
-        gplate.push_back(true);
-      }
-      grp[i] = cur_plate;
+    const bool plates_only = m->set[PLATE].size() && !m->set[TRUSS].size() && !m->set[BEAM].size();
+    if (plates_only && nn <= plate_cap && plates_flat_identity(m)) {
+      if (cur_plate >= 0 && gnodes[size_t(cur_plate)] + nn > plate_cap) cur_plate = -1;
+      grp[i] = open_group(cur_plate);
+    } else if (!m->set[PLATE].size()) {
+      grp[i] = open_group(bars);
     } else {
-      if (generic < 0) {
-        generic = int(gnodes.size());
-        gnodes.push_back(0);
-        gplate.push_back(false);
-      }
-      grp[i] = generic;
+      grp[i] = open_group(generic);
     }
     gnodes[size_t(grp[i])] += nn;
   }
@@ -1389,6 +1455,8 @@ int fem_batch_autotune(fem_batch* b) {
             if (rc) return rc;
             if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
           }
+        } else if (g->path == "bar_owner") {
+          cfg = {pn, g->pf_wg_per_cu, 256};  // fixed workgroup shape (node ranges cut on the host)
         } else {
           for (int bs : {128, 256, 512}) {
             femdev::set_generic_block(g->dev, bs);
@@ -1424,8 +1492,9 @@ int fem_batch_autotune(fem_batch* b) {
       b->config += "group " + std::to_string(k) + ": plate_fused patch_nodes=" + std::to_string(best[k][0]) +
                    " wg_per_cu=" + std::to_string(best[k][1]) + "; ";
     } else {
-      femdev::set_generic_block(g->dev, best[k][2]);
-      b->config += "group " + std::to_string(k) + ": generic block=" + std::to_string(best[k][2]) + "; ";
+      if (g->path == "generic") femdev::set_generic_block(g->dev, best[k][2]);
+      b->config += "group " + std::to_string(k) + ": " + g->path +
+                   (g->path == "generic" ? " block=" + std::to_string(best[k][2]) : std::string()) + "; ";
     }
   }
   return batch_capture(b);

@@ -150,12 +150,13 @@ int fem_update_nodes(fem_model* m, const double* xyz_device, void* stream);
  *
  * Many independent meshes assembled by one hipGraph launch.  The meshes'
  * nodes and elements are copied (at create) into per-group disjoint-union
- * models: flat plate-only meshes (fused plate kernel) and all others (generic
- * path); the input handles are not modified and may be destroyed afterwards.
- * Each mesh keeps its own CSR: its rows and values are contiguous slices of
- * its group's CSR, and every value is summed in the mesh's own element order,
- * so the results equal those of assembling the mesh alone.  All meshes must
- * share rel_tol and abs_tol. */
+ * models: flat plate-only meshes (fused plate kernel), truss and beam meshes
+ * (node-owner kernel) and all others (generic path); the input handles are
+ * not modified and may be destroyed afterwards.  Each mesh keeps its own CSR:
+ * its rows and values are contiguous slices of its group's CSR, and every
+ * value is summed in the mesh's own element order by the same value path as
+ * when the mesh is assembled alone, so the results are identical.  All
+ * meshes must share rel_tol and abs_tol. */
 typedef struct fem_batch fem_batch;
 int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out);
 void fem_batch_destroy(fem_batch* b);

@@ -31,7 +31,7 @@ def test_batch_matches_single_assembly_and_oracle(gpu):
     meshes = _meshes()
     models = [FEM.from_mesh(m) for m in meshes]
     batch = FEMBatch(models)
-    assert batch.groups() == 2  # flat plates | the rest
+    assert batch.groups() == 3  # flat plates | trusses and beams | the rest
     batch.assemble()
     for i, (mesh, model) in enumerate(zip(meshes, models)):
         print("mesh", i, mesh.kind, mesh.n_elements, flush=True)
@@ -49,7 +49,7 @@ def test_batch_autotune_and_replay(gpu):
     batch = FEMBatch(models)
     cfg = batch.autotune()
     print("config", cfg, flush=True)
-    assert "plate_fused" in cfg and "generic" in cfg
+    assert "plate_fused" in cfg and "bar_owner" in cfg
     batch.assemble()
     first = [batch.mesh_csr(i) for i in range(len(meshes))]
     for _ in range(3):

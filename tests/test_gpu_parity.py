@@ -21,7 +21,8 @@ CASES = {
 }
 
 EXPECTED_PATH = {"plate_8": "plate_fused", "plate_17x9": "plate_fused", "plate_64_shuffled": "plate_fused",
-                 "plate_12_tilted": "generic", "beam_frame_6": "generic", "truss_lattice_5": "generic"}
+                 "plate_12_tilted": "generic", "beam_frame_6": "bar_owner", "truss_lattice_5": "bar_owner",
+                 "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner"}
 
 
 def _tilt():
@@ -44,6 +45,19 @@ def test_assembly_matches_oracle(gpu, name):
         assert m.assembly_path() == EXPECTED_PATH[name]
     assert r["pattern_equal"]
     assert r["max_ratio"] <= 1.0
+
+
+@pytest.mark.parametrize("name", ["plate_17x9", "beam_frame_6", "truss_lattice_5"])
+def test_generic_path_matches_oracle(gpu, name):
+    """The generic two-pass path (forced) on meshes that normally take a fused path."""
+    mesh = CASES[name]()
+    ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)
+    m = FEM.from_mesh(mesh)
+    m.set_option("force_generic", 1)
+    got = m.stiffness_csr()
+    assert m.assembly_path() == "generic"
+    r = compare(got, ref)
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r
 
 
 @pytest.mark.parametrize("name", ["plate_8", "plate_64_shuffled", "beam_frame_6", "truss_lattice_5"])
