@@ -80,25 +80,6 @@ __device__ inline void gather_block(const AdjArgs& A, uint32_t k, uint32_t a, do
   }
 }
 
-// Flat standard blocks (frame = identity, masks (u,v)^2 + (w,tx,ty)^2 and the
-// drilling diagonal on self blocks): 13 or 14 entries at fixed places.
-__device__ inline void gather_block_flat(const AdjArgs& A, uint32_t k, uint32_t a, bool self, double G[36]) {
-  int64_t rs[6];
-#pragma unroll
-  for (int i = 0; i < 6; ++i) rs[i] = A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i];
-#pragma unroll
-  for (int i = 0; i < 36; ++i) G[i] = 0.0;
-  G[0] = A.g[rs[0]];
-  G[1] = A.g[rs[0] + 1];
-  G[6] = A.g[rs[1]];
-  G[7] = A.g[rs[1] + 1];
-#pragma unroll
-  for (int i = 2; i < 5; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) G[i * 6 + 2 + j] = A.g[rs[i] + j];
-  if (self) G[35] = A.g[rs[5]];
-}
-
 __device__ inline bool is_identity(const double R[9]) {
   return R[0] == 1.0 && R[4] == 1.0 && R[8] == 1.0 && R[1] == 0.0 && R[2] == 0.0 && R[3] == 0.0 && R[5] == 0.0 &&
          R[6] == 0.0 && R[7] == 0.0;
@@ -145,9 +126,48 @@ __device__ inline double quad_sum(double v) {
 
 // ---------------------------------------------------------------- plates
 // Four lanes per element (lane p = row block of node p), 64 elements per block.
-// FLAT (dL/dK mode only): every plate frame is the identity and every block
-// standard; then the frame-rotation term vanishes identically (the pattern has
-// no in-plane / out-of-plane couplings) and the blocks have fixed layouts.
+// Flat standard block (13 values, 14 on the diagonal) held compactly:
+// (u,u) (u,v) (v,u) (v,v) | (w,tx,ty) x (w,tx,ty) row-major | (tz,tz)
+struct FlatBlock {
+  double v[14];
+};
+
+__device__ inline void load_flat(const AdjArgs& A, const int64_t rp[6], const uint32_t bo[6], bool self,
+                                 FlatBlock& g) {
+  g.v[0] = A.g[rp[0] + bo[0]];
+  g.v[1] = A.g[rp[0] + bo[0] + 1];
+  g.v[2] = A.g[rp[1] + bo[1]];
+  g.v[3] = A.g[rp[1] + bo[1] + 1];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g.v[4 + 3 * i + j] = A.g[rp[2 + i] + bo[2 + i] + j];
+  g.v[13] = self ? A.g[rp[5] + bo[5]] : 0.0;
+}
+
+__device__ inline void expand_flat(const FlatBlock& g, double G[36]) {
+#pragma unroll
+  for (int i = 0; i < 36; ++i) G[i] = 0.0;
+  G[0] = g.v[0];
+  G[1] = g.v[1];
+  G[6] = g.v[2];
+  G[7] = g.v[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) G[(2 + i) * 6 + 2 + j] = g.v[4 + 3 * i + j];
+  G[35] = g.v[13];
+}
+
+__device__ inline void load_boff(const AdjArgs& A, uint32_t k, uint32_t bo[6]) {
+#pragma unroll
+  for (int i = 0; i < 6; ++i) bo[i] = A.boff[6 * size_t(k) + i];
+}
+
+// FLAT: every plate frame is the identity (constants instead of loads).  In
+// dL/dK mode every block is also standard: then the frame-rotation term
+// vanishes identically (the pattern has no in-plane / out-of-plane
+// couplings) and the blocks have fixed layouts.
 template <int MODE, bool FLAT>
 __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
@@ -184,6 +204,7 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   const fem::PlateMat mat = fem::plate_material(E, nu, t, kk);
   fem::PlateShear sv;
   fem::plate_shear_vectors(xl, sv);
+  const uint32_t ap = p == 0 ? nd[0] : (p == 1 ? nd[1] : (p == 2 ? nd[2] : nd[3]));
   // pass 1: integrals of row p
   double acc[24];
 #pragma unroll
@@ -198,32 +219,61 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
     const double wbe = g.w * fem::sel4(g.be[0], g.be[1], g.be[2], g.be[3], p);
     fem::plate_row_accumulate(g, wdx, wdy, wal, wbe, acc);
   }
-  // pass 2: gradient blocks of row p
-  double P[7], c[24], gxl[6], hx = 0.0, hy = 0.0;
+  // pass 2: gradient blocks of row p.  acc[] doubles as c[]: column q's
+  // integral coefficients replace its integrals once they are consumed.
+  double* c = acc;
+  double P[7], gxl[6], hx = 0.0, hy = 0.0;
 #pragma unroll
   for (int i = 0; i < 7; ++i) P[i] = 0.0;
 #pragma unroll
   for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
-  const uint32_t ap = p == 0 ? nd[0] : (p == 1 ? nd[1] : (p == 2 ? nd[2] : nd[3]));
   double up[6];
   if (MODE == MODE_U) local_u(A.g, ap, R, up);
+  double vp_[3], up_[3];
 #pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vp_[i] = fem::sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], p);
+    up_[i] = fem::sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], p);
+  }
+  // one block per iteration (q is wave-uniform): only one block's temporaries live
+#pragma unroll 1
   for (int q = 0; q < 4; ++q) {
     double gl[36];
     if (MODE == MODE_DLDK && FLAT) {
-      gather_block_flat(A, A.bid[16 * e + 4 * p + q], ap, p == q, gl);
+      const uint32_t k = A.bid[16 * e + 4 * p + q];
+      int64_t rp[6];
+      uint32_t bo[6];
+#pragma unroll
+      for (int i = 0; i < 6; ++i) rp[i] = A.row_ptr[6 * size_t(ap) + i];
+      load_boff(A, k, bo);
+      FlatBlock g;
+      load_flat(A, rp, bo, p == q, g);
+      expand_flat(g, gl);
     } else if (MODE == MODE_DLDK) {
       gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
       if (!ident) to_local(gl, R);
     } else {
       double uq[6];
-      local_u(A.g, nd[q], R, uq);
+      local_u(A.g, q == 0 ? nd[0] : (q == 1 ? nd[1] : (q == 2 ? nd[2] : nd[3])), R, uq);
 #pragma unroll
       for (int i = 0; i < 6; ++i)
 #pragma unroll
         for (int j = 0; j < 6; ++j) gl[i * 6 + j] = -(up[i] * uq[j]);
     }
-    fem::plate_block_adjoint(acc, mat, sv, p, q, gl, P, c, gxl, hx, hy, !FLAT);
+    double Mq[6], vq[3], uq_[3], cq[6];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) Mq[k] = fem::sel4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3], q);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      vq[i] = fem::sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], q);
+      uq_[i] = fem::sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], q);
+    }
+    // the frame term vanishes for flat dL/dK blocks only (dense -u u^T blocks keep it)
+    fem::plate_block_adjoint_q(Mq, mat, vp_, up_, vq, uq_, p, q, gl, P, cq, gxl, hx, hy, !(FLAT && MODE == MODE_DLDK));
+#pragma unroll
+    for (int k = 0; k < 6; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[4 * k + j] = j == q ? cq[k] : c[4 * k + j];
   }
   // pass 3: reverse sweep over the integration points
 #pragma unroll 1
@@ -412,6 +462,7 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
       const unsigned blocks = unsigned((n + 63) / 64);
       if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<MODE_DLDK, true><<<blocks, 256, 0, s>>>(A);
       else if (mode == MODE_DLDK) plate_adjoint_kernel<MODE_DLDK, false><<<blocks, 256, 0, s>>>(A);
+      else if (d->adj_flat) plate_adjoint_kernel<MODE_U, true><<<blocks, 256, 0, s>>>(A);
       else plate_adjoint_kernel<MODE_U, false><<<blocks, 256, 0, s>>>(A);
     } else {
       const unsigned blocks = unsigned((n + 255) / 256);

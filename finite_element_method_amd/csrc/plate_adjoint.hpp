@@ -147,14 +147,17 @@ __host__ __device__ inline void plate_frame_adjoint_sub(const double* k, const d
   }
 }
 
-// Per-lane share of the adjoint sums for block q of row p.  acc: the row's
-// integrals (plate_row_accumulate), sv: shear vectors, gl: local-frame
-// gradient block (p, q), 6x6 row-major.  c receives the integral coefficients
-// of column q (layout of plate_point_adjoint).
-__host__ __device__ inline void plate_block_adjoint(const double acc[24], const PlateMat& m, const PlateShear& sv,
-                                                    int p, int q, const double gl[36], double P[7], double c[24],
-                                                    double gxl[6], double& hx, double& hy, bool frame = true) {
-  const double Mxx = acc[q], Mxy = acc[4 + q], Myx = acc[8 + q], Myy = acc[12 + q], A = acc[16 + q], B = acc[20 + q];
+// Per-lane share of the adjoint sums for block q of row p.  Mq: the six
+// integrals of column q (Mxx, Mxy, Myx, Myy, A, B; plate_row_accumulate),
+// vp/up and vq/uq: the unscaled shear vectors of nodes p and q, gl: the
+// local-frame gradient block (p, q), 6x6 row-major.  cq receives the integral
+// coefficients of column q (layout of plate_point_adjoint: cxx, cxy, cyx, cyy,
+// cA, cB).
+__host__ __device__ inline void plate_block_adjoint_q(const double Mq[6], const PlateMat& m, const double vp[3],
+                                                      const double up[3], const double vq[3], const double uq[3],
+                                                      int p, int q, const double gl[36], double P[7], double cq[6],
+                                                      double gxl[6], double& hx, double& hy, bool frame = true) {
+  const double Mxx = Mq[0], Mxy = Mq[1], Myx = Mq[2], Myy = Mq[3], A = Mq[4], B = Mq[5];
   const double Guu = gl[0], Guv = gl[1], Gvu = gl[6], Gvv = gl[7];
   const double Gxx = gl[21], Gxy = gl[22], Gyx = gl[27], Gyy = gl[28];
   // constitutive partials
@@ -165,12 +168,6 @@ __host__ __device__ inline void plate_block_adjoint(const double acc[24], const 
   P[4] -= Gxy * Myx + Gyx * Mxy;
   P[5] += ((Gxx * Mxx - Gxy * Mxy) - Gyx * Myx) + Gyy * Myy;
   // shear: Gs = gl[2..4][2..4]; v_p^T Gs v_q, u_p^T Gs u_q and the one-sided products
-  double vp[3], up[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i) {
-    vp[i] = sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], p);
-    up[i] = sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], p);
-  }
   double Gv_q[3], Gu_q[3], vG[3], uG[3];  // Gs v_q, Gs u_q, v_p^T Gs, u_p^T Gs
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -181,8 +178,8 @@ __host__ __device__ inline void plate_block_adjoint(const double acc[24], const 
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       const double gs = gl[(2 + i) * 6 + 2 + j];
-      Gv_q[i] += gs * sv.v[q][j];
-      Gu_q[i] += gs * sv.u[q][j];
+      Gv_q[i] += gs * vq[j];
+      Gu_q[i] += gs * uq[j];
       vG[j] += vp[i] * gs;
       uG[j] += up[i] * gs;
     }
@@ -190,12 +187,12 @@ __host__ __device__ inline void plate_block_adjoint(const double acc[24], const 
   const double su_ = (up[0] * Gu_q[0] + up[1] * Gu_q[1]) + up[2] * Gu_q[2];
   P[6] += A * sv_ + B * su_;
   // integral coefficients of column q
-  c[q] = ((m.cm00 * Guu + m.cm22 * Gvv) + m.cb22 * Gxx) + m.cb00 * Gyy;
-  c[4 + q] = ((m.cm01 * Guv + m.cm22 * Gvu) - m.cb22 * Gxy) - m.cb01 * Gyx;
-  c[8 + q] = ((m.cm22 * Guv + m.cm01 * Gvu) - m.cb01 * Gxy) - m.cb22 * Gyx;
-  c[12 + q] = ((m.cm22 * Guu + m.cm00 * Gvv) + m.cb00 * Gxx) + m.cb22 * Gyy;
-  c[16 + q] = m.cs * sv_;
-  c[20 + q] = m.cs * su_;
+  cq[0] = ((m.cm00 * Guu + m.cm22 * Gvv) + m.cb22 * Gxx) + m.cb00 * Gyy;
+  cq[1] = ((m.cm01 * Guv + m.cm22 * Gvu) - m.cb22 * Gxy) - m.cb01 * Gyx;
+  cq[2] = ((m.cm22 * Guv + m.cm01 * Gvu) - m.cb01 * Gxy) - m.cb22 * Gyx;
+  cq[3] = ((m.cm22 * Guu + m.cm00 * Gvv) + m.cb00 * Gxx) + m.cb22 * Gyy;
+  cq[4] = m.cs * sv_;
+  cq[5] = m.cs * su_;
   // shear vectors: d/dv_p = cs A Gs v_q, d/dv_q = cs A v_p^T Gs (same with u, B)
   const double sa = m.cs * A, sb = m.cs * B;
   plate_shear_vector_adjoint(p, sa * Gv_q[1], sa * Gv_q[2], sb * Gu_q[1], sb * Gu_q[2], gxl);
@@ -213,7 +210,7 @@ __host__ __device__ inline void plate_block_adjoint(const double acc[24], const 
     vps[i] = m.cs * vp[i];
     ups[i] = m.cs * up[i];
   }
-  plate_block_entries(Mxx, Mxy, Myx, Myy, A, B, m, vps, ups, sv.v[q], sv.u[q], o);
+  plate_block_entries(Mxx, Mxy, Myx, Myy, A, B, m, vps, ups, vq, uq, o);
   kb[0] = o[0]; kb[1] = o[1]; kb[6] = o[2]; kb[7] = o[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i)
@@ -224,6 +221,24 @@ __host__ __device__ inline void plate_block_adjoint(const double acc[24], const 
   plate_frame_adjoint_sub(kb, gl, 0, 3, hx, hy);
   plate_frame_adjoint_sub(kb, gl, 3, 0, hx, hy);
   plate_frame_adjoint_sub(kb, gl, 3, 3, hx, hy);
+}
+
+// Same for block q of row p given the whole row's integrals acc[24] and the
+// element's shear vectors; writes column q of c[24].
+__host__ __device__ inline void plate_block_adjoint(const double acc[24], const PlateMat& m, const PlateShear& sv,
+                                                    int p, int q, const double gl[36], double P[7], double c[24],
+                                                    double gxl[6], double& hx, double& hy, bool frame = true) {
+  const double Mq[6] = {acc[q], acc[4 + q], acc[8 + q], acc[12 + q], acc[16 + q], acc[20 + q]};
+  double vp[3], up[3];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    vp[i] = sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], p);
+    up[i] = sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], p);
+  }
+  double cq[6];
+  plate_block_adjoint_q(Mq, m, vp, up, sv.v[q], sv.u[q], p, q, gl, P, cq, gxl, hx, hy, frame);
+#pragma unroll
+  for (int k = 0; k < 6; ++k) c[4 * k + q] = cq[k];
 }
 
 // Element-level finish from the four lanes' sums.

@@ -101,7 +101,7 @@ __device__ __forceinline__ void pf_load_maps(PatchRegs& P, const PatchArgs& A, i
   const uint32_t le = min(ut >> 2, P.nel - 1);
   P.c = A.pconn[P.elem_off + le];
   P.eid = A.peid[P.elem_off + le];
-  P.row = A.rows[P.row_off + min(ut, P.nrows - 1)];
+  P.row = A.rows[P.row_off + min(ut >> 1, P.nrows - 1)];  // two lanes per owned row
   const uint32_t o = P.node_off + min(ut, P.nown - 1);
   P.base = A.obase[o];
   P.info = A.oinfo[o];
@@ -178,36 +178,47 @@ __device__ __forceinline__ bool pf_process(PatchRegs& cur, PatchRegs& nxt, int p
   __syncthreads();
   pf_load_maps(nxt, A, t);
 
-  // ---- step 2: owned rows folded over the four points (24 integrals per row)
-  double acc[24];
+  // ---- step 2: owned rows folded over the four points; two lanes per row,
+  //      lane half h takes the column nodes q = 2h, 2h + 1 (12 integrals each)
+  const int row = t >> 1, h = t & 1;
+  double acc[12];
   int p = 0, re = 0;
-  if (t < int(cur.nrows)) {
+  if (row < int(cur.nrows)) {
     re = int(cur.row & 255u);
     p = int(cur.row >> 8);
 #pragma unroll
-    for (int k = 0; k < 24; ++k) acc[k] = 0.0;
+    for (int k = 0; k < 12; ++k) acc[k] = 0.0;
 #pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-      const double2* gi = geo2 + (re * 4 + q) * (PF_GEO / 2);
+    for (int ip = 0; ip < 4; ++ip) {
+      const double2* gi = geo2 + (re * 4 + ip) * (PF_GEO / 2);
       double gv[18];
 #pragma unroll
-      for (int h = 0; h < 9; ++h) { const double2 v = gi[h]; gv[2 * h] = v.x; gv[2 * h + 1] = v.y; }
-      fem::PlatePoint g;
-      g.w = gv[0];
+      for (int k = 0; k < 9; ++k) { const double2 v = gi[k]; gv[2 * k] = v.x; gv[2 * k + 1] = v.y; }
+      const double w = gv[0];
+      const double wdx = w * fem::sel4(gv[1], gv[2], gv[3], gv[4], p);
+      const double wdy = w * fem::sel4(gv[5], gv[6], gv[7], gv[8], p);
+      const double wal = w * fem::sel4(gv[9], gv[10], gv[11], gv[12], p);
+      const double wbe = w * fem::sel4(gv[13], gv[14], gv[15], gv[16], p);
 #pragma unroll
-      for (int a = 0; a < 4; ++a) { g.dx[a] = gv[1 + a]; g.dy[a] = gv[5 + a]; g.al[a] = gv[9 + a]; g.be[a] = gv[13 + a]; }
-      const double wdx = g.w * fem::sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p);
-      const double wdy = g.w * fem::sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
-      const double wal = g.w * fem::sel4(g.al[0], g.al[1], g.al[2], g.al[3], p);
-      const double wbe = g.w * fem::sel4(g.be[0], g.be[1], g.be[2], g.be[3], p);
-      fem::plate_row_accumulate(g, wdx, wdy, wal, wbe, acc);
+      for (int c = 0; c < 2; ++c) {
+        const int q = 2 * h + c;
+        const double dxq = h ? gv[3 + c] : gv[1 + c], dyq = h ? gv[7 + c] : gv[5 + c];
+        const double alq = h ? gv[11 + c] : gv[9 + c], beq = h ? gv[15 + c] : gv[13 + c];
+        (void)q;
+        acc[6 * c + 0] += wdx * dxq;
+        acc[6 * c + 1] += wdx * dyq;
+        acc[6 * c + 2] += wdy * dxq;
+        acc[6 * c + 3] += wdy * dyq;
+        acc[6 * c + 4] += wal * alq;
+        acc[6 * c + 5] += wbe * beq;
+      }
     }
-    L.stage_p[t] = p;
+    if (h == 0) L.stage_p[row] = p;
   }
   __syncthreads();
 
-  // ---- step 3: row blocks from the integrals, staged over the point data
-  if (t < int(cur.nrows)) {
+  // ---- step 3: the row's blocks q = 2h, 2h + 1 from the integrals, staged over the point data
+  if (row < int(cur.nrows)) {
     const double* ed = L.edat + re * PF_EDAT;
     fem::PlateMat m;
     m.cm00 = ed[0]; m.cm01 = ed[1]; m.cm22 = ed[2]; m.cb00 = ed[3]; m.cb01 = ed[4]; m.cb22 = ed[5]; m.cs = ed[6];
@@ -215,16 +226,17 @@ __device__ __forceinline__ bool pf_process(PatchRegs& cur, PatchRegs& nxt, int p
 #pragma unroll
     for (int i = 0; i < 3; ++i) { vp[i] = m.cs * ed[8 + 3 * p + i]; up[i] = m.cs * ed[20 + 3 * p + i]; }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int c = 0; c < 2; ++c) {
+      const int q = 2 * h + c;
       double vq[3], uq[3], o[14];
 #pragma unroll
       for (int i = 0; i < 3; ++i) { vq[i] = ed[8 + 3 * q + i]; uq[i] = ed[20 + 3 * q + i]; }
-      fem::plate_block_entries(acc[q], acc[4 + q], acc[8 + q], acc[12 + q], acc[16 + q], acc[20 + q], m, vp, up,
-                               vq, uq, o);
+      fem::plate_block_entries(acc[6 * c], acc[6 * c + 1], acc[6 * c + 2], acc[6 * c + 3], acc[6 * c + 4],
+                               acc[6 * c + 5], m, vp, up, vq, uq, o);
       o[13] = 0.0;
-      double2* dst = L.region + (t * PF_SLOT + PF_BS * q) / 2;
+      double2* dst = L.region + (row * PF_SLOT + PF_BS * q) / 2;
 #pragma unroll
-      for (int h = 0; h < 7; ++h) dst[h] = make_double2(o[2 * h], o[2 * h + 1]);
+      for (int k = 0; k < 7; ++k) dst[k] = make_double2(o[2 * k], o[2 * k + 1]);
     }
   }
   __syncthreads();
