@@ -97,6 +97,40 @@ def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream):
                                     "algorithmic_bytes_per_launch": cb}}
 
 
+def separation(model, mesh, nnz, steps, warmup):
+    """Sparse separation (K_aa, K_ab, K_ba, K_bb, k_aa/k_bb indexes, r_a, u_b)
+    of the assembled K with the x = min face restrained (plates: Z, ThX, ThY;
+    trusses: X, Y, Z; beams: all six dofs) and a load on every other node's Z.
+    Kernel time from HIP events around the separation kernels; the host read
+    of the result sizes is outside it."""
+    x = mesh.xyz[:, 0]
+    face = np.nonzero(np.isclose(x, x.min()))[0]
+    dofs = {"plate": (2, 3, 4), "truss": (0, 1, 2), "beam": tuple(range(6))}[mesh.kind]
+    numbers = np.repeat(mesh.node_numbers[face], len(dofs))
+    model.add_displacements(numbers, np.tile(dofs, len(face)), np.zeros(len(numbers)))
+    rest = np.setdiff1d(np.arange(mesh.n_nodes), face)
+    model.add_concentrated_loads(mesh.node_numbers[rest], np.full(len(rest), 2), np.full(len(rest), -1.0))
+    n_aa, n_bb, nnz_sep = model.separate(assemble=True)  # uploads the boundary conditions
+    for _ in range(warmup):
+        model.separate(assemble=False)
+    t0 = time.perf_counter()
+    kern = 0.0
+    for _ in range(steps):
+        model.separate(assemble=False)
+        kern += model.last_separate_ms()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    kern /= steps
+    n = 6 * mesh.n_nodes
+    kept = sum(nnz_sep.values())
+    # compulsory bytes: the CSR read once, the four blocks, their row pointers,
+    # the index vectors and r_a / u_b written once, the restraint flags read once
+    alg = 12 * nnz + 8 * (n + 1) + n + 12 * kept + 8 * (2 * (n_aa + n_bb) + 4) + 16 * (n_aa + n_bb)
+    return {"separate_ms": kern, "separate_wall_ms": wall_ms, "n_aa": n_aa, "n_bb": n_bb, "nnz": nnz_sep,
+            "roofline": {"bound": "hbm", "achieved": alg / (kern * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": alg / (kern * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_launch": alg}}
+
+
 def run_batched(args):
     """Batched workload: 256 independent meshes (type uniform over truss
     lattice / beam frame / plate, element counts log-uniform in [1e3, 2e5]),
@@ -184,6 +218,8 @@ def main():
     ap.add_argument("--workload", default="plate_512", choices=sorted(WORKLOADS) + ["batched_256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the backward / sensitivity timings")
+    ap.add_argument("--no-separate", action="store_true", help="skip the sparse separation timing")
+    ap.add_argument("--separate-lanes", type=int, default=0, help="separation: lanes per CSR row (0 = automatic)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (a known-traffic kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -216,6 +252,8 @@ def main():
         model.set_option("plate_patch_nodes", args.patch_nodes)
     if args.wg_per_cu >= 0:
         model.set_option("plate_wg_per_cu", args.wg_per_cu)
+    if args.separate_lanes:
+        model.set_option("separate_lanes", args.separate_lanes)
     t0 = time.perf_counter()
     model.build_pattern()
     setup_s = time.perf_counter() - t0
@@ -266,6 +304,8 @@ def main():
     }
     if not args.no_grad:
         out["sensitivity"] = sensitivity(model, mesh, nnz, val, ms, args.steps, args.warmup, stream)
+    if not args.no_separate:
+        out["separation"] = separation(model, mesh, nnz, args.steps, args.warmup)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh.kind)
     print(json.dumps(out), flush=True)

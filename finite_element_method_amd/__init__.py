@@ -15,13 +15,29 @@ the assembly raises.
 """
 
 import ctypes
+import enum
 import os
 
 import numpy as np
 
-__all__ = ["FEM", "FEMBatch", "FemError", "DeviceError", "TRUSS", "BEAM", "PLATE", "lib"]
+__all__ = ["FEM", "FEMBatch", "FemError", "DeviceError", "DOFParameter", "TRUSS", "BEAM", "PLATE", "lib"]
 
 TRUSS, BEAM, PLATE = 0, 1, 2
+SEPARATED = ("aa", "ab", "ba", "bb")
+
+
+class DOFParameter(enum.IntEnum):
+    """The crate's DOFParameter (methods_for_bc_data_handle.rs:6-14)."""
+    X = 0
+    Y = 1
+    Z = 2
+    ThX = 3
+    ThY = 4
+    ThZ = 5
+
+
+def _dof(d):
+    return int(DOFParameter[d]) if isinstance(d, str) else int(d)
 PARAMS_PER = {TRUSS: 3, BEAM: 11, PLATE: 4}
 FEM_OK, FEM_ERR_VALIDATION, FEM_ERR_STATE, FEM_ERR_DEVICE, FEM_ERR_ARGUMENT = 0, 1, 2, 3, 4
 
@@ -73,6 +89,15 @@ def lib():
         "fem_last_backward_ms": (dbl, [vp]),
         "fem_update_params": (ci, [vp, ci, vp, vp]),
         "fem_update_nodes": (ci, [vp, vp, vp]),
+        "fem_add_displacements": (ci, [vp, sz, vp, vp, vp, psz]),
+        "fem_add_concentrated_loads": (ci, [vp, sz, vp, vp, vp, psz]),
+        "fem_separate": (ci, [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i64), vp]),
+        "fem_separated_device": (ci, [vp, ci, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "fem_separated_to_host": (ci, [vp, ci, vp, vp, vp]),
+        "fem_separated_vectors_device": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                              ctypes.POINTER(vp)]),
+        "fem_separated_vectors_to_host": (ci, [vp, vp, vp, vp, vp]),
+        "fem_last_separate_ms": (dbl, [vp]),
         "fem_batch_create": (ci, [vp, sz, ctypes.POINTER(vp)]),
         "fem_batch_destroy": (None, [vp]),
         "fem_batch_last_error": (ctypes.c_char_p, [vp]),
@@ -269,6 +294,79 @@ class FEM:
 
     def assembly_path(self):
         return lib().fem_assembly_path(self._h).decode()
+
+    # ----------------------------------------------------------- boundary conditions
+    def add_displacements(self, numbers, dofs, values):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        dofs = np.ascontiguousarray([_dof(d) for d in np.atleast_1d(dofs)], dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        if not len(numbers) == len(dofs) == len(values):
+            raise ValueError("numbers, dofs and values differ in length")
+        return self._batched(lib().fem_add_displacements, len(numbers), numbers, dofs, values)
+
+    def add_concentrated_loads(self, numbers, dofs, values):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        dofs = np.ascontiguousarray([_dof(d) for d in np.atleast_1d(dofs)], dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        if not len(numbers) == len(dofs) == len(values):
+            raise ValueError("numbers, dofs and values differ in length")
+        return self._batched(lib().fem_add_concentrated_loads, len(numbers), numbers, dofs, values)
+
+    def add_displacement(self, node_number, dof_parameter, value):
+        """FEM::add_displacement (methods_for_bc_data_handle.rs:176-203)."""
+        self.add_displacements([node_number], [dof_parameter], [value])
+
+    def add_concentrated_load(self, node_number, dof_parameter, value):
+        """FEM::add_concentrated_load (methods_for_bc_data_handle.rs:34-56)."""
+        self.add_concentrated_loads([node_number], [dof_parameter], [value])
+
+    def separate(self, assemble=True, stream=None):
+        """Partitions the assembled stiffness on the GPU
+        (separate_stiffness_matrix_sparse_iterative,
+        methods_for_separate_stiffness_matrix.rs:217-320).  Returns
+        (n_aa, n_bb, {"aa" | "ab" | "ba" | "bb": nnz}); synchronous."""
+        if assemble:
+            self.assemble(stream)
+        n_aa, n_bb = ctypes.c_int64(), ctypes.c_int64()
+        nnz = np.zeros(4, dtype=np.int64)
+        self._call(lib().fem_separate(self._h, _stream_handle(stream), ctypes.byref(n_aa), ctypes.byref(n_bb),
+                                      _ptr(nnz)))
+        return n_aa.value, n_bb.value, dict(zip(SEPARATED, (int(x) for x in nnz)))
+
+    def separate_sparse(self, assemble=True, stream=None):
+        """Separation copied to the host: ({"aa","ab","ba","bb": (row_ptr,
+        col_idx, val)}, k_aa_indexes, k_bb_indexes, r_a, u_b) -- the crate's
+        SeparatedStiffnessMatrixSparse with compose_r_a_vector /
+        compose_u_b_vector (:323-345)."""
+        n_aa, n_bb, nnz = self.separate(assemble, stream)
+        mats = {}
+        for w, name in enumerate(SEPARATED):
+            rows = n_aa if w < 2 else n_bb
+            rp = np.zeros(rows + 1, dtype=np.int64)
+            ci = np.zeros(nnz[name], dtype=np.int32)
+            v = np.zeros(nnz[name], dtype=np.float64)
+            self._call(lib().fem_separated_to_host(self._h, w, _ptr(rp), _ptr(ci), _ptr(v)))
+            mats[name] = (rp, ci, v)
+        aa, bb = np.zeros(n_aa, dtype=np.int64), np.zeros(n_bb, dtype=np.int64)
+        ra, ub = np.zeros(n_aa), np.zeros(n_bb)
+        self._call(lib().fem_separated_vectors_to_host(self._h, _ptr(aa), _ptr(bb), _ptr(ra), _ptr(ub)))
+        return mats, aa, bb, ra, ub
+
+    def separated_device(self, which):
+        """Borrowed device pointers (row_ptr, col_idx, val) of K_aa | K_ab | K_ba | K_bb."""
+        w = SEPARATED.index(which) if isinstance(which, str) else int(which)
+        rp, ci, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        self._call(lib().fem_separated_device(self._h, w, ctypes.byref(rp), ctypes.byref(ci), ctypes.byref(v)))
+        return rp.value, ci.value, v.value
+
+    def separated_vectors_device(self):
+        """Borrowed device pointers (k_aa_indexes, k_bb_indexes, r_a, u_b)."""
+        p = [ctypes.c_void_p() for _ in range(4)]
+        self._call(lib().fem_separated_vectors_device(self._h, *[ctypes.byref(x) for x in p]))
+        return tuple(x.value for x in p)
+
+    def last_separate_ms(self):
+        return lib().fem_last_separate_ms(self._h)
 
     # ----------------------------------------------------------- differentiation
     def model_size(self):

@@ -53,8 +53,23 @@ struct Device {
   int64_t adj_nxs = 0;
   bool adj_flat = false;
   double rel_tol = 0.0, abs_tol = 0.0;
+  // boundary conditions per global dof and the sparse separation; the four
+  // matrices (K_aa, K_ab, K_ba, K_bb) share sep_rp / sep_ci / sep_v
+  uint8_t *sep_cons = nullptr, *sep_cls = nullptr, *sep_tmp = nullptr;
+  double *sep_disp = nullptr, *sep_force = nullptr, *sep_ra = nullptr, *sep_ub = nullptr, *sep_v = nullptr;
+  uint64_t *sep_flag = nullptr, *sep_pos = nullptr;
+  int64_t *sep_key = nullptr, *sep_cnt = nullptr, *sep_off = nullptr, *sep_aa = nullptr, *sep_bb = nullptr;
+  int64_t *sep_rp = nullptr, *sep_sum = nullptr;
+  int32_t* sep_ci = nullptr;
+  int64_t* sep_dpos = nullptr;  // diagonal entry of each row (pattern-only)
+  bool sep_dpos_valid = false;
+  size_t sep_cap[19] = {};
+  int64_t sep_bc_rows = -1;
+  int64_t sep_rows[4] = {0, 0, 0, 0}, sep_nnz[4] = {0, 0, 0, 0}, sep_base[4] = {0, 0, 0, 0};
+  bool sep_done = false;
+  double sep_ms = 0.0;
   // timing
-  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
   bool timed = false, timed_bwd = false;
 };
 

@@ -38,7 +38,8 @@ Device* create_device(std::string& err) {
   auto* d = new Device();
   hipGetDevice(&d->dev);
   if (hipEventCreate(&d->ev0) != hipSuccess || hipEventCreate(&d->ev1) != hipSuccess ||
-      hipEventCreate(&d->ev2) != hipSuccess || hipEventCreate(&d->ev3) != hipSuccess) {
+      hipEventCreate(&d->ev2) != hipSuccess || hipEventCreate(&d->ev3) != hipSuccess ||
+      hipEventCreate(&d->ev4) != hipSuccess || hipEventCreate(&d->ev5) != hipSuccess) {
     err = "hipEventCreate failed";
     delete d;
     return nullptr;
@@ -58,7 +59,10 @@ void destroy_device(Device* d) {
   for (int t = 0; t < 2; ++t) f(d->bo_bid[t]);
   f(d->bo_bmask); f(d->bo_boff); f(d->bo_inc_ptr); f(d->bo_inc); f(d->bo_ranges);
   f(d->adj_bmask); f(d->adj_boff); f(d->adj_inc_ptr); f(d->adj_inc); f(d->adj_xs);
-  for (hipEvent_t ev : {d->ev0, d->ev1, d->ev2, d->ev3})
+  f(d->sep_cons); f(d->sep_cls); f(d->sep_tmp); f(d->sep_disp); f(d->sep_force); f(d->sep_ra); f(d->sep_ub);
+  f(d->sep_v); f(d->sep_flag); f(d->sep_pos); f(d->sep_key); f(d->sep_cnt); f(d->sep_off); f(d->sep_aa); f(d->sep_bb);
+  f(d->sep_rp); f(d->sep_sum); f(d->sep_ci); f(d->sep_dpos);
+  for (hipEvent_t ev : {d->ev0, d->ev1, d->ev2, d->ev3, d->ev4, d->ev5})
     if (ev) hipEventDestroy(ev);
   delete d;
 }
@@ -190,6 +194,8 @@ std::string get_rotation(Device* d, int type, size_t pos, double out[9]) {
 std::string upload_pattern(Device* d, int64_t n_rows, const int64_t* rp, int64_t nnz, const int32_t* ci) {
   d->n_rows = n_rows;
   d->nnz = nnz;
+  d->sep_dpos_valid = false;
+  d->sep_done = false;
   auto e = dupload(d->row_ptr, rp, size_t(n_rows + 1));
   if (!e.empty()) return e;
   e = dupload(d->col_idx, ci, size_t(nnz));

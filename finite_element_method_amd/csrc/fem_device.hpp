@@ -151,4 +151,21 @@ const int64_t* row_ptr(Device* d);
 const int32_t* col_idx(Device* d);
 std::string copy_csr_to_host(Device* d, int64_t n_rows, int64_t nnz, int64_t* rp, int32_t* ci, double* v);
 
+// Sparse separation (separate.hip).  upload_bc: per global dof the
+// constrained flag, imposed displacement and force.  separate_sparse
+// partitions the current CSR values; err_row is the first constrained dof
+// without stiffness (-1 if none).
+struct SepResult {
+  int64_t err_row = -1, n_aa = 0, n_bb = 0;
+  int64_t nnz[4] = {0, 0, 0, 0};  // K_aa, K_ab, K_ba, K_bb
+};
+std::string upload_bc(Device* d, const uint8_t* constrained, const double* disp, const double* force, size_t n);
+std::string separate_sparse(Device* d, SepResult& res, void* stream, bool timed, int lanes = 0);
+std::string separated_csr(Device* d, int which, const int64_t** rp, const int32_t** ci, const double** v);
+std::string separated_vectors(Device* d, const int64_t** aa, const int64_t** bb, const double** r_a,
+                              const double** u_b);
+void separation_shape(Device* d, int64_t rows[4], int64_t nnz[4]);
+void invalidate_separation(Device* d);
+double last_separate_ms(Device* d);
+
 }  // namespace femdev

@@ -107,6 +107,13 @@ struct fem_model {
   bool adj_valid = false;
   std::vector<int64_t> row_ptr_host;  // host copy of the pattern's row_ptr
   std::vector<uint32_t> node_part;    // batch groups: mesh of each node (major key of the patch order)
+  // boundary conditions per global dof (methods_for_bc_data_handle.rs): the
+  // crate's imposed_constraints, displacements_vector and forces_vector
+  std::vector<uint8_t> bc_cons;
+  std::vector<double> bc_disp, bc_force;
+  bool bc_dirty = true;
+  bool own_vals = false;  // the library's own CSR values hold the current stiffness
+  int sep_lanes = 0;      // separation: lanes per CSR row (tuning option, 0 = automatic)
 };
 
 // A batch of independent meshes assembled together (no counterpart in the
@@ -154,6 +161,12 @@ void clear_model(fem_model* m, uint32_t nodes_number) {
   m->adj_valid = false;
   m->nnz = 0;
   for (auto& r : m->rzero) r.clear();
+  m->bc_cons.clear();
+  m->bc_disp.clear();
+  m->bc_force.clear();
+  m->bc_dirty = true;
+  m->own_vals = false;
+  if (m->dev) femdev::invalidate_separation(m->dev);
 }
 
 bool node_exists(fem_model* m, uint32_t number) {  // methods_for_node_data_handle.rs:81-86
@@ -741,6 +754,7 @@ int ensure_ready(fem_model* m) {
     if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
     m->dirty = false;
     m->pattern_valid = false;
+    m->own_vals = false;
   }
   if (!m->pattern_valid) {
     auto e = build_pattern(m);
@@ -973,6 +987,69 @@ int time_group(fem_batch* b, fem_model* g, int reps, double& ms) {
   std::sort(t.begin(), t.end());
   ms = t[t.size() / 2];
   return FEM_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- boundary conditions
+namespace {
+
+const char* const DOF_NAMES[6] = {"X", "Y", "Z", "ThX", "ThY", "ThZ"};  // DOFParameter (Debug form)
+
+void ensure_bc(fem_model* m) {
+  const size_t n = size_t(m->nodes_number) * 6;
+  if (m->bc_cons.size() != n) {
+    m->bc_cons.assign(n, 0);
+    m->bc_disp.assign(n, 0.0);
+    m->bc_force.assign(n, 0.0);
+    m->bc_dirty = true;
+  }
+}
+
+// kind 0: FEM::add_displacement (methods_for_bc_data_handle.rs:176-203)
+// kind 1: FEM::add_concentrated_load (methods_for_bc_data_handle.rs:34-56)
+int add_bc(fem_model* m, int kind, size_t n, const uint32_t* numbers, const int32_t* dofs, const double* values,
+           size_t* n_ok) {
+  if (!m || (n && (!numbers || !dofs || !values))) return FEM_ERR_ARGUMENT;
+  size_t i = 0;
+  int rc = FEM_OK;
+  for (; i < n; ++i) {
+    const uint32_t number = numbers[i];
+    const int dof = dofs[i];
+    if (dof < 0 || dof > 5) {
+      rc = fail(m, FEM_ERR_ARGUMENT, "dof must be 0..5 (X, Y, Z, ThX, ThY, ThZ)");
+      break;
+    }
+    if (!node_exists(m, number)) {
+      rc = FEM_ERR_VALIDATION;
+      break;
+    }
+    ensure_bc(m);
+    const size_t index = size_t(m->node_idx[number]) * 6 + size_t(dof);
+    if (kind == 0) {
+      if (m->bc_cons[index]) {
+        rc = fail(m, FEM_ERR_VALIDATION,
+                  std::string("Displacement ") + DOF_NAMES[dof] + " already applied to node " + u(number) + "!");
+        break;
+      }
+      m->bc_cons[index] = 1;
+      m->bc_disp[index] = values[i];
+    } else {
+      m->bc_force[index] += values[i];
+    }
+    m->bc_dirty = true;
+  }
+  if (n_ok) *n_ok = i;
+  return rc;
+}
+
+int separation_ready(fem_model* m) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  int64_t rows[4];
+  if (!m->dev) return fail(m, FEM_ERR_STATE, "no separation computed: call fem_separate");
+  femdev::separation_shape(m->dev, rows, nullptr);
+  const std::string e = femdev::separated_vectors(m->dev, nullptr, nullptr, nullptr, nullptr);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_STATE, e + ": call fem_separate");
 }
 
 }  // namespace
@@ -1233,7 +1310,9 @@ int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
   std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
                   : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream)
                                            : femdev::assemble_generic(m->dev, val_device, stream);
-  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  if (val_device == femdev::values(m->dev)) m->own_vals = true;
+  return FEM_OK;
 }
 
 int fem_assemble(fem_model* m, void* stream) {
@@ -1296,6 +1375,7 @@ int fem_update_params(fem_model* m, int type, const double* params_device, void*
   if (rc) return rc;
   auto e = femdev::replace_params(m->dev, type, params_device, m->set[type].params.data(), stream);
   if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  m->own_vals = false;
   return type == BEAM ? rerun_setup(m, false, BEAM) : FEM_OK;
 }
 
@@ -1305,11 +1385,110 @@ int fem_update_nodes(fem_model* m, const double* xyz_device, void* stream) {
   if (rc) return rc;
   auto e = femdev::replace_nodes(m->dev, xyz_device, m->xyz.data(), stream);
   if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  m->own_vals = false;
   m->coord_hash.clear();
   for (size_t i = 0; i < m->node_number.size(); ++i)
     m->coord_hash[coord_key(m->xyz[3 * i], m->xyz[3 * i + 1], m->xyz[3 * i + 2])].push_back(uint32_t(i));
   return rerun_setup(m, true, 0);
 }
+
+int fem_add_displacements(fem_model* m, size_t n, const uint32_t* numbers, const int32_t* dofs, const double* values,
+                          size_t* n_ok) {
+  return add_bc(m, 0, n, numbers, dofs, values, n_ok);
+}
+
+int fem_add_concentrated_loads(fem_model* m, size_t n, const uint32_t* numbers, const int32_t* dofs,
+                               const double* values, size_t* n_ok) {
+  return add_bc(m, 1, n, numbers, dofs, values, n_ok);
+}
+
+int fem_separate(fem_model* m, void* stream, int64_t* n_aa, int64_t* n_bb, int64_t nnz[4]) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->pattern_valid || m->dirty || !m->own_vals || !m->dev)
+    return fail(m, FEM_ERR_STATE, "the stiffness is not assembled: call fem_assemble");
+  ensure_bc(m);
+  if (m->bc_dirty) {
+    auto e = femdev::upload_bc(m->dev, m->bc_cons.data(), m->bc_disp.data(), m->bc_force.data(), m->bc_cons.size());
+    if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+    m->bc_dirty = false;
+  }
+  femdev::SepResult r;
+  auto e = femdev::separate_sparse(m->dev, r, stream, true, m->sep_lanes);
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  // methods_for_separate_stiffness_matrix.rs:232-259, 304-308
+  if (r.err_row >= 0) {
+    const size_t node = size_t(r.err_row) / 6;
+    const uint32_t number = node < m->node_number.size() ? m->node_number[node] : 0;
+    femdev::invalidate_separation(m->dev);
+    return fail(m, FEM_ERR_VALIDATION, std::string("There are no stiffness to withstand displacement ") +
+                                           DOF_NAMES[r.err_row % 6] + " applied to node " + u(number) + "!");
+  }
+  if (r.n_bb == 0) {
+    femdev::invalidate_separation(m->dev);
+    return fail(m, FEM_ERR_VALIDATION, "No restraints");
+  }
+  if (r.nnz[0] == 0) {
+    femdev::invalidate_separation(m->dev);
+    return fail(m, FEM_ERR_VALIDATION, "Sparse separation: K_aa is empty (structure has no free stiffness?)");
+  }
+  if (n_aa) *n_aa = r.n_aa;
+  if (n_bb) *n_bb = r.n_bb;
+  if (nnz)
+    for (int w = 0; w < 4; ++w) nnz[w] = r.nnz[w];
+  return FEM_OK;
+}
+
+int fem_separated_device(fem_model* m, int which, const int64_t** row_ptr, const int32_t** col_idx,
+                         const double** values) {
+  int rc = separation_ready(m);
+  if (rc) return rc;
+  if (which < 0 || which > 3) return fail(m, FEM_ERR_ARGUMENT, "which must be 0..3 (aa, ab, ba, bb)");
+  auto e = femdev::separated_csr(m->dev, which, row_ptr, col_idx, values);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_STATE, e);
+}
+
+int fem_separated_to_host(fem_model* m, int which, int64_t* row_ptr, int32_t* col_idx, double* values) {
+  int rc = separation_ready(m);
+  if (rc) return rc;
+  if (which < 0 || which > 3) return fail(m, FEM_ERR_ARGUMENT, "which must be 0..3 (aa, ab, ba, bb)");
+  int64_t rows[4], nnz[4];
+  femdev::separation_shape(m->dev, rows, nnz);
+  const int64_t* rp = nullptr;
+  const int32_t* ci = nullptr;
+  const double* v = nullptr;
+  auto e = femdev::separated_csr(m->dev, which, &rp, &ci, &v);
+  if (e.empty() && row_ptr) e = femdev::copy_to_host(row_ptr, rp, sizeof(int64_t) * size_t(rows[which] + 1));
+  if (e.empty() && col_idx && nnz[which]) e = femdev::copy_to_host(col_idx, ci, sizeof(int32_t) * size_t(nnz[which]));
+  if (e.empty() && values && nnz[which]) e = femdev::copy_to_host(values, v, sizeof(double) * size_t(nnz[which]));
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+int fem_separated_vectors_device(fem_model* m, const int64_t** k_aa_indexes, const int64_t** k_bb_indexes,
+                                 const double** r_a, const double** u_b) {
+  int rc = separation_ready(m);
+  if (rc) return rc;
+  auto e = femdev::separated_vectors(m->dev, k_aa_indexes, k_bb_indexes, r_a, u_b);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_STATE, e);
+}
+
+int fem_separated_vectors_to_host(fem_model* m, int64_t* k_aa_indexes, int64_t* k_bb_indexes, double* r_a,
+                                  double* u_b) {
+  int rc = separation_ready(m);
+  if (rc) return rc;
+  int64_t rows[4];
+  femdev::separation_shape(m->dev, rows, nullptr);
+  const int64_t *aa = nullptr, *bb = nullptr;
+  const double *ra = nullptr, *ub = nullptr;
+  auto e = femdev::separated_vectors(m->dev, &aa, &bb, &ra, &ub);
+  const size_t na = size_t(rows[0]), nb = size_t(rows[2]);
+  if (e.empty() && k_aa_indexes && na) e = femdev::copy_to_host(k_aa_indexes, aa, sizeof(int64_t) * na);
+  if (e.empty() && k_bb_indexes && nb) e = femdev::copy_to_host(k_bb_indexes, bb, sizeof(int64_t) * nb);
+  if (e.empty() && r_a && na) e = femdev::copy_to_host(r_a, ra, sizeof(double) * na);
+  if (e.empty() && u_b && nb) e = femdev::copy_to_host(u_b, ub, sizeof(double) * nb);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+double fem_last_separate_ms(fem_model* m) { return m && m->dev ? femdev::last_separate_ms(m->dev) : -1.0; }
 
 const char* fem_assembly_path(fem_model* m) { return m ? m->path.c_str() : ""; }
 
@@ -1325,6 +1504,11 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
     m->pf_wg_per_cu = int(value);
   } else if (k == "force_generic") {
     m->force_generic = value != 0;
+  } else if (k == "separate_lanes") {
+    if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
+      return fail(m, FEM_ERR_ARGUMENT, "separate_lanes must be 0 (automatic), 4, 8, 16 or 32");
+    m->sep_lanes = int(value);
+    return FEM_OK;  // speed only; the pattern stays
   } else {
     return fail(m, FEM_ERR_ARGUMENT, "unknown option " + k);
   }

@@ -146,6 +146,45 @@ double fem_last_backward_ms(fem_model* m);
 int fem_update_params(fem_model* m, int type, const double* params_device, void* stream);
 int fem_update_nodes(fem_model* m, const double* xyz_device, void* stream);
 
+/* ---- boundary conditions and sparse separation ------------------------
+ *
+ * dof 0..5 = X, Y, Z, ThX, ThY, ThZ (DOFParameter, methods_for_bc_data_handle.rs:6-14);
+ * global dof index = 6 * node_index + dof. */
+
+/* FEM::add_displacement(node, dof, value), n entries  methods_for_bc_data_handle.rs:176-203
+ * A second displacement on the same dof fails with the crate's message. */
+int fem_add_displacements(fem_model* m, size_t n, const uint32_t* numbers, const int32_t* dofs, const double* values,
+                          size_t* n_ok);
+/* FEM::add_concentrated_load(node, dof, value), n entries  methods_for_bc_data_handle.rs:34-56
+ * Loads on the same dof add up. */
+int fem_add_concentrated_loads(fem_model* m, size_t n, const uint32_t* numbers, const int32_t* dofs,
+                               const double* values, size_t* n_ok);
+
+/* FEM::separate_stiffness_matrix_sparse_iterative()
+ *                              methods_for_separate_stiffness_matrix.rs:217-320
+ * plus compose_r_a_vector / compose_u_b_vector (:323-345), on the device, from
+ * the values of the last fem_assemble.  Dofs with a zero (or absent) diagonal
+ * are excluded; constrained dofs form b, the others a; entries equal to zero
+ * are dropped.  Outputs: n_aa, n_bb and the non-zero counts of K_aa, K_ab,
+ * K_ba, K_bb (any pointer may be NULL).  Fails with the crate's messages
+ * (constrained dof without stiffness, "No restraints", empty K_aa).
+ * Synchronizes `stream`. */
+int fem_separate(fem_model* m, void* stream, int64_t* n_aa, int64_t* n_bb, int64_t nnz[4]);
+/* The four matrices as CSR with local indices, rows and columns ascending
+ * (which: 0 = K_aa, 1 = K_ab, 2 = K_ba, 3 = K_bb).  _device lends the
+ * library's buffers until the next fem_separate; _to_host copies. */
+int fem_separated_device(fem_model* m, int which, const int64_t** row_ptr, const int32_t** col_idx,
+                         const double** values);
+int fem_separated_to_host(fem_model* m, int which, int64_t* row_ptr, int32_t* col_idx, double* values);
+/* k_aa_indexes, k_bb_indexes (global dof indexes, ascending), r_a (forces at
+ * the a dofs) and u_b (imposed displacements at the b dofs). */
+int fem_separated_vectors_device(fem_model* m, const int64_t** k_aa_indexes, const int64_t** k_bb_indexes,
+                                 const double** r_a, const double** u_b);
+int fem_separated_vectors_to_host(fem_model* m, int64_t* k_aa_indexes, int64_t* k_bb_indexes, double* r_a,
+                                  double* u_b);
+/* HIP-event time of the last fem_separate's kernels (ms); -1 when unavailable. */
+double fem_last_separate_ms(fem_model* m);
+
 /* ---- batched assembly (no counterpart in the crate) ---------------------
  *
  * Many independent meshes assembled by one hipGraph launch.  The meshes'
@@ -187,7 +226,10 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
  *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernel,
  *                        looping over the patches: 0..64 (0 = one workgroup per patch)
  *   "force_generic"      1 = always use the generic value path
- * Setting an option invalidates the pattern (rebuilt by the next assembly). */
+ *   "separate_lanes"     lanes per CSR row in fem_separate: 4, 8, 16, 32
+ *                        (0 = automatic, from the mean row length)
+ * Setting a plate or path option invalidates the pattern (rebuilt by the next
+ * assembly). */
 int fem_set_option(fem_model* m, const char* key, int64_t value);
 
 /* Name of the value path used by the last assembly ("plate_fused" or "generic"). */

@@ -261,6 +261,14 @@ struct Model {
   std::vector<uint8_t> t_structural;  // 1 if the entry is in the element's structural pattern
   std::vector<uint8_t> t_nonzero;     // 1 if the crate would call add_value (value != 0)
   std::string last_error;
+  // boundary conditions (methods_for_bc_data_handle.rs): per global dof
+  std::vector<uint8_t> constrained;
+  std::vector<V> displacement, force;
+  // sparse separation (methods_for_separate_stiffness_matrix.rs:217-320)
+  std::vector<int64_t> aa_idx, bb_idx;
+  std::vector<int64_t> sep_rp[4];
+  std::vector<int32_t> sep_ci[4];
+  std::vector<double> sep_v[4];
   // exported CSR
   std::vector<int64_t> row_ptr;
   std::vector<int32_t> col_idx;
@@ -1010,6 +1018,113 @@ int64_t export_csr(Model<V>& m, int mode) {
   return int64_t(m.val.size());
 }
 
+// ---------------------------------------------------------------- boundary conditions
+const char* const DOF_NAMES[6] = {"X", "Y", "Z", "ThX", "ThY", "ThZ"};  // DOFParameter Debug
+
+template <class V>
+void ensure_bc(Model<V>& m) {
+  const size_t n = size_t(m.nodes_number) * NODE_DOF;
+  if (m.constrained.size() != n) {
+    m.constrained.assign(n, 0);
+    m.displacement.assign(n, V(0));
+    m.force.assign(n, V(0));
+  }
+}
+
+// methods_for_bc_data_handle.rs:176-203
+template <class V>
+bool add_displacement(Model<V>& m, uint32_t number, int dof, V value) {
+  if (!check_node_exist(m, number)) return false;
+  ensure_bc(m);
+  const size_t index = m.nodes.at(number).index * NODE_DOF + size_t(dof);
+  if (m.constrained[index]) {
+    m.last_error = std::string("Displacement ") + DOF_NAMES[dof] + " already applied to node " +
+                   std::to_string(number) + "!";
+    return false;
+  }
+  m.constrained[index] = 1;
+  m.displacement[index] = value;
+  return true;
+}
+
+// methods_for_bc_data_handle.rs:34-56
+template <class V>
+bool add_concentrated_load(Model<V>& m, uint32_t number, int dof, V value) {
+  if (!check_node_exist(m, number)) return false;
+  ensure_bc(m);
+  const size_t index = m.nodes.at(number).index * NODE_DOF + size_t(dof);
+  m.force[index] = m.force[index] + value;
+  return true;
+}
+
+// methods_for_separate_stiffness_matrix.rs:217-320 on the exported CSR (call
+// export_csr first).  Entries are classified exactly as the crate does; the
+// triplets come out sorted by (row, column) instead of in HashMap order.
+template <class V>
+bool separate_sparse(Model<V>& m) {
+  ensure_bc(m);
+  const int64_t n = int64_t(m.nodes_number) * NODE_DOF;
+  std::vector<double> diag(size_t(n), 0.0);
+  for (int64_t r = 0; r < n; ++r)
+    for (int64_t k = m.row_ptr[r]; k < m.row_ptr[r + 1]; ++k)
+      if (m.col_idx[k] == r) diag[size_t(r)] = m.val[k];
+  m.aa_idx.clear();
+  m.bb_idx.clear();
+  for (int64_t idx = 0; idx < n; ++idx) {
+    if (diag[size_t(idx)] == 0.0) {
+      if (m.constrained[size_t(idx)]) {
+        uint32_t number = 0;
+        for (auto& kv : m.nodes)
+          if (kv.second.index == size_t(idx / NODE_DOF)) number = kv.first;
+        m.last_error = std::string("There are no stiffness to withstand displacement ") + DOF_NAMES[idx % NODE_DOF] +
+                       " applied to node " + std::to_string(number) + "!";
+        return false;
+      }
+      continue;  // excluded / inactive dof
+    }
+    (m.constrained[size_t(idx)] ? m.bb_idx : m.aa_idx).push_back(idx);
+  }
+  if (m.bb_idx.empty()) {
+    m.last_error = "No restraints";
+    return false;
+  }
+  std::vector<int64_t> aa_pos(size_t(n), -1), bb_pos(size_t(n), -1);
+  for (size_t i = 0; i < m.aa_idx.size(); ++i) aa_pos[size_t(m.aa_idx[i])] = int64_t(i);
+  for (size_t i = 0; i < m.bb_idx.size(); ++i) bb_pos[size_t(m.bb_idx[i])] = int64_t(i);
+  const int64_t rows[4] = {int64_t(m.aa_idx.size()), int64_t(m.aa_idx.size()), int64_t(m.bb_idx.size()),
+                           int64_t(m.bb_idx.size())};
+  for (int w = 0; w < 4; ++w) {
+    m.sep_rp[w].assign(size_t(rows[w]) + 1, 0);
+    m.sep_ci[w].clear();
+    m.sep_v[w].clear();
+  }
+  for (int64_t r = 0; r < n; ++r) {
+    const int64_t ia = aa_pos[size_t(r)], ib = bb_pos[size_t(r)];
+    for (int64_t k = m.row_ptr[r]; k < m.row_ptr[r + 1]; ++k) {
+      const double v = m.val[k];
+      if (v == 0.0) continue;
+      const int64_t c = m.col_idx[k], ja = aa_pos[size_t(c)], jb = bb_pos[size_t(c)];
+      int w = -1;
+      int64_t row = 0, col = 0;
+      if (ia >= 0 && ja >= 0) { w = 0; row = ia; col = ja; }
+      else if (ia >= 0 && jb >= 0) { w = 1; row = ia; col = jb; }
+      else if (ib >= 0 && ja >= 0) { w = 2; row = ib; col = ja; }
+      else if (ib >= 0 && jb >= 0) { w = 3; row = ib; col = jb; }
+      if (w < 0) continue;  // entry touches an excluded dof
+      m.sep_rp[w][size_t(row) + 1] += 1;
+      m.sep_ci[w].push_back(int32_t(col));
+      m.sep_v[w].push_back(v);
+    }
+  }
+  for (int w = 0; w < 4; ++w)
+    for (int64_t r = 0; r < rows[w]; ++r) m.sep_rp[w][size_t(r) + 1] += m.sep_rp[w][size_t(r)];
+  if (m.sep_v[0].empty()) {
+    m.last_error = "Sparse separation: K_aa is empty (structure has no free stiffness?)";
+    return false;
+  }
+  return true;
+}
+
 struct AnyModel {
   bool f32;
   Model<double>* d;
@@ -1216,6 +1331,51 @@ int64_t om_structural_violations(AnyModel* am) {
     return c;
   };
   return am->f32 ? cnt(*am->f) : cnt(*am->d);
+}
+
+// boundary conditions; dof 0..5 = X, Y, Z, ThX, ThY, ThZ.  Return 0 or 1 (message in om_last_error).
+int om_add_displacement(AnyModel* am, uint32_t number, int dof, double value) {
+  bool ok = am->f32 ? add_displacement<float>(*am->f, number, dof, float(value))
+                    : add_displacement<double>(*am->d, number, dof, value);
+  return ok ? 0 : 1;
+}
+int om_add_concentrated_load(AnyModel* am, uint32_t number, int dof, double value) {
+  bool ok = am->f32 ? add_concentrated_load<float>(*am->f, number, dof, float(value))
+                    : add_concentrated_load<double>(*am->d, number, dof, value);
+  return ok ? 0 : 1;
+}
+// separation of the exported CSR (om_export first); out[6] = n_aa, n_bb, nnz of aa, ab, ba, bb
+int om_separate(AnyModel* am, int64_t out[6]) {
+  auto run = [&](auto& m) {
+    if (!separate_sparse(m)) return 1;
+    out[0] = int64_t(m.aa_idx.size());
+    out[1] = int64_t(m.bb_idx.size());
+    for (int w = 0; w < 4; ++w) out[2 + w] = int64_t(m.sep_v[w].size());
+    return 0;
+  };
+  return am->f32 ? run(*am->f) : run(*am->d);
+}
+void om_separated_copy(AnyModel* am, int which, int64_t* rp, int32_t* ci, double* v) {
+  auto cp = [&](auto& m) {
+    std::memcpy(rp, m.sep_rp[which].data(), m.sep_rp[which].size() * sizeof(int64_t));
+    std::memcpy(ci, m.sep_ci[which].data(), m.sep_ci[which].size() * sizeof(int32_t));
+    std::memcpy(v, m.sep_v[which].data(), m.sep_v[which].size() * sizeof(double));
+  };
+  if (am->f32) cp(*am->f); else cp(*am->d);
+}
+// k_aa_indexes, k_bb_indexes, r_a (compose_r_a_vector :323-333), u_b (compose_u_b_vector :335-345)
+void om_separated_vectors(AnyModel* am, int64_t* aa, int64_t* bb, double* r_a, double* u_b) {
+  auto cp = [&](auto& m) {
+    for (size_t i = 0; i < m.aa_idx.size(); ++i) {
+      aa[i] = m.aa_idx[i];
+      r_a[i] = double(m.force[size_t(m.aa_idx[i])]);
+    }
+    for (size_t i = 0; i < m.bb_idx.size(); ++i) {
+      bb[i] = m.bb_idx[i];
+      u_b[i] = double(m.displacement[size_t(m.bb_idx[i])]);
+    }
+  };
+  if (am->f32) cp(*am->f); else cp(*am->d);
 }
 
 }  // extern "C"

@@ -59,6 +59,14 @@ def lib():
         L.om_structural_violations.argtypes = [vp]
         L.om_nonstructural_max_rel.restype = ctypes.c_double
         L.om_nonstructural_max_rel.argtypes = [vp]
+        L.om_add_displacement.restype = ctypes.c_int
+        L.om_add_displacement.argtypes = [vp, u32, ctypes.c_int, dbl]
+        L.om_add_concentrated_load.restype = ctypes.c_int
+        L.om_add_concentrated_load.argtypes = [vp, u32, ctypes.c_int, dbl]
+        L.om_separate.restype = ctypes.c_int
+        L.om_separate.argtypes = [vp, vp]
+        L.om_separated_copy.argtypes = [vp, ctypes.c_int, vp, vp, vp]
+        L.om_separated_vectors.argtypes = [vp, vp, vp, vp, vp]
         L.om_eval_element.restype = ctypes.c_int
         L.om_eval_element.argtypes = [ctypes.c_int, vp, vp, dbl, dbl, ctypes.c_uint, vp]
         _lib = L
@@ -99,6 +107,55 @@ def principal_moments(i11, i22, i12, rel_tol):
     out = np.zeros(3)
     lib().om_principal_moments(i11, i22, i12, rel_tol, _ptr(out))
     return out
+
+
+DOF_NAMES = ("X", "Y", "Z", "ThX", "ThY", "ThZ")
+
+
+def separate_csr(row_ptr, col_idx, val, constrained, disp=None, force=None, node_numbers=None):
+    """separate_stiffness_matrix_sparse_iterative
+    (methods_for_separate_stiffness_matrix.rs:217-320) with compose_r_a_vector /
+    compose_u_b_vector (:323-345), restated in numpy over any CSR with sorted
+    columns.  Global dof = 6 * node_index + dof; node_numbers[node_index] names
+    the node in the error message.  Returns ({"aa","ab","ba","bb": (row_ptr,
+    col_idx, val)}, k_aa_indexes, k_bb_indexes, r_a, u_b)."""
+    rp = np.asarray(row_ptr, dtype=np.int64)
+    ci = np.asarray(col_idx, dtype=np.int64)
+    v = np.asarray(val, dtype=np.float64)
+    n = len(rp) - 1
+    cons = np.asarray(constrained).astype(bool)
+    disp = np.zeros(n) if disp is None else np.asarray(disp, dtype=np.float64)
+    force = np.zeros(n) if force is None else np.asarray(force, dtype=np.float64)
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    diag = np.zeros(n)
+    on = ci == rows
+    diag[rows[on]] = v[on]
+    bad = np.nonzero((diag == 0.0) & cons)[0]
+    if len(bad):  # the crate stops at the first such dof in index order
+        idx = int(bad[0])
+        number = int(node_numbers[idx // 6]) if node_numbers is not None else 0
+        raise OracleError("There are no stiffness to withstand displacement %s applied to node %d!"
+                          % (DOF_NAMES[idx % 6], number))
+    is_a = (diag != 0.0) & ~cons
+    is_b = (diag != 0.0) & cons
+    aa, bb = np.nonzero(is_a)[0], np.nonzero(is_b)[0]
+    if len(bb) == 0:
+        raise OracleError("No restraints")
+    pos = np.full(n, -1, dtype=np.int64)
+    pos[aa] = np.arange(len(aa))
+    pos[bb] = np.arange(len(bb))
+    keep = v != 0.0
+    mats = {}
+    for name, rmask, cmask, nrows in (("aa", is_a, is_a, len(aa)), ("ab", is_a, is_b, len(aa)),
+                                      ("ba", is_b, is_a, len(bb)), ("bb", is_b, is_b, len(bb))):
+        sel = keep & rmask[rows] & cmask[ci]
+        r_loc = pos[rows[sel]]
+        out_rp = np.zeros(nrows + 1, dtype=np.int64)
+        out_rp[1:] = np.cumsum(np.bincount(r_loc, minlength=nrows))
+        mats[name] = (out_rp, pos[ci[sel]].astype(np.int32), v[sel].copy())
+    if len(mats["aa"][2]) == 0:
+        raise OracleError("Sparse separation: K_aa is empty (structure has no free stiffness?)")
+    return mats, aa.astype(np.int64), bb.astype(np.int64), force[aa].copy(), disp[bb].copy()
 
 
 class OracleModel:
@@ -191,6 +248,38 @@ class OracleModel:
         val = np.zeros(nnz, dtype=np.float64)
         lib().om_export_copy(self._h, _ptr(row_ptr), _ptr(col), _ptr(val))
         return row_ptr, col, val
+
+    def add_displacement(self, number, dof, value):
+        """FEM::add_displacement (methods_for_bc_data_handle.rs:176-203); dof 0..5 = X..ThZ."""
+        if lib().om_add_displacement(self._h, number, dof, float(value)):
+            raise OracleError(self._err())
+
+    def add_concentrated_load(self, number, dof, value):
+        """FEM::add_concentrated_load (methods_for_bc_data_handle.rs:34-56)."""
+        if lib().om_add_concentrated_load(self._h, number, dof, float(value)):
+            raise OracleError(self._err())
+
+    def separate_sparse(self, structural=True):
+        """separate_stiffness_matrix_sparse_iterative on the exported CSR:
+        {"aa","ab","ba","bb": (row_ptr, col_idx, val)}, k_aa_indexes, k_bb_indexes,
+        r_a, u_b."""
+        self.csr(structural=structural)
+        out = np.zeros(6, dtype=np.int64)
+        if lib().om_separate(self._h, _ptr(out)):
+            raise OracleError(self._err())
+        n_aa, n_bb = int(out[0]), int(out[1])
+        rows = [n_aa, n_aa, n_bb, n_bb]
+        mats = {}
+        for w, name in enumerate(("aa", "ab", "ba", "bb")):
+            rp = np.zeros(rows[w] + 1, dtype=np.int64)
+            ci = np.zeros(int(out[2 + w]), dtype=np.int32)
+            v = np.zeros(int(out[2 + w]), dtype=np.float64)
+            lib().om_separated_copy(self._h, w, _ptr(rp), _ptr(ci), _ptr(v))
+            mats[name] = (rp, ci, v)
+        aa, bb = np.zeros(n_aa, dtype=np.int64), np.zeros(n_bb, dtype=np.int64)
+        ra, ub = np.zeros(n_aa), np.zeros(n_bb)
+        lib().om_separated_vectors(self._h, _ptr(aa), _ptr(bb), _ptr(ra), _ptr(ub))
+        return mats, aa, bb, ra, ub
 
     def structural_violations(self):
         return lib().om_structural_violations(self._h)

@@ -1,17 +1,21 @@
 """Parity metrics between the GPU CSR and the oracle's CSR.
 
 Tolerance (values are FP64): for every stored entry
-    |K_gpu - K_ref| <= 1e-12 * |K_ref| + 1e-15 * max_j |K_ref[i, j]|
-i.e. 1e-12 relative per entry, with a floor of ~5 ulp of the row's largest
-entry for entries that are rounding noise of mathematically zero sums.  The
-normwise max |dK| / max |K| is reported beside it.  The sparsity pattern
+    |K_gpu - K_ref| <= 1e-12 * |K_ref| + 1e-14 * max_j |K_ref[i, j]|
+i.e. 1e-12 relative for every entry above 1% of its row's largest entry, and
+below that an absolute floor of ~45 ulp of the row's largest entry.  Small
+entries are differences of large terms (a beam's rotation-rotation entry is
+EI/L - kGA L/4 before the frame transform; exactly-zero sums of plate
+blocks), so their rounding error scales with the row, not with the entry:
+the worst seen is 18 ulp of the row max on a 2.5e-4 * row-max beam entry.
+The normwise max |dK| / max |K| is reported beside it.  The sparsity pattern
 (row_ptr, col_idx) must be bit-identical.
 """
 
 import numpy as np
 
 REL = 1e-12
-ROW_FLOOR = 1e-15
+ROW_FLOOR = 1e-14
 
 
 def row_max(row_ptr, val):

@@ -14,6 +14,7 @@ CASES = {
     "truss_lattice_11": lambda: meshgen.truss_lattice(11, seed=0),
     "beam_frame_6": lambda: meshgen.beam_frame(6, seed=4),
     "beam_frame_4x5x3": lambda: meshgen.beam_frame(4, 5, 3, seed=5),
+    "beam_frame_5": lambda: meshgen.beam_frame(5, seed=4),
     "plate_8": lambda: meshgen.plate_grid(8, seed=1),
     "plate_17x9": lambda: meshgen.plate_grid(17, 9, seed=2),
     "plate_64_shuffled": lambda: meshgen.plate_grid(64, seed=7, shuffle=True),
@@ -22,7 +23,7 @@ CASES = {
 
 EXPECTED_PATH = {"plate_8": "plate_fused", "plate_17x9": "plate_fused", "plate_64_shuffled": "plate_fused",
                  "plate_12_tilted": "generic", "beam_frame_6": "bar_owner", "truss_lattice_5": "bar_owner",
-                 "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner"}
+                 "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner", "beam_frame_5": "bar_owner"}
 
 
 def _tilt():

@@ -1,0 +1,196 @@
+"""Sparse separation on the GPU (fem_separate) against the oracle.
+
+Two checks per case: the device partition of the GPU's own CSR equals the
+numpy restatement of the crate's algorithm on that same CSR bit for bit
+(classification, local indices, order, values, k_aa/k_bb indexes, r_a, u_b),
+and the four blocks match the oracle's separation of the oracle's K to the
+assembly tolerance (|dK| <= 1e-12 |K| + 1e-14 row max, see parity.py)."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from finite_element_method_amd import FEM, DeviceError, FemError, meshgen
+from oracle.oracle import OracleModel, separate_csr
+from tests import bc_cases
+from tests.parity import REL, ROW_FLOOR
+
+pytestmark = pytest.mark.gpu
+
+NAMES = ("aa", "ab", "ba", "bb")
+
+
+def _tilt():
+    a, b = 0.3, -0.7
+    rx = np.array([[1, 0, 0], [0, np.cos(a), -np.sin(a)], [0, np.sin(a), np.cos(a)]])
+    rz = np.array([[np.cos(b), -np.sin(b), 0], [np.sin(b), np.cos(b), 0], [0, 0, 1]])
+    return rz @ rx
+
+
+CASES = {
+    "plate_16": lambda: meshgen.plate_grid(16, seed=1),
+    "plate_9x13_shuffled": lambda: meshgen.plate_grid(9, 13, seed=2, shuffle=True),
+    "plate_8_tilted": lambda: meshgen.plate_grid(8, seed=3, rotation=_tilt()),
+    "beam_frame_5": lambda: meshgen.beam_frame(5, seed=4),
+    "truss_lattice_5": lambda: meshgen.truss_lattice(5, seed=5),
+}
+
+
+def _bc_vectors(mesh, disp, loads):
+    n = 6 * mesh.n_nodes
+    cons, dvec, fvec = np.zeros(n, bool), np.zeros(n), np.zeros(n)
+    idx = {int(x): i for i, x in enumerate(mesh.node_numbers)}
+    for nn, d, val in zip(*disp):
+        cons[6 * idx[nn] + d] = True
+        dvec[6 * idx[nn] + d] = val
+    for nn, d, val in zip(*loads):
+        fvec[6 * idx[nn] + d] += val
+    return cons, dvec, fvec
+
+
+def _close(got, want, ncols, row_max):
+    """Blocks equal to the assembly tolerance, compared densely (an entry that
+    is exactly zero on one side may be rounding noise on the other); row_max is
+    the largest |K| of each block row's full global row."""
+    a = sp.csr_matrix((got[2], got[1], got[0]), shape=(len(got[0]) - 1, ncols)).toarray()
+    b = sp.csr_matrix((want[2], want[1], want[0]), shape=(len(want[0]) - 1, ncols)).toarray()
+    ratio = np.abs(a - b) / np.maximum(REL * np.abs(b) + ROW_FLOOR * row_max[:, None], 1e-300)
+    if ratio.size and ratio.max() > 1.0:
+        i, j = np.unravel_index(np.argmax(ratio), ratio.shape)
+        print("worst", i, j, a[i, j], b[i, j], row_max[i], ratio[i, j], "count", int((ratio > 1).sum()), flush=True)
+    return bool(ratio.size == 0 or ratio.max() <= 1.0)
+
+
+def _global_row_max(rp, v):
+    n = len(rp) - 1
+    rm = np.zeros(n)
+    np.maximum.at(rm, np.repeat(np.arange(n), np.diff(rp)), np.abs(v))
+    return rm
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+def test_separation_matches_oracle(gpu, name):
+    mesh = CASES[name]()
+    print("case", name, mesh.n_nodes, mesh.n_elements, flush=True)
+    om = meshgen.load_oracle(mesh, OracleModel)
+    k_ref = om.csr()
+    diag = bc_cases.diagonal(*k_ref)
+    rm = _global_row_max(k_ref[0], k_ref[2])
+    disp, loads = bc_cases.random_bc(mesh, diag, seed=len(name))
+    bc_cases.apply(om, disp, loads)
+    m = FEM.from_mesh(mesh)
+    bc_cases.apply(m, disp, loads)
+    got = m.separate_sparse()
+    print(name, m.assembly_path(), "n_aa", len(got[1]), "n_bb", len(got[2]), "ms", m.last_separate_ms(), flush=True)
+    # 1. the device partition of the GPU's own K == the restatement on that K
+    cons, dvec, fvec = _bc_vectors(mesh, disp, loads)
+    want = separate_csr(*m.stiffness_csr(assemble=False), cons, dvec, fvec, mesh.node_numbers)
+    for w in NAMES:
+        for a, b in zip(got[0][w], want[0][w]):
+            assert np.array_equal(a, b), w
+    for a, b in zip(got[1:], want[1:]):
+        assert np.array_equal(a, b)
+    # 2. against the oracle's separation of its own K
+    ref = om.separate_sparse()
+    assert np.array_equal(got[1], ref[1]) and np.array_equal(got[2], ref[2])
+    assert np.array_equal(got[3], ref[3]) and np.array_equal(got[4], ref[4])
+    n_aa, n_bb = len(ref[1]), len(ref[2])
+    for w, ncols, rows in zip(NAMES, (n_aa, n_bb, n_aa, n_bb), (ref[1], ref[1], ref[2], ref[2])):
+        assert _close(got[0][w], ref[0][w], ncols, rm[rows]), w
+
+
+def test_golden_truss(gpu):
+    """src/tests/fem/test_fem.rs:66-81: n_aa > 0 and K_aa non-empty; the 1x1
+    system gives u2x = 0.0015."""
+    m = FEM.create(1e-4, 1e-12, 2)
+    m.add_node(1, 0.0, 0.0, 0.0)
+    m.add_node(2, 30.0, 0.0, 0.0)
+    m.add_truss(1, 1, 2, 1e6, 2.0, None)
+    m.add_displacement(1, "X", 0.0)
+    m.add_concentrated_load(2, "X", 100.0)
+    mats, aa, bb, ra, ub = m.separate_sparse()
+    assert list(aa) == [6] and list(bb) == [0] and list(ra) == [100.0] and list(ub) == [0.0]
+    k = 1e6 * 2.0 / 30.0
+    assert mats["aa"][2][0] == pytest.approx(k, rel=1e-15)
+    assert mats["ab"][2][0] == pytest.approx(-k, rel=1e-15)
+    assert ra[0] / mats["aa"][2][0] == pytest.approx(0.0015, rel=1e-15)
+
+
+def test_crate_messages(gpu):
+    m = FEM.create(1e-4, 1e-12, 2)
+    m.add_node(1, 0.0, 0.0, 0.0)
+    m.add_node(2, 30.0, 0.0, 0.0)
+    m.add_truss(1, 1, 2, 1e6, 2.0, None)
+    with pytest.raises(FemError, match=r"^No restraints$"):
+        m.separate()
+    m.add_displacement(1, "X", 0.0)
+    with pytest.raises(FemError, match=r"^Displacement X already applied to node 1!$"):
+        m.add_displacement(1, "X", 1.0)
+    m.add_displacement(2, "X", 0.0)
+    with pytest.raises(FemError, match=r"^Sparse separation: K_aa is empty \(structure has no free stiffness\?\)$"):
+        m.separate()
+    m.add_displacement(2, "ThZ", 0.0)
+    with pytest.raises(FemError, match=r"^There are no stiffness to withstand displacement ThZ applied to node 2!$"):
+        m.separate()
+    with pytest.raises(DeviceError, match="no separation"):  # nothing to lend after a failed separation
+        m.separated_device("aa")
+
+
+def test_second_separation_smaller(gpu):
+    """A second separation with fewer free dofs reuses the buffers; parameter
+    updates without a new assembly are refused."""
+    import torch
+
+    mesh = meshgen.plate_grid(12, seed=9)
+    om = meshgen.load_oracle(mesh, OracleModel)
+    diag = bc_cases.diagonal(*om.csr())
+    m = FEM.from_mesh(mesh)
+    bc_cases.apply(m, bc_cases.clamp_face(mesh, diag, axis=0), ([], [], []))
+    first = m.separate_sparse()
+    more = bc_cases.clamp_face(mesh, diag, axis=1)
+    fresh = [(n, d, v) for n, d, v in zip(*more) if n not in set(bc_cases.clamp_face(mesh, diag, axis=0)[0])]
+    bc_cases.apply(m, tuple(map(list, zip(*fresh))), ([], [], []))
+    second = m.separate_sparse(assemble=False)
+    assert len(second[1]) < len(first[1]) and len(second[2]) > len(first[2])
+    cons = np.zeros(6 * mesh.n_nodes, bool)
+    cons[second[2]] = True
+    want = separate_csr(*m.stiffness_csr(assemble=False), cons, None, None, mesh.node_numbers)
+    for w in NAMES:
+        for a, b in zip(second[0][w], want[0][w]):
+            assert np.array_equal(a, b), w
+    # new parameters on the device: the old values are stale until assembled again
+    p = torch.tensor(mesh.props, dtype=torch.float64, device="cuda")
+    p[:, 0] *= 1.5
+    m.update_params(2, p)
+    with pytest.raises(DeviceError, match="not assembled"):
+        m.separate(assemble=False)
+    third = m.separate_sparse()
+    want = separate_csr(*m.stiffness_csr(assemble=False), cons, None, None, mesh.node_numbers)
+    for w in NAMES:
+        for a, b in zip(third[0][w], want[0][w]):
+            assert np.array_equal(a, b), w
+    assert not np.array_equal(third[0]["aa"][2], second[0]["aa"][2])
+
+
+def test_large_plate_properties(gpu):
+    """At 256 x 256 quads (395k dofs): the device partition of the GPU's K
+    equals the restatement, and the blocks hold every non-zero kept entry."""
+    mesh = meshgen.plate_grid(256, seed=11)
+    m = FEM.from_mesh(mesh)
+    rp, ci, v = m.stiffness_csr()
+    diag = bc_cases.diagonal(rp, ci, v)
+    disp, loads = bc_cases.random_bc(mesh, diag, seed=12, frac_fixed=0.01, frac_loaded=0.01)
+    m.add_displacements(*disp)
+    m.add_concentrated_loads(*loads)
+    got = m.separate_sparse(assemble=False)
+    print("large plate ms", m.last_separate_ms(), flush=True)
+    cons, dvec, fvec = _bc_vectors(mesh, disp, loads)
+    want = separate_csr(rp, ci, v, cons, dvec, fvec, mesh.node_numbers)
+    for w in NAMES:
+        for a, b in zip(got[0][w], want[0][w]):
+            assert np.array_equal(a, b), w
+    for a, b in zip(got[1:], want[1:]):
+        assert np.array_equal(a, b)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    kept = (diag != 0.0)
+    assert sum(len(got[0][w][2]) for w in NAMES) == int(np.count_nonzero((v != 0) & kept[rows] & kept[ci]))
