@@ -187,9 +187,11 @@ def run_batched(args):
 
 
 def cpu_baseline(kind):
-    """The oracle (single-threaded restatement of the crate's algorithm: dense
-    element k, (T^T k) T, sequential accumulation, CSR export) on a bounded
-    sample of the same workload family."""
+    """The oracle (the crate's algorithm restated: dense element k, (T^T k) T,
+    insertion-order sums into the CSR) on a bounded sample of the same workload
+    family: the steady-state value recompute timed single-threaded and on all
+    OpenMP threads of the host share (results checked bitwise against the
+    sequential export)."""
     from finite_element_method_amd import meshgen
     from oracle.oracle import OracleModel
 
@@ -202,12 +204,18 @@ def cpu_baseline(kind):
     else:
         sample = meshgen.truss_lattice(11, seed=0)
         desc = "truss_lattice(11): full workload"
-    t0 = time.perf_counter()
     om = meshgen.load_oracle(sample, OracleModel)
     om.csr(structural=True)
-    dt = time.perf_counter() - t0
-    return {"value": sample.n_elements / dt, "unit": "elements/s", "cores": 1, "kind": "port",
-            "sample": desc + ", oracle add_* (validation, element k, T^T k T, scatter) + CSR export, %.1f s" % dt}
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+    one, bad1 = om.time_values(1, 1)
+    par, badp = om.time_values(threads, 3)
+    if bad1 or badp:
+        raise RuntimeError("CPU baseline values differ from the sequential export")
+    return {"value": sample.n_elements / par, "unit": "elements/s", "cores": threads, "kind": "port",
+            "single_thread_value": sample.n_elements / one,
+            "sample": desc + ", oracle value recompute (element k, T^T k T, insertion-order CSR sums), "
+                      "median of 3 on %d OpenMP threads (%.3f s), one run on one thread (%.3f s)"
+                      % (threads, par, one)}
 
 
 def main():
@@ -220,6 +228,7 @@ def main():
     ap.add_argument("--no-grad", action="store_true", help="skip the backward / sensitivity timings")
     ap.add_argument("--no-separate", action="store_true", help="skip the sparse separation timing")
     ap.add_argument("--separate-lanes", type=int, default=0, help="separation: lanes per CSR row (0 = automatic)")
+    ap.add_argument("--plate-kernel", type=int, default=-1, help="fused plate kernel: 0 = row staging, 1 = block lanes")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (a known-traffic kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
@@ -254,6 +263,8 @@ def main():
         model.set_option("plate_wg_per_cu", args.wg_per_cu)
     if args.separate_lanes:
         model.set_option("separate_lanes", args.separate_lanes)
+    if args.plate_kernel >= 0:
+        model.set_option("plate_kernel", args.plate_kernel)
     t0 = time.perf_counter()
     model.build_pattern()
     setup_s = time.perf_counter() - t0

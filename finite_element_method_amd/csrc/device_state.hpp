@@ -37,6 +37,9 @@ struct Device {
   uint32_t *pf_hdr = nullptr, *pf_conn = nullptr, *pf_eid = nullptr, *pf_rows = nullptr, *pf_oinfo = nullptr,
            *pf_oboff = nullptr, *pf_oooff = nullptr, *pf_blk = nullptr;
   int64_t* pf_obase = nullptr;
+  // plate block-lane maps
+  int pb_patches = 0, pb_grid = 0;
+  uint32_t *pb_hdr = nullptr, *pb_conn = nullptr, *pb_eid = nullptr, *pb_blk = nullptr;
   // bar owner maps
   uint32_t* bo_bid[2] = {nullptr, nullptr};
   uint64_t* bo_bmask = nullptr;

@@ -62,6 +62,7 @@ void destroy_device(Device* d) {
   f(d->sep_cons); f(d->sep_cls); f(d->sep_tmp); f(d->sep_disp); f(d->sep_force); f(d->sep_ra); f(d->sep_ub);
   f(d->sep_v); f(d->sep_flag); f(d->sep_pos); f(d->sep_key); f(d->sep_cnt); f(d->sep_off); f(d->sep_aa); f(d->sep_bb);
   f(d->sep_rp); f(d->sep_sum); f(d->sep_ci); f(d->sep_dpos);
+  f(d->pb_hdr); f(d->pb_conn); f(d->pb_eid); f(d->pb_blk);
   for (hipEvent_t ev : {d->ev0, d->ev1, d->ev2, d->ev3, d->ev4, d->ev5})
     if (ev) hipEventDestroy(ev);
   delete d;

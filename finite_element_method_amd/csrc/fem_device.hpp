@@ -79,6 +79,25 @@ struct PlateMaps {
 // workgroup per patch)
 std::string upload_plate_maps(Device* d, const PlateMaps& pm, int wg_per_cu);
 
+// Block-lane fused plate path (plate_blocks.hip): same preconditions as the
+// maps above; one lane per owned node-pair block.
+constexpr int PB_MAX_ELEMS = 46;    // elements touching a patch
+constexpr int PB_MAX_BLOCKS = 256;  // owned node-pair blocks of a patch
+struct PlateBlockMaps {
+  int n_patches = 0;
+  std::vector<uint32_t> hdr;    // per patch: elem_off, blk_off, n_elems, n_blocks
+  std::vector<uint32_t> pconn;  // 4 per patch element: node indices
+  std::vector<uint32_t> peid;   // per patch element: element id (insertion order within the patch)
+  // per block, 4 words: CSR position of row 6a of the owned node a;
+  // k | nb << 5 | self << 10 | contributors << 11 (k: block within a's row, nb: blocks of a);
+  // contributors in element-insertion order: self blocks one byte each
+  // (patch element | p << 6, words 3 and 4), others 10 bits each
+  // (patch element | p << 6 | q << 8, three in word 3, the fourth in word 4)
+  std::vector<uint32_t> blk;
+};
+std::string upload_plate_block_maps(Device* d, const PlateBlockMaps& pm, int wg_per_cu);
+std::string assemble_plate_blocks(Device* d, double* val, void* stream, bool timed = true);
+
 // Bar owner path (bar_owner.hip): meshes of trusses and beams in which every
 // off-diagonal node pair belongs to one element.  One thread per (node, row).
 struct BarMaps {

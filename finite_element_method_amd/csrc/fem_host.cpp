@@ -97,6 +97,7 @@ struct fem_model {
   std::string path = "none";
   int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
   int pf_wg_per_cu = 2;  // fused plate: resident workgroups per CU looping over patches (0 = one per patch)
+  int pf_kernel = 0;     // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block (plate_blocks)
   bool force_generic = false;
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
@@ -506,6 +507,127 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
   return flush();
 }
 
+// Maps of the block-lane plate kernel (one lane per owned node-pair block).
+// Called only after build_plate_maps accepted the mesh, which checked the
+// preconditions (plates only, identity frames, standard block masks, valence
+// and row lengths).
+bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
+                            const std::vector<int64_t>& row_ptr, femdev::PlateBlockMaps& pm) {
+  const auto& S = m->set[PLATE];
+  const size_t N = m->nodes_number, E = S.size();
+  std::vector<uint32_t> deg(N + 1, 0);
+  for (size_t e = 0; e < E; ++e)
+    for (int p = 0; p < 4; ++p) deg[S.conn[4 * e + p]]++;
+  std::vector<uint32_t> nptr(N + 1, 0);
+  for (size_t a = 0; a < N; ++a) nptr[a + 1] = nptr[a] + deg[a];
+  std::vector<uint32_t> nel(nptr[N]);
+  {
+    std::vector<uint32_t> f(nptr.begin(), nptr.end() - 1);
+    for (size_t e = 0; e < E; ++e)
+      for (int p = 0; p < 4; ++p) nel[f[S.conn[4 * e + p]]++] = uint32_t(e) | uint32_t(p) << 30;
+  }
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (size_t a = 0; a < N; ++a)
+    for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], m->xyz[3 * a + k]); hi[k] = std::max(hi[k], m->xyz[3 * a + k]); }
+  std::vector<std::pair<uint64_t, uint32_t>> order;
+  order.reserve(N);
+  for (size_t a = 0; a < N; ++a) {
+    if (!deg[a]) continue;
+    uint64_t code = 0;
+    for (int k = 0; k < 3; ++k) {
+      double span = hi[k] - lo[k];
+      uint64_t qv = span > 0 ? uint64_t((m->xyz[3 * a + k] - lo[k]) / span * 2097151.0) : 0;
+      code |= spread3(qv) << k;
+    }
+    order.push_back({code, uint32_t(a)});
+  }
+  if (m->node_part.size() == N) {
+    std::stable_sort(order.begin(), order.end(), [&](const std::pair<uint64_t, uint32_t>& x,
+                                                     const std::pair<uint64_t, uint32_t>& y) {
+      const uint32_t px = m->node_part[x.second], py = m->node_part[y.second];
+      return px != py ? px < py : x < y;
+    });
+  } else {
+    std::sort(order.begin(), order.end());
+  }
+  std::vector<uint32_t> cur, cur_elems;
+  uint32_t cur_blocks = 0;
+  std::vector<int32_t> elem_local(E, -1);
+  std::vector<uint8_t> in_patch(E, 0);
+  auto flush = [&]() -> bool {
+    if (cur.empty()) return true;
+    std::sort(cur.begin(), cur.end());
+    std::sort(cur_elems.begin(), cur_elems.end());  // insertion order
+    for (size_t i = 0; i < cur_elems.size(); ++i) elem_local[cur_elems[i]] = int32_t(i);
+    const uint32_t elem_off = uint32_t(pm.peid.size()), blk_off = uint32_t(pm.blk.size() / 4);
+    for (uint32_t a : cur) {
+      const uint32_t* el = &nel[nptr[a]];
+      const uint32_t d = deg[a];
+      const uint64_t k0 = bptr[a], k1 = bptr[a + 1];
+      const uint32_t nb = uint32_t(k1 - k0);
+      for (uint64_t k = k0; k < k1; ++k) {
+        const uint32_t b = blocks[k].b;
+        uint32_t w1 = uint32_t(k - k0) | nb << 5, w2 = 0, w3 = 0;
+        if (b == a) {
+          w1 |= 1u << 10 | d << 11;
+          for (uint32_t r = 0; r < d; ++r) {
+            const uint32_t e = el[r] & 0x3FFFFFFFu, p = el[r] >> 30;
+            const uint32_t cb = uint32_t(elem_local[e]) | p << 6;
+            if (r < 4) w2 |= cb << (8 * r);
+            else w3 |= cb << (8 * (r - 4));
+          }
+        } else {
+          uint32_t cnt = 0;
+          for (uint32_t r = 0; r < d; ++r) {
+            const uint32_t e = el[r] & 0x3FFFFFFFu, p = el[r] >> 30;
+            for (uint32_t q = 0; q < 4; ++q)
+              if (S.conn[4 * size_t(e) + q] == b) {
+                if (cnt == uint32_t(femdev::PF_MAX_CONTRIB)) return false;
+                const uint32_t cb = uint32_t(elem_local[e]) | p << 6 | q << 8;
+                if (cnt < 3) w2 |= cb << (10 * cnt);
+                else w3 = cb;
+                cnt++;
+              }
+          }
+          if (!cnt) return false;
+          w1 |= cnt << 11;
+        }
+        pm.blk.insert(pm.blk.end(), {uint32_t(row_ptr[6 * size_t(a)]), w1, w2, w3});
+      }
+    }
+    for (uint32_t e : cur_elems) {
+      pm.peid.push_back(e);
+      for (int q = 0; q < 4; ++q) pm.pconn.push_back(S.conn[4 * size_t(e) + q]);
+      elem_local[e] = -1;
+    }
+    pm.hdr.insert(pm.hdr.end(), {elem_off, blk_off, uint32_t(cur_elems.size()), uint32_t(pm.blk.size() / 4) - blk_off});
+    pm.n_patches++;
+    cur.clear();
+    cur_elems.clear();
+    cur_blocks = 0;
+    return true;
+  };
+  for (auto& o : order) {
+    const uint32_t a = o.second;
+    const uint32_t nb = uint32_t(bptr[a + 1] - bptr[a]);
+    uint32_t new_elems = 0;
+    for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_patch[nel[nptr[a] + r] & 0x3FFFFFFFu];
+    if (cur_blocks + nb > uint32_t(femdev::PB_MAX_BLOCKS) ||
+        cur_elems.size() + new_elems > size_t(femdev::PB_MAX_ELEMS)) {
+      for (uint32_t e : cur_elems) in_patch[e] = 0;
+      if (!flush()) return false;
+    }
+    cur.push_back(a);
+    cur_blocks += nb;
+    for (uint32_t r = 0; r < deg[a]; ++r) {
+      const uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
+      if (!in_patch[e]) { in_patch[e] = 1; cur_elems.push_back(e); }
+    }
+  }
+  for (uint32_t e : cur_elems) in_patch[e] = 0;
+  return flush();
+}
+
 std::string build_pattern(fem_model* m) {
   std::string e;
   const size_t N = m->nodes_number;
@@ -634,9 +756,16 @@ std::string build_pattern(fem_model* m) {
   {
     femdev::PlateMaps pm;
     if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
-      e = femdev::upload_plate_maps(m->dev, pm, m->pf_wg_per_cu);
-      if (!e.empty()) return e;
-      m->path = "plate_fused";
+      femdev::PlateBlockMaps pbm;
+      if (m->pf_kernel == 1 && build_plate_block_maps(m, bptr, blocks, row_ptr, pbm)) {
+        e = femdev::upload_plate_block_maps(m->dev, pbm, m->pf_wg_per_cu);
+        if (!e.empty()) return e;
+        m->path = "plate_blocks";
+      } else {
+        e = femdev::upload_plate_maps(m->dev, pm, m->pf_wg_per_cu);
+        if (!e.empty()) return e;
+        m->path = "plate_fused";
+      }
       m->pattern_valid = true;
       return "";
     }
@@ -942,6 +1071,7 @@ int batch_ready(fem_batch* b) {
 
 std::string launch_values(fem_model* g, double* val, void* stream, bool timed) {
   return g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, val, stream, timed)
+         : g->path == "plate_blocks" ? femdev::assemble_plate_blocks(g->dev, val, stream, timed)
          : g->path == "bar_owner" ? femdev::assemble_bar_owner(g->dev, val, stream, timed)
                                   : femdev::assemble_generic(g->dev, val, stream, timed);
 }
@@ -1308,6 +1438,7 @@ int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
   int rc = ensure_ready(m);
   if (rc) return rc;
   std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
+                  : m->path == "plate_blocks" ? femdev::assemble_plate_blocks(m->dev, val_device, stream)
                   : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream)
                                            : femdev::assemble_generic(m->dev, val_device, stream);
   if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
@@ -1504,6 +1635,9 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
     m->pf_wg_per_cu = int(value);
   } else if (k == "force_generic") {
     m->force_generic = value != 0;
+  } else if (k == "plate_kernel") {
+    if (value != 0 && value != 1) return fail(m, FEM_ERR_ARGUMENT, "plate_kernel must be 0 or 1");
+    m->pf_kernel = int(value);
   } else if (k == "separate_lanes") {
     if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
       return fail(m, FEM_ERR_ARGUMENT, "separate_lanes must be 0 (automatic), 4, 8, 16 or 32");
@@ -1639,8 +1773,8 @@ int fem_batch_autotune(fem_batch* b) {
             if (rc) return rc;
             if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
           }
-        } else if (g->path == "bar_owner") {
-          cfg = {pn, g->pf_wg_per_cu, 256};  // fixed workgroup shape (node ranges cut on the host)
+        } else if (g->path == "bar_owner" || g->path == "plate_blocks") {
+          cfg = {pn, g->pf_wg_per_cu, 256};  // fixed workgroup shape
         } else {
           for (int bs : {128, 256, 512}) {
             femdev::set_generic_block(g->dev, bs);

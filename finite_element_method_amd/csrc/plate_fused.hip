@@ -120,13 +120,17 @@ __device__ __forceinline__ void pf_load_element(PatchRegs& P, const PatchArgs& A
   P.prm = *reinterpret_cast<const double4*>(A.params + 4 * size_t(P.eid));
 }
 
-// One patch: compute `cur`, prefetch `nxt` (the patch gridDim.x further on).
+// One patch: compute `cur`, prefetch `nxt` (the patch gridDim.x further on),
+// whose header was loaded by the previous call.  Vector-memory counters
+// retire in issue order, so a wait on a load issued after this patch's
+// stores would wait for the stores too: every load whose value is needed in
+// the next call is issued before the stores (the header of the patch after
+// `nxt` just before them, into `cur`, which is then free).
 __device__ __forceinline__ bool pf_process(PatchRegs& cur, PatchRegs& nxt, int patch, const PatchArgs& A,
                                            PatchLds& L) {
   const int t = threadIdx.x;
   const int next = patch + int(gridDim.x);
   const bool has_next = next < A.n_patches;
-  pf_load_header(nxt, A, has_next ? next : patch);
   double* lds = reinterpret_cast<double*>(L.region);
   double2* geo2 = L.region;  // [elem][point][PF_GEO / 2]
   uint32_t* pos_off = reinterpret_cast<uint32_t*>(lds + PF_MAX_OUT);  // byte offset of each staged value
@@ -299,17 +303,29 @@ __device__ __forceinline__ bool pf_process(PatchRegs& cur, PatchRegs& nxt, int p
   __syncthreads();
 
   // ---- step 5: the patch's CSR values, 16 buffer stores per lane (lanes past
-  //      the end store to an out-of-range offset, which the hardware drops)
+  //      the end store to an out-of-range offset, which the hardware drops).
+  //      All LDS reads are issued before the first store (no branches), so
+  //      their latencies overlap.
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(A.val, 0, int(A.val_bytes), 0x00020000);
   const uint32_t total = L.ooff[cur.nown];
+  {
+    const int after = next + int(gridDim.x);
+    pf_load_header(cur, A, after < A.n_patches ? after : patch);
+  }
+  constexpr int NS = PF_MAX_OUT / PF_THREADS;
+  uint32_t off[NS];
+  u32x2 v[NS];
 #pragma unroll
-  for (int k = 0; k < PF_MAX_OUT / PF_THREADS; ++k) {
+  for (int k = 0; k < NS; ++k) {
     const uint32_t pos = uint32_t(t + k * PF_THREADS);
-    const bool valid = pos < total;
-    const uint32_t pc = valid ? pos : 0u;
-    const uint32_t off = valid ? pos_off[pc] : 0xFFFFFFFFu;
-    const u32x2 v = __builtin_bit_cast(u32x2, lds[pc]);
-    __builtin_amdgcn_raw_buffer_store_b64(v, rsrc, off, 0, 0);
+    const uint32_t pc = min(pos, total - 1u);  // total >= 1: a patch owns at least one node
+    off[k] = pos_off[pc];
+    v[k] = __builtin_bit_cast(u32x2, lds[pc]);
+  }
+#pragma unroll
+  for (int k = 0; k < NS; ++k) {
+    const uint32_t pos = uint32_t(t + k * PF_THREADS);
+    __builtin_amdgcn_raw_buffer_store_b64(v[k], rsrc, pos < total ? off[k] : 0xFFFFFFFFu, 0, 0);
   }
   return has_next;
 }
@@ -322,6 +338,13 @@ __global__ __launch_bounds__(PF_THREADS, 2) void plate_fused_kernel(PatchArgs A)
   pf_load_header(P0, A, patch);
   pf_load_maps(P0, A, threadIdx.x);
   pf_load_element(P0, A);
+  const int second = patch + int(gridDim.x);
+  pf_load_header(P1, A, second < A.n_patches ? second : patch);
+  // Start the loop with nothing in flight, so that at the loop head every
+  // path has the same outstanding vector-memory operations (this patch's
+  // loads older than the previous patch's 16 stores) and waits on them
+  // leave the stores in flight.
+  __builtin_amdgcn_s_waitcnt(0);
   while (true) {
     if (!pf_process(P0, P1, patch, A, L)) break;
     patch += int(gridDim.x);

@@ -26,6 +26,7 @@
 #include <algorithm>
 #include <array>
 #include <charconv>
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -260,6 +261,13 @@ struct Model {
   std::vector<V> t_val;
   std::vector<uint8_t> t_structural;  // 1 if the entry is in the element's structural pattern
   std::vector<uint8_t> t_nonzero;     // 1 if the crate would call add_value (value != 0)
+  std::vector<uint16_t> t_local;      // position li * n + lj of the entry in its element matrix
+  struct ERange {
+    int type;
+    uint32_t number;
+    size_t t0, t1;
+  };
+  std::vector<ERange> erange;  // elements in insertion order and their triplet ranges
   std::string last_error;
   // boundary conditions (methods_for_bc_data_handle.rs): per global dof
   std::vector<uint8_t> constrained;
@@ -827,6 +835,7 @@ void scatter(Model<V>& m, const std::vector<V>& K, const std::vector<uint8_t>& p
           m.t_val.push_back(v);
           m.t_structural.push_back(structural);
           m.t_nonzero.push_back(v != V(0));
+          m.t_local.push_back(uint16_t(li * n + lj));
         }
 }
 
@@ -858,7 +867,9 @@ bool add_truss(Model<V>& m, uint32_t number, uint32_t n1, uint32_t n2, V E, V A,
   auto P = rotate_pattern(truss_local_pattern(), e.R, 6);
   // only the translational 3x3 blocks are scattered (truss.rs TRUSS_NODE_DOF = 3)
   std::vector<size_t> idx = {m.nodes.at(n1).index, m.nodes.at(n2).index};
+  const size_t t0 = m.t_row.size();
   scatter(m, K, P, 6, idx, 3);
+  m.erange.push_back({0, number, t0, m.t_row.size()});
   m.trusses[number] = e;
   m.truss_pairs[key] = number;
   return true;
@@ -908,7 +919,9 @@ bool add_beam(Model<V>& m, uint32_t number, uint32_t n1, uint32_t n2, V E, V nu,
   auto K = rotate_tkt(k, e.R, 12);
   auto P = beam_global_pattern(e.R);
   std::vector<size_t> idx = {m.nodes.at(n1).index, m.nodes.at(n2).index};
+  const size_t t0 = m.t_row.size();
   scatter(m, K, P, 12, idx, 6);
+  m.erange.push_back({1, number, t0, m.t_row.size()});
   m.beams[number] = e;
   m.beam_pairs[key] = number;
   return true;
@@ -971,7 +984,9 @@ bool add_plate(Model<V>& m, uint32_t number, const uint32_t nn[4], V E, V nu, V 
   auto P = rotate_pattern(plate_local_pattern(), e.R, 24);
   std::vector<size_t> idx = {m.nodes.at(nn[0]).index, m.nodes.at(nn[1]).index, m.nodes.at(nn[2]).index,
                              m.nodes.at(nn[3]).index};
+  const size_t t0 = m.t_row.size();
   scatter(m, K, P, 24, idx, 6);
+  m.erange.push_back({2, number, t0, m.t_row.size()});
   m.plates[number] = e;
   for (int i = 0; i < 4; ++i) m.node_plates[nn[i]].push_back(number);
   return true;
@@ -1123,6 +1138,77 @@ bool separate_sparse(Model<V>& m) {
     return false;
   }
   return true;
+}
+
+// ---------------------------------------------------------------- timed CPU baseline
+// Steady-state value recompute of the assembled K on the host, for the CPU
+// baseline: every element matrix again (dense local k, (T^T k) T, as add_*
+// computes it), then every CSR value as the sum of its contributions in
+// element-insertion order (the order export_csr sums them in).  Elements are
+// spread over OpenMP threads; each CSR value is summed by one thread, so the
+// result does not depend on the thread count.  Returns the median seconds of
+// `reps` runs and the number of values that differ from export_csr(mode 1).
+template <class V>
+double time_values(Model<V>& m, int threads, int reps, int64_t* mismatches) {
+  // contributions of each structural CSR value, in insertion order (as export_csr mode 1)
+  std::vector<size_t> order;
+  order.reserve(m.t_row.size());
+  for (size_t i = 0; i < m.t_row.size(); ++i)
+    if (m.t_structural[i]) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return m.t_row[a] != m.t_row[b] ? m.t_row[a] < m.t_row[b] : m.t_col[a] < m.t_col[b];
+  });
+  std::vector<size_t> slot;  // slot s sums order[slot[s] .. slot[s + 1])
+  for (size_t i = 0; i < order.size(); ++i)
+    if (i == 0 || m.t_row[order[i]] != m.t_row[order[i - 1]] || m.t_col[order[i]] != m.t_col[order[i - 1]])
+      slot.push_back(i);
+  slot.push_back(order.size());
+  const size_t ns = slot.size() - 1;
+  std::vector<V> tv(m.t_val.size());
+  std::vector<double> val(ns);
+  std::vector<double> times;
+  for (int rep = 0; rep < reps; ++rep) {
+    auto t0 = std::chrono::steady_clock::now();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 256)
+    for (int64_t ei = 0; ei < int64_t(m.erange.size()); ++ei) {
+      const auto& er = m.erange[size_t(ei)];
+      std::vector<V> K;
+      int n = 0;
+      if (er.type == 0) {
+        const auto& e = m.trusses.at(er.number);
+        auto d = element_vector(m, e.n1, e.n2);
+        K = rotate_tkt(truss_local_k(e, norm3(d)), e.R, 6);
+        n = 6;
+      } else if (er.type == 1) {
+        const auto& e = m.beams.at(er.number);
+        auto d = element_vector(m, e.n1, e.n2);
+        K = rotate_tkt(beam_local_k(e, norm3(d)), e.R, 12);
+        n = 12;
+      } else {
+        const auto& e = m.plates.at(er.number);
+        K = rotate_tkt(plate_local_k(m, e), e.R, 24);
+        n = 24;
+      }
+      (void)n;
+      for (size_t t = er.t0; t < er.t1; ++t) tv[t] = K[m.t_local[t]];
+    }
+#pragma omp parallel for num_threads(threads) schedule(static)
+    for (int64_t s = 0; s < int64_t(ns); ++s) {
+      V acc = tv[order[slot[size_t(s)]]];
+      for (size_t j = slot[size_t(s)] + 1; j < slot[size_t(s) + 1]; ++j) acc += tv[order[j]];
+      val[size_t(s)] = double(acc);
+    }
+    auto t1 = std::chrono::steady_clock::now();
+    times.push_back(std::chrono::duration<double>(t1 - t0).count());
+  }
+  std::sort(times.begin(), times.end());
+  int64_t bad = 0;
+  export_csr(m, 1);
+  if (m.val.size() != ns) bad = -1;
+  else
+    for (size_t i = 0; i < ns; ++i) bad += std::memcmp(&m.val[i], &val[i], sizeof(double)) != 0;
+  if (mismatches) *mismatches = bad;
+  return times.empty() ? 0.0 : times[times.size() / 2];
 }
 
 struct AnyModel {
@@ -1331,6 +1417,12 @@ int64_t om_structural_violations(AnyModel* am) {
     return c;
   };
   return am->f32 ? cnt(*am->f) : cnt(*am->d);
+}
+
+// CPU baseline: median seconds of `reps` value recomputes on `threads` OpenMP
+// threads; *mismatches = values differing from the sequential export.
+double om_time_values(AnyModel* am, int threads, int reps, int64_t* mismatches) {
+  return am->f32 ? time_values(*am->f, threads, reps, mismatches) : time_values(*am->d, threads, reps, mismatches);
 }
 
 // boundary conditions; dof 0..5 = X, Y, Z, ThX, ThY, ThZ.  Return 0 or 1 (message in om_last_error).

@@ -67,6 +67,8 @@ def lib():
         L.om_separate.argtypes = [vp, vp]
         L.om_separated_copy.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.om_separated_vectors.argtypes = [vp, vp, vp, vp, vp]
+        L.om_time_values.restype = ctypes.c_double
+        L.om_time_values.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.om_eval_element.restype = ctypes.c_int
         L.om_eval_element.argtypes = [ctypes.c_int, vp, vp, dbl, dbl, ctypes.c_uint, vp]
         _lib = L
@@ -280,6 +282,14 @@ class OracleModel:
         ra, ub = np.zeros(n_aa), np.zeros(n_bb)
         lib().om_separated_vectors(self._h, _ptr(aa), _ptr(bb), _ptr(ra), _ptr(ub))
         return mats, aa, bb, ra, ub
+
+    def time_values(self, threads=1, reps=3):
+        """Median seconds of a host value recompute (element matrices on
+        `threads` OpenMP threads, insertion-order sums) and the number of
+        values that differ from the sequential CSR export (0 expected)."""
+        bad = np.zeros(1, dtype=np.int64)
+        sec = lib().om_time_values(self._h, int(threads), int(reps), _ptr(bad))
+        return sec, int(bad[0])
 
     def structural_violations(self):
         return lib().om_structural_violations(self._h)
