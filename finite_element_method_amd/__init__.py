@@ -91,6 +91,8 @@ def lib():
         "fem_update_nodes": (ci, [vp, vp, vp]),
         "fem_add_displacements": (ci, [vp, sz, vp, vp, vp, psz]),
         "fem_add_concentrated_loads": (ci, [vp, sz, vp, vp, vp, psz]),
+        "fem_add_uniformly_distributed_line_loads": (ci, [vp, sz, vp, vp, vp, psz]),
+        "fem_add_uniformly_distributed_surface_loads": (ci, [vp, sz, vp, vp, vp, psz]),
         "fem_separate": (ci, [vp, vp, ctypes.POINTER(i64), ctypes.POINTER(i64), vp]),
         "fem_separated_device": (ci, [vp, ci, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp)]),
         "fem_separated_to_host": (ci, [vp, ci, vp, vp, vp]),
@@ -98,6 +100,14 @@ def lib():
                                               ctypes.POINTER(vp)]),
         "fem_separated_vectors_to_host": (ci, [vp, vp, vp, vp, vp]),
         "fem_last_separate_ms": (dbl, [vp]),
+        "fem_solve_pcg": (ci, [vp, ci, ctypes.c_uint64, vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ci)]),
+        "fem_solution_device": (ci, [vp, ctypes.POINTER(vp), ctypes.POINTER(vp)]),
+        "fem_solution_to_host": (ci, [vp, vp, vp]),
+        "fem_last_solve_ms": (dbl, [vp]),
+        "fem_compose_global_result": (ci, [vp]),
+        "fem_global_result": (ci, [vp, vp, vp, vp, vp]),
+        "fem_solution_displacements": (ci, [vp, vp, ctypes.POINTER(vp)]),
+        "fem_element_forces": (ci, [vp, vp, vp, vp, vp, vp]),
         "fem_batch_create": (ci, [vp, sz, ctypes.POINTER(vp)]),
         "fem_batch_destroy": (None, [vp]),
         "fem_batch_last_error": (ctypes.c_char_p, [vp]),
@@ -140,6 +150,7 @@ class FEM:
         if rc != FEM_OK:
             raise DeviceError("fem_create failed")
         self._h = h
+        self._sep_sizes = (0, 0)
         self.rel_tol, self.abs_tol, self.nodes_number = float(rel_tol), float(abs_tol), int(nodes_number)
 
     @classmethod
@@ -320,6 +331,33 @@ class FEM:
         """FEM::add_concentrated_load (methods_for_bc_data_handle.rs:34-56)."""
         self.add_concentrated_loads([node_number], [dof_parameter], [value])
 
+    def _distributed(self, fn, numbers, dofs, values):
+        numbers = np.ascontiguousarray(numbers, dtype=np.uint32)
+        dofs = np.ascontiguousarray([_dof(d) for d in np.atleast_1d(dofs)], dtype=np.int32)
+        values = np.ascontiguousarray(values, dtype=np.float64)
+        if not len(numbers) == len(dofs) == len(values):
+            raise ValueError("numbers, dofs and values differ in length")
+        return self._batched(fn, len(numbers), numbers, dofs, values)
+
+    def add_uniformly_distributed_line_loads(self, beam_numbers, dofs, values):
+        return self._distributed(lib().fem_add_uniformly_distributed_line_loads, beam_numbers, dofs, values)
+
+    def add_uniformly_distributed_surface_loads(self, plate_numbers, dofs, values):
+        return self._distributed(lib().fem_add_uniformly_distributed_surface_loads, plate_numbers, dofs, values)
+
+    def add_uniformly_distributed_line_load(self, beam_element_number, dof_parameter, value):
+        """FEM::add_uniformly_distributed_line_load (methods_for_bc_data_handle.rs:58-100)."""
+        self.add_uniformly_distributed_line_loads([beam_element_number], [dof_parameter], [value])
+
+    def add_uniformly_distributed_surface_load(self, plate_element_number, dof_parameter, value):
+        """FEM::add_uniformly_distributed_surface_load (methods_for_bc_data_handle.rs:102-174)."""
+        self.add_uniformly_distributed_surface_loads([plate_element_number], [dof_parameter], [value])
+
+    def loads(self):
+        """The load vector (6 per node, node insertion order) as
+        extract_global_analysis_result reports it."""
+        return np.array([r[3] for r in self.extract_global_analysis_result()])
+
     def separate(self, assemble=True, stream=None):
         """Partitions the assembled stiffness on the GPU
         (separate_stiffness_matrix_sparse_iterative,
@@ -331,6 +369,7 @@ class FEM:
         nnz = np.zeros(4, dtype=np.int64)
         self._call(lib().fem_separate(self._h, _stream_handle(stream), ctypes.byref(n_aa), ctypes.byref(n_bb),
                                       _ptr(nnz)))
+        self._sep_sizes = (n_aa.value, n_bb.value)
         return n_aa.value, n_bb.value, dict(zip(SEPARATED, (int(x) for x in nnz)))
 
     def separate_sparse(self, assemble=True, stream=None):
@@ -367,6 +406,71 @@ class FEM:
 
     def last_separate_ms(self):
         return lib().fem_last_separate_ms(self._h)
+
+    # ----------------------------------------------------------- global analysis
+    def solve_pcg(self, preconditioner="jacobi", max_iter=1000, stream=None):
+        """PCG on the last separation (find_ua_vector_iterative_pcg_{jacobi,
+        block_jacobi}_sparse + find_r_r_vector_sparse,
+        methods_for_global_analysis.rs:186-333).  Returns (u_a, r_r,
+        iterations, converged) with host copies of u_a and r_r."""
+        pre = {"jacobi": 0, "block_jacobi": 1}.get(preconditioner, preconditioner)
+        it, conv = ctypes.c_uint64(), ctypes.c_int()
+        self._call(lib().fem_solve_pcg(self._h, int(pre), int(max_iter), _stream_handle(stream), ctypes.byref(it),
+                                       ctypes.byref(conv)))
+        n_aa, n_bb = self._sep_sizes
+        ua, rr = np.zeros(n_aa), np.zeros(n_bb)
+        self._call(lib().fem_solution_to_host(self._h, _ptr(ua), _ptr(rr)))
+        return ua, rr, int(it.value), bool(conv.value)
+
+    def last_solve_ms(self):
+        return lib().fem_last_solve_ms(self._h)
+
+    def compose_global_analysis_result(self):
+        """methods_for_global_analysis.rs:335-358 for the last solve."""
+        self._call(lib().fem_compose_global_result(self._h))
+
+    def extract_global_analysis_result(self):
+        """methods_for_global_analysis.rs:360-384: [(node, DOFParameter, displacement, load)]
+        in node insertion order."""
+        n = 6 * self.model_size()[0]
+        nums, dofs = np.zeros(n, dtype=np.uint32), np.zeros(n, dtype=np.int32)
+        disp, load = np.zeros(n), np.zeros(n)
+        self._call(lib().fem_global_result(self._h, _ptr(nums), _ptr(dofs), _ptr(disp), _ptr(load)))
+        return [(int(a), DOFParameter(int(b)), float(c), float(d)) for a, b, c, d in zip(nums, dofs, disp, load)]
+
+    FORCES_PER = {TRUSS: 1, BEAM: 10, PLATE: 8}
+    FORCE_NAMES = {
+        TRUSS: ("ForceR",),
+        BEAM: ("ForceR", "ForceS", "ForceT", "MomentR", "MomentS", "MomentS", "MomentS", "MomentT", "MomentT",
+               "MomentT"),
+        PLATE: ("MembraneForceR", "MembraneForceS", "MembraneForceRS", "BendingMomentR", "BendingMomentS",
+                "BendingMomentRS", "ShearForceRT", "ShearForceST"),
+    }
+
+    def element_forces(self, u=None, stream=None):
+        """Element forces (extract_elements_analysis_result) from global
+        displacements u (a float64 CUDA tensor of 6 per node; default: the
+        last solve's).  Returns {TRUSS: (n, 1), BEAM: (n, 10), PLATE: (n, 8)}
+        float64 CUDA tensors, elements in insertion order."""
+        import torch
+
+        if stream is None:
+            stream = torch.cuda.current_stream()
+        if u is None:
+            p = ctypes.c_void_p()
+            self._call(lib().fem_solution_displacements(self._h, _stream_handle(stream), ctypes.byref(p)))
+            uptr = p
+        else:
+            if u.numel() != 6 * self.nodes_number or u.dtype != torch.float64 or not u.is_cuda or \
+                    not u.is_contiguous():
+                raise ValueError("expected a contiguous float64 CUDA tensor of %d values" % (6 * self.nodes_number))
+            uptr = ctypes.c_void_p(u.data_ptr())
+        counts = self.model_size()[1:]
+        outs = {t: torch.empty((n, self.FORCES_PER[t]), dtype=torch.float64, device="cuda")
+                for t, n in zip((TRUSS, BEAM, PLATE), counts)}
+        ptrs = [ctypes.c_void_p(outs[t].data_ptr()) if outs[t].numel() else None for t in (TRUSS, BEAM, PLATE)]
+        self._call(lib().fem_element_forces(self._h, uptr, *ptrs, _stream_handle(stream)))
+        return outs
 
     # ----------------------------------------------------------- differentiation
     def model_size(self):

@@ -71,6 +71,19 @@ struct Device {
   int64_t sep_rows[4] = {0, 0, 0, 0}, sep_nnz[4] = {0, 0, 0, 0}, sep_base[4] = {0, 0, 0, 0};
   bool sep_done = false;
   double sep_ms = 0.0;
+  // sparse global analysis (solve.hip): b, x = u_a, r, z, p, Ap
+  double* sol_vec[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  double *sol_rr = nullptr, *sol_rb = nullptr, *sol_part = nullptr, *sol_s = nullptr, *sol_host = nullptr;
+  double *sol_diag = nullptr, *sol_inv = nullptr;
+  uint32_t *sol_flag = nullptr, *sol_bid = nullptr;
+  int32_t* sol_starts = nullptr;
+  uint8_t* sol_tmp = nullptr;
+  size_t sol_cap[16] = {};
+  int64_t sol_na = 0, sol_nb = 0;
+  bool sol_done = false;
+  double sol_ms = 0.0;
+  double* u_glob = nullptr;  // global displacements of the last solve (6 per node)
+  size_t ucap = 0;
   // timing
   hipEvent_t ev0 = nullptr, ev1 = nullptr, ev2 = nullptr, ev3 = nullptr, ev4 = nullptr, ev5 = nullptr;
   bool timed = false, timed_bwd = false;

@@ -187,4 +187,28 @@ void separation_shape(Device* d, int64_t rows[4], int64_t nnz[4]);
 void invalidate_separation(Device* d);
 double last_separate_ms(Device* d);
 
+// Sparse global analysis on the last separation (solve.hip): b = r_a - K_ab
+// u_b, K_aa u_a = b by PCG (precond 0: Jacobi, 1: node-block Jacobi),
+// r_r = K_ba u_a + K_bb u_b - r_b.  Synchronous.
+struct SolveResult {
+  int64_t iterations = 0;
+  bool converged = false;
+  double residual = 0.0, stop = 0.0;
+};
+std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
+                      SolveResult& res);
+std::string solution(Device* d, const double** u_a, const double** r_r, int64_t* n_aa, int64_t* n_bb);
+double last_solve_ms(Device* d);
+
+// Element post-processing (forces.hip): global displacements of the last
+// solve (imposed values with u_a at the k_aa indexes, device-owned) and the
+// element forces from any device displacement vector (6 per node); out[t]
+// holds 1 (truss), 10 (beam) or 8 (plate) values per element, or is null.
+std::string solution_displacements(Device* d, void* stream, const double** u);
+std::string element_forces(Device* d, const double* u, double* const out[3], void* stream);
+// Nodal loads of uniformly distributed loads (type 1: beams, 2 per item;
+// type 2: plates, 4 per item) on the elements at positions pos; host arrays in
+// and out, synchronous.
+std::string nodal_loads(Device* d, int type, const uint32_t* pos, const double* values, size_t n, double* out);
+
 }  // namespace femdev

@@ -1182,6 +1182,54 @@ int separation_ready(fem_model* m) {
   return e.empty() ? FEM_OK : fail(m, FEM_ERR_STATE, e + ": call fem_separate");
 }
 
+// FEM::add_uniformly_distributed_line_load (methods_for_bc_data_handle.rs:58-100, type 1)
+// and add_uniformly_distributed_surface_load (:102-174, type 2): the nodal
+// loads are computed on the device and added in call order
+int add_distributed(fem_model* m, int type, size_t n, const uint32_t* numbers, const int32_t* dofs,
+                    const double* values, size_t* n_ok) {
+  if (!m || (n && (!numbers || !dofs || !values))) return FEM_ERR_ARGUMENT;
+  static const char* names[3] = {"Truss", "Beam", "Plate"};
+  size_t i = 0;
+  int rc = FEM_OK;
+  std::vector<uint32_t> pos;
+  for (; i < n; ++i) {
+    if (dofs[i] < 0 || dofs[i] > 5) {
+      rc = fail(m, FEM_ERR_ARGUMENT, "dof must be 0..5 (X, Y, Z, ThX, ThY, ThZ)");
+      break;
+    }
+    auto it = m->set[type].by_number.find(numbers[i]);
+    if (it == m->set[type].by_number.end()) {
+      rc = fail(m, FEM_ERR_VALIDATION, std::string(names[type]) + " element with number " + u(numbers[i]) +
+                                           " does not exist!");
+      break;
+    }
+    pos.push_back(it->second);
+  }
+  if (!pos.empty()) {
+    int rc2 = ensure_ready(m);
+    if (rc2) {
+      if (n_ok) *n_ok = 0;
+      return rc2;
+    }
+    const int per = type == BEAM ? 2 : 4;
+    std::vector<double> f(pos.size() * size_t(per));
+    auto e = femdev::nodal_loads(m->dev, type, pos.data(), values, pos.size(), f.data());
+    if (!e.empty()) {
+      if (n_ok) *n_ok = 0;
+      return fail(m, FEM_ERR_DEVICE, e);
+    }
+    ensure_bc(m);
+    for (size_t k = 0; k < pos.size(); ++k)
+      for (int a = 0; a < per; ++a) {
+        const size_t g = size_t(m->set[type].conn[size_t(per) * pos[k] + size_t(a)]) * 6 + size_t(dofs[k]);
+        m->bc_force[g] += f[size_t(per) * k + size_t(a)];
+      }
+    m->bc_dirty = true;
+  }
+  if (n_ok) *n_ok = i;
+  return rc;
+}
+
 }  // namespace
 
 // =========================================================================== C ABI
@@ -1533,6 +1581,16 @@ int fem_add_concentrated_loads(fem_model* m, size_t n, const uint32_t* numbers, 
   return add_bc(m, 1, n, numbers, dofs, values, n_ok);
 }
 
+int fem_add_uniformly_distributed_line_loads(fem_model* m, size_t n, const uint32_t* beam_numbers,
+                                             const int32_t* dofs, const double* values, size_t* n_ok) {
+  return add_distributed(m, BEAM, n, beam_numbers, dofs, values, n_ok);
+}
+
+int fem_add_uniformly_distributed_surface_loads(fem_model* m, size_t n, const uint32_t* plate_numbers,
+                                                const int32_t* dofs, const double* values, size_t* n_ok) {
+  return add_distributed(m, PLATE, n, plate_numbers, dofs, values, n_ok);
+}
+
 int fem_separate(fem_model* m, void* stream, int64_t* n_aa, int64_t* n_bb, int64_t nnz[4]) {
   if (!m) return FEM_ERR_ARGUMENT;
   if (!m->pattern_valid || m->dirty || !m->own_vals || !m->dev)
@@ -1620,6 +1678,103 @@ int fem_separated_vectors_to_host(fem_model* m, int64_t* k_aa_indexes, int64_t* 
 }
 
 double fem_last_separate_ms(fem_model* m) { return m && m->dev ? femdev::last_separate_ms(m->dev) : -1.0; }
+
+int fem_solve_pcg(fem_model* m, int preconditioner, uint64_t max_iter, void* stream, uint64_t* iterations,
+                  int* converged) {
+  int rc = separation_ready(m);
+  if (rc) return rc;
+  if (preconditioner != 0 && preconditioner != 1)
+    return fail(m, FEM_ERR_ARGUMENT, "preconditioner must be 0 (Jacobi) or 1 (node-block Jacobi)");
+  if (max_iter > uint64_t(INT32_MAX)) return fail(m, FEM_ERR_ARGUMENT, "max_iter too large");
+  femdev::SolveResult r;
+  auto e = femdev::solve_pcg(m->dev, preconditioner, int(max_iter), m->rel_tol, m->abs_tol, stream, r);
+  if (!e.empty()) {
+    const bool numeric = e.rfind("PCG", 0) == 0;
+    return fail(m, numeric ? FEM_ERR_VALIDATION : FEM_ERR_DEVICE, e);
+  }
+  if (iterations) *iterations = uint64_t(r.iterations);
+  if (converged) *converged = r.converged ? 1 : 0;
+  return FEM_OK;
+}
+
+int fem_solution_device(fem_model* m, const double** u_a, const double** r_r) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->dev) return fail(m, FEM_ERR_STATE, "no solution computed: call fem_solve_pcg");
+  auto e = femdev::solution(m->dev, u_a, r_r, nullptr, nullptr);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_STATE, e + ": call fem_solve_pcg");
+}
+
+int fem_solution_to_host(fem_model* m, double* u_a, double* r_r) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->dev) return fail(m, FEM_ERR_STATE, "no solution computed: call fem_solve_pcg");
+  const double *ua = nullptr, *rr = nullptr;
+  int64_t na = 0, nb = 0;
+  auto e = femdev::solution(m->dev, &ua, &rr, &na, &nb);
+  if (!e.empty()) return fail(m, FEM_ERR_STATE, e + ": call fem_solve_pcg");
+  if (u_a && na) e = femdev::copy_to_host(u_a, ua, sizeof(double) * size_t(na));
+  if (e.empty() && r_r && nb) e = femdev::copy_to_host(r_r, rr, sizeof(double) * size_t(nb));
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+double fem_last_solve_ms(fem_model* m) { return m && m->dev ? femdev::last_solve_ms(m->dev) : -1.0; }
+
+int fem_compose_global_result(fem_model* m) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->dev) return fail(m, FEM_ERR_STATE, "no solution computed: call fem_solve_pcg");
+  const double *ua = nullptr, *rr = nullptr;
+  int64_t na = 0, nb = 0;
+  auto e = femdev::solution(m->dev, &ua, &rr, &na, &nb);
+  if (!e.empty()) return fail(m, FEM_ERR_STATE, e + ": call fem_solve_pcg");
+  const int64_t *aa = nullptr, *bb = nullptr;
+  e = femdev::separated_vectors(m->dev, &aa, &bb, nullptr, nullptr);
+  if (!e.empty()) return fail(m, FEM_ERR_STATE, e);
+  std::vector<int64_t> aah(static_cast<size_t>(na)), bbh(static_cast<size_t>(nb));
+  std::vector<double> uah(static_cast<size_t>(na)), rrh(static_cast<size_t>(nb));
+  if (na) e = femdev::copy_to_host(aah.data(), aa, sizeof(int64_t) * size_t(na));
+  if (e.empty() && nb) e = femdev::copy_to_host(bbh.data(), bb, sizeof(int64_t) * size_t(nb));
+  if (e.empty() && na) e = femdev::copy_to_host(uah.data(), ua, sizeof(double) * size_t(na));
+  if (e.empty() && nb) e = femdev::copy_to_host(rrh.data(), rr, sizeof(double) * size_t(nb));
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  ensure_bc(m);
+  // methods_for_global_analysis.rs:335-358
+  for (int64_t i = 0; i < na; ++i) m->bc_disp[size_t(aah[size_t(i)])] = uah[size_t(i)];
+  for (int64_t i = 0; i < nb; ++i) m->bc_force[size_t(bbh[size_t(i)])] = rrh[size_t(i)];
+  m->bc_dirty = true;
+  return FEM_OK;
+}
+
+int fem_solution_displacements(fem_model* m, void* stream, const double** u) {
+  if (!m || !u) return FEM_ERR_ARGUMENT;
+  if (!m->dev) return fail(m, FEM_ERR_STATE, "no solution computed: call fem_solve_pcg");
+  auto e = femdev::solution_displacements(m->dev, stream, u);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_STATE, e + ": call fem_solve_pcg");
+}
+
+int fem_element_forces(fem_model* m, const double* u_device, double* truss_out, double* beam_out, double* plate_out,
+                       void* stream) {
+  if (!m || !u_device) return FEM_ERR_ARGUMENT;
+  int rc = ensure_ready(m);
+  if (rc) return rc;
+  double* const out[3] = {truss_out, beam_out, plate_out};
+  auto e = femdev::element_forces(m->dev, u_device, out, stream);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
+}
+
+int fem_global_result(fem_model* m, uint32_t* node_numbers, int32_t* dofs, double* displacements, double* loads) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  ensure_bc(m);
+  // methods_for_global_analysis.rs:360-384, nodes in insertion order
+  const size_t nn = m->node_number.size();
+  for (size_t i = 0; i < nn; ++i)
+    for (int k = 0; k < 6; ++k) {
+      const size_t o = 6 * i + size_t(k), g = 6 * size_t(i) + size_t(k);
+      if (node_numbers) node_numbers[o] = m->node_number[i];
+      if (dofs) dofs[o] = k;
+      if (displacements) displacements[o] = m->bc_disp[g];
+      if (loads) loads[o] = m->bc_force[g];
+    }
+  return FEM_OK;
+}
 
 const char* fem_assembly_path(fem_model* m) { return m ? m->path.c_str() : ""; }
 

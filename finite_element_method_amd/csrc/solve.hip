@@ -1,0 +1,443 @@
+// solve.hip -- the crate's sparse global analysis on the device
+// (methods_for_global_analysis.rs:28-133, 135-275):
+//   b   = r_a - K_ab u_b                                  (find_b_sparse)
+//   K_aa u_a = b by preconditioned conjugate gradients, Jacobi or node-block
+//   Jacobi                       (pcg_jacobi_csr / pcg_block_jacobi_csr)
+//   r_r = (K_ba u_a + K_bb u_b) - r_b                     (find_r_r_sparse)
+// on the matrices of the last separation (separate.hip).
+//
+// PCG is Saad's Algorithm 9.1 with x0 = 0, stopped when
+// ||r||_2 <= max(rel_tol ||b||_2, abs_tol) and counting the x updates.  All
+// reductions use a fixed grid and fixed trees, so a solve is bitwise
+// reproducible.  The scalars (alpha, beta, r.z) stay on the device; the host
+// reads r.r once per iteration for the stopping test.
+#include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+
+#include "device_state.hpp"
+#include "fem_device.hpp"
+
+namespace femdev {
+
+namespace {
+
+constexpr int SV_B = 256;         // threads per block of the vector kernels
+constexpr int SV_PARTS = 1024;    // blocks of the reductions (fixed: reproducible sums)
+constexpr int SPMV_LANES = 8;     // lanes per CSR row
+constexpr int BJ_MAX = 6;         // dofs per node block
+
+// scalar slots (device): 0 r.z, 1 p.Ap, 2 alpha, 3 beta, 4 r.r, 5 (r.z)_new, 6 b.b, 7 status
+enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_RZN, S_BB, S_STATUS, S_COUNT };
+
+template <class T>
+std::string dgrow(T*& p, size_t& cap, size_t n) {
+  if (n <= cap && p) return "";
+  auto e = dalloc(p, n ? n : 1);
+  if (!e.empty()) { cap = 0; return e; }
+  cap = n ? n : 1;
+  return "";
+}
+
+// y[r] = (base ? base[r] : 0) + sign * (A x)[r] - (sub ? sub[r] : 0); SPMV_LANES lanes per row,
+// each summing its strided entries in order, combined by a fixed butterfly.
+__global__ void sv_spmv(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, const double* __restrict__ v,
+                        const double* __restrict__ x, const double* base, double sign, const double* __restrict__ sub,
+                        double* y, int64_t n) {
+  const int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SPMV_LANES;
+  const int lig = threadIdx.x & (SPMV_LANES - 1);
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int64_t e = rp[r] + lig; e < rp[r + 1]; e += SPMV_LANES) acc += v[e] * x[ci[e]];
+#pragma unroll
+  for (int o = SPMV_LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, SPMV_LANES);
+  if (lig == 0) {
+    double out = base ? base[r] + sign * acc : sign * acc;
+    if (sub) out -= sub[r];
+    y[r] = out;
+  }
+}
+
+// block tree sum of per-thread values into part[blockIdx.x]
+__device__ inline void block_sum2(double a, double b, double* part_a, double* part_b) {
+  __shared__ double sa[SV_B], sb[SV_B];
+  sa[threadIdx.x] = a;
+  sb[threadIdx.x] = b;
+  __syncthreads();
+  for (int s = SV_B / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) {
+      sa[threadIdx.x] += sa[threadIdx.x + s];
+      sb[threadIdx.x] += sb[threadIdx.x + s];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part_a[blockIdx.x] = sa[0];
+    if (part_b) part_b[blockIdx.x] = sb[0];
+  }
+}
+
+// partials of (x, y) and optionally (u, w); grid SV_PARTS x SV_B, strided
+__global__ void sv_dot2(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ u,
+                        const double* __restrict__ w, int64_t n, double* part_a, double* part_b) {
+  double a = 0.0, b = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B) {
+    a += x[i] * y[i];
+    if (u) b += u[i] * w[i];
+  }
+  block_sum2(a, b, part_a, u ? part_b : nullptr);
+}
+
+// final sums of the partials into scalars, then the PCG scalar step:
+// mode 0: s[S_RZ] = a, s[S_BB] = b (start);  mode 1: s[S_PAP] = a, alpha = rz / pAp;
+// mode 2: s[S_RZN] = a, s[S_RR] = b, beta = rzn / rz, rz = rzn
+__global__ void sv_final(const double* __restrict__ part_a, const double* __restrict__ part_b, double* s, int mode) {
+  __shared__ double sa[SV_PARTS], sb[SV_PARTS];
+  for (int i = threadIdx.x; i < SV_PARTS; i += blockDim.x) {
+    sa[i] = part_a[i];
+    sb[i] = part_b ? part_b[i] : 0.0;
+  }
+  __syncthreads();
+  for (int st = SV_PARTS / 2; st > 0; st >>= 1) {
+    for (int i = threadIdx.x; i < st; i += blockDim.x) {
+      sa[i] += sa[i + st];
+      sb[i] += sb[i + st];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    s[S_RZ] = sa[0];
+    s[S_BB] = sb[0];
+    s[S_RR] = sb[0];
+  } else if (mode == 1) {
+    s[S_PAP] = sa[0];
+    s[S_ALPHA] = s[S_RZ] / sa[0];
+  } else {
+    s[S_RZN] = sa[0];
+    s[S_RR] = sb[0];
+    s[S_BETA] = sa[0] / s[S_RZ];
+    s[S_RZ] = sa[0];
+  }
+}
+
+// Jacobi: d[i] = K_aa(i, i)
+__global__ void sv_diag(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci, const double* __restrict__ v,
+                        int64_t n, double* __restrict__ d) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  double dd = 0.0;
+  for (int64_t e = rp[r]; e < rp[r + 1]; ++e)
+    if (ci[e] == r) dd = v[e];
+  d[r] = dd;
+}
+
+// node blocks of K_aa: flag[i] = row i starts a node (methods_for_global_analysis.rs:135-160)
+__global__ void sv_block_flags(const int64_t* __restrict__ aa, int64_t n, uint32_t* __restrict__ flag) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  flag[i] = i < n && (i == 0 || aa[i] / 6 != aa[i - 1] / 6);
+}
+__global__ void sv_block_starts(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ id, int64_t n,
+                                int32_t* __restrict__ starts) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i > n) return;
+  if (i == n) starts[id[n]] = int32_t(n);  // id[n] = number of blocks
+  else if (flag[i]) starts[id[i]] = int32_t(i);
+}
+
+// inverse of each node block (Gauss-Jordan, partial pivoting); status bit 1 on a singular block
+__global__ void sv_block_inverse(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                 const double* __restrict__ v, const int32_t* __restrict__ starts, int64_t nblk,
+                                 double* __restrict__ inv, double* s) {
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nblk) return;
+  const int s0 = starts[b], m = starts[b + 1] - s0;
+  double* o = inv + 36 * b;
+  if (m > BJ_MAX) {  // a node has at most six dofs; guard the fixed-size arrays anyway
+    for (int i = 0; i < 36; ++i) o[i] = 0.0;
+    s[S_STATUS] = 1.0;
+    return;
+  }
+  double a[BJ_MAX][2 * BJ_MAX];
+  for (int i = 0; i < BJ_MAX; ++i)
+    for (int j = 0; j < 2 * BJ_MAX; ++j) a[i][j] = 0.0;
+  for (int i = 0; i < m; ++i) {
+    for (int64_t e = rp[s0 + i]; e < rp[s0 + i + 1]; ++e) {
+      const int c = ci[e] - s0;
+      if (c >= 0 && c < m) a[i][c] = v[e];
+    }
+    a[i][BJ_MAX + i] = 1.0;
+  }
+  bool ok = true;
+  for (int c = 0; c < m && ok; ++c) {
+    int piv = c;
+    for (int i = c + 1; i < m; ++i)
+      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
+    if (a[piv][c] == 0.0) { ok = false; break; }
+    if (piv != c)
+      for (int j = 0; j < 2 * BJ_MAX; ++j) { const double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
+    const double d = a[c][c];
+    for (int j = 0; j < 2 * BJ_MAX; ++j) a[c][j] /= d;
+    for (int i = 0; i < m; ++i)
+      if (i != c) {
+        const double f = a[i][c];
+        if (f != 0.0)
+          for (int j = 0; j < 2 * BJ_MAX; ++j) a[i][j] -= f * a[c][j];
+      }
+  }
+  for (int i = 0; i < BJ_MAX; ++i)
+    for (int j = 0; j < BJ_MAX; ++j) o[6 * i + j] = (i < m && j < m) ? a[i][BJ_MAX + j] : 0.0;
+  if (!ok) s[S_STATUS] = 1.0;  // any writer stores the same value
+}
+
+// start: r = b, z = M^-1 r, p = z, partials of (r, z) and (b, b)
+__global__ void sv_start_jacobi(const double* __restrict__ b, const double* __restrict__ d, int64_t n,
+                                double* __restrict__ x, double* __restrict__ r, double* __restrict__ z,
+                                double* __restrict__ p, double* part_a, double* part_b) {
+  double ra = 0.0, rb = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B) {
+    const double bi = b[i], zi = bi / d[i];
+    x[i] = 0.0;
+    r[i] = bi;
+    z[i] = zi;
+    p[i] = zi;
+    ra += bi * zi;
+    rb += bi * bi;
+  }
+  block_sum2(ra, rb, part_a, part_b);
+}
+
+// update: x += alpha p, r -= alpha Ap, z = r / d; partials of (r, z) and (r, r)
+__global__ void sv_update_jacobi(const double* __restrict__ d, const double* __restrict__ p,
+                                 const double* __restrict__ ap, const double* __restrict__ s, int64_t n,
+                                 double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, double* part_a,
+                                 double* part_b) {
+  const double alpha = s[S_ALPHA];
+  double ra = 0.0, rb = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * ap[i];
+    r[i] = ri;
+    const double zi = ri / d[i];
+    z[i] = zi;
+    ra += ri * zi;
+    rb += ri * ri;
+  }
+  block_sum2(ra, rb, part_a, part_b);
+}
+
+// block Jacobi variants: one thread per node block
+__device__ inline void bj_apply(const double* __restrict__ inv, int64_t b, int m, const double* rr, double* zz) {
+  const double* o = inv + 36 * b;
+  for (int i = 0; i < m; ++i) {
+    double t = 0.0;
+    for (int j = 0; j < m; ++j) t += o[6 * i + j] * rr[j];
+    zz[i] = t;
+  }
+}
+__global__ void sv_start_block(const double* __restrict__ b, const double* __restrict__ inv,
+                               const int32_t* __restrict__ starts, int64_t nblk, double* __restrict__ x,
+                               double* __restrict__ r, double* __restrict__ z, double* __restrict__ p, double* part_a,
+                               double* part_b) {
+  double ra = 0.0, rb = 0.0;
+  for (int64_t k = int64_t(blockIdx.x) * SV_B + threadIdx.x; k < nblk; k += int64_t(SV_PARTS) * SV_B) {
+    const int s0 = starts[k], m = starts[k + 1] - s0;
+    double rr[BJ_MAX], zz[BJ_MAX];
+    for (int i = 0; i < m; ++i) {
+      rr[i] = b[s0 + i];
+      x[s0 + i] = 0.0;
+      r[s0 + i] = rr[i];
+    }
+    bj_apply(inv, k, m, rr, zz);
+    for (int i = 0; i < m; ++i) {
+      z[s0 + i] = zz[i];
+      p[s0 + i] = zz[i];
+      ra += rr[i] * zz[i];
+      rb += rr[i] * rr[i];
+    }
+  }
+  block_sum2(ra, rb, part_a, part_b);
+}
+__global__ void sv_update_block(const double* __restrict__ inv, const int32_t* __restrict__ starts, int64_t nblk,
+                                const double* __restrict__ p, const double* __restrict__ ap,
+                                const double* __restrict__ s, double* __restrict__ x, double* __restrict__ r,
+                                double* __restrict__ z, double* part_a, double* part_b) {
+  const double alpha = s[S_ALPHA];
+  double ra = 0.0, rb = 0.0;
+  for (int64_t k = int64_t(blockIdx.x) * SV_B + threadIdx.x; k < nblk; k += int64_t(SV_PARTS) * SV_B) {
+    const int s0 = starts[k], m = starts[k + 1] - s0;
+    double rr[BJ_MAX], zz[BJ_MAX];
+    for (int i = 0; i < m; ++i) {
+      x[s0 + i] += alpha * p[s0 + i];
+      rr[i] = r[s0 + i] - alpha * ap[s0 + i];
+      r[s0 + i] = rr[i];
+    }
+    bj_apply(inv, k, m, rr, zz);
+    for (int i = 0; i < m; ++i) {
+      z[s0 + i] = zz[i];
+      ra += rr[i] * zz[i];
+      rb += rr[i] * rr[i];
+    }
+  }
+  block_sum2(ra, rb, part_a, part_b);
+}
+
+// p = z + beta p
+__global__ void sv_direction(const double* __restrict__ z, const double* __restrict__ s, int64_t n,
+                             double* __restrict__ p) {
+  const double beta = s[S_BETA];
+  for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B)
+    p[i] = z[i] + beta * p[i];
+}
+
+// r_b: forces at the k_bb indexes
+__global__ void sv_gather(const double* __restrict__ src, const int64_t* __restrict__ idx, int64_t n,
+                          double* __restrict__ dst) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[idx[i]];
+}
+
+unsigned grid_for(int64_t items, int per_block) { return unsigned(std::max<int64_t>(1, (items + per_block - 1) / per_block)); }
+
+}  // namespace
+
+std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
+                      SolveResult& res) {
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  res = SolveResult();
+  d->sol_done = false;
+  if (!d->sep_done) return "no separation computed";
+  const int64_t na = d->sep_rows[0], nb = d->sep_rows[2];
+  const size_t una = size_t(std::max<int64_t>(na, 1)), unb = size_t(std::max<int64_t>(nb, 1));
+  std::string e;
+  for (int k = 0; k < 6; ++k)  // b, x, r, z, p, Ap
+    if (!(e = dgrow(d->sol_vec[k], d->sol_cap[k], una)).empty()) return e;
+  if (!(e = dgrow(d->sol_rr, d->sol_cap[6], unb)).empty()) return e;
+  if (!(e = dgrow(d->sol_rb, d->sol_cap[7], unb)).empty()) return e;
+  if (!(e = dgrow(d->sol_part, d->sol_cap[8], size_t(2 * SV_PARTS))).empty()) return e;
+  if (!(e = dgrow(d->sol_s, d->sol_cap[9], size_t(S_COUNT))).empty()) return e;
+  if (!d->sol_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&d->sol_host), 8 * sizeof(double)));
+  double *b = d->sol_vec[0], *x = d->sol_vec[1], *r = d->sol_vec[2], *z = d->sol_vec[3], *p = d->sol_vec[4],
+         *ap = d->sol_vec[5];
+  double *pa = d->sol_part, *pb = d->sol_part + SV_PARTS;
+  const int64_t* rp[4];
+  const int32_t* ci[4];
+  const double* v[4];
+  for (int w = 0; w < 4; ++w)
+    if (!(e = separated_csr(d, w, &rp[w], &ci[w], &v[w])).empty()) return e;
+  const int64_t *aa_idx = d->sep_aa, *bb_idx = d->sep_bb;
+  const double *r_a = d->sep_ra, *u_b = d->sep_ub;
+  const unsigned B = SV_B;
+  HIP_TRY(hipEventRecord(d->ev4, s));
+  // b = r_a - K_ab u_b
+  if (na) {
+    sv_spmv<<<grid_for(na * SPMV_LANES, B), B, 0, s>>>(rp[1], ci[1], v[1], u_b, r_a, -1.0, nullptr, b, na);
+    HIP_TRY(hipGetLastError());
+  }
+  // preconditioner
+  int64_t nblk = 0;
+  double* zero_s = d->sol_host;
+  for (int k = 0; k < S_COUNT && k < 8; ++k) zero_s[k] = 0.0;
+  HIP_TRY(hipMemcpyAsync(d->sol_s, zero_s, sizeof(double) * S_COUNT, hipMemcpyHostToDevice, s));
+  if (precond == 0) {
+    if (!(e = dgrow(d->sol_diag, d->sol_cap[10], una)).empty()) return e;
+    if (na) {
+      sv_diag<<<grid_for(na, B), B, 0, s>>>(rp[0], ci[0], v[0], na, d->sol_diag);
+      HIP_TRY(hipGetLastError());
+    }
+  } else {
+    if (!(e = dgrow(d->sol_flag, d->sol_cap[11], una + 1)).empty()) return e;
+    if (!(e = dgrow(d->sol_bid, d->sol_cap[12], una + 1)).empty()) return e;
+    if (!(e = dgrow(d->sol_starts, d->sol_cap[13], una + 1)).empty()) return e;
+    sv_block_flags<<<grid_for(na + 1, B), B, 0, s>>>(aa_idx, na, d->sol_flag);
+    HIP_TRY(hipGetLastError());
+    size_t bytes = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d->sol_flag, d->sol_bid, int(na + 1), s));
+    if (!(e = dgrow(d->sol_tmp, d->sol_cap[14], bytes)).empty()) return e;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d->sol_tmp, bytes, d->sol_flag, d->sol_bid, int(na + 1), s));
+    sv_block_starts<<<grid_for(na + 1, B), B, 0, s>>>(d->sol_flag, d->sol_bid, na, d->sol_starts);
+    HIP_TRY(hipGetLastError());
+    uint32_t nb32 = 0;
+    HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_bid + na, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::memcpy(&nb32, d->sol_host, sizeof(uint32_t));
+    nblk = int64_t(nb32);
+    if (!(e = dgrow(d->sol_inv, d->sol_cap[15], size_t(36 * std::max<int64_t>(nblk, 1)))).empty()) return e;
+    if (nblk) {
+      sv_block_inverse<<<grid_for(nblk, B), B, 0, s>>>(rp[0], ci[0], v[0], d->sol_starts, nblk, d->sol_inv, d->sol_s);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // start
+  if (precond == 0)
+    sv_start_jacobi<<<SV_PARTS, B, 0, s>>>(b, d->sol_diag, na, x, r, z, p, pa, pb);
+  else
+    sv_start_block<<<SV_PARTS, B, 0, s>>>(b, d->sol_inv, d->sol_starts, nblk, x, r, z, p, pa, pb);
+  HIP_TRY(hipGetLastError());
+  sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 0);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_s, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  if (d->sol_host[S_STATUS] != 0.0) return "PCG(BlockJacobi): singular node block";
+  const double stop = std::max(rel_tol * std::sqrt(d->sol_host[S_BB]), abs_tol);
+  int it = 0;
+  bool conv = std::sqrt(d->sol_host[S_RR]) <= stop;
+  while (!conv && it < max_iter) {
+    ++it;
+    sv_spmv<<<grid_for(na * SPMV_LANES, B), B, 0, s>>>(rp[0], ci[0], v[0], p, nullptr, 1.0, nullptr, ap, na);
+    sv_dot2<<<SV_PARTS, B, 0, s>>>(p, ap, nullptr, nullptr, na, pa, pb);
+    sv_final<<<1, 1024, 0, s>>>(pa, nullptr, d->sol_s, 1);
+    if (precond == 0)
+      sv_update_jacobi<<<SV_PARTS, B, 0, s>>>(d->sol_diag, p, ap, d->sol_s, na, x, r, z, pa, pb);
+    else
+      sv_update_block<<<SV_PARTS, B, 0, s>>>(d->sol_inv, d->sol_starts, nblk, p, ap, d->sol_s, x, r, z, pa, pb);
+    sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 2);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipMemcpyAsync(d->sol_host + S_RR, d->sol_s + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    conv = std::sqrt(d->sol_host[S_RR]) <= stop;
+    if (!conv) {
+      sv_direction<<<SV_PARTS, B, 0, s>>>(z, d->sol_s, na, p);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // r_r = (K_ba u_a + K_bb u_b) - r_b
+  if (nb) {
+    sv_gather<<<grid_for(nb, B), B, 0, s>>>(d->sep_force, bb_idx, nb, d->sol_rb);
+    sv_spmv<<<grid_for(nb * SPMV_LANES, B), B, 0, s>>>(rp[2], ci[2], v[2], x, nullptr, 1.0, nullptr, d->sol_rr, nb);
+    sv_spmv<<<grid_for(nb * SPMV_LANES, B), B, 0, s>>>(rp[3], ci[3], v[3], u_b, d->sol_rr, 1.0, d->sol_rb, d->sol_rr,
+                                                       nb);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipEventRecord(d->ev5, s));
+  HIP_TRY(hipStreamSynchronize(s));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, d->ev4, d->ev5));
+  d->sol_ms = ms;
+  res.iterations = it;
+  res.converged = conv;
+  res.residual = std::sqrt(d->sol_host[S_RR]);
+  res.stop = stop;
+  d->sol_na = na;
+  d->sol_nb = nb;
+  d->sol_done = true;
+  return "";
+}
+
+std::string solution(Device* d, const double** u_a, const double** r_r, int64_t* n_aa, int64_t* n_bb) {
+  if (!d->sol_done) return "no solution computed";
+  if (u_a) *u_a = d->sol_vec[1];
+  if (r_r) *r_r = d->sol_rr;
+  if (n_aa) *n_aa = d->sol_na;
+  if (n_bb) *n_bb = d->sol_nb;
+  return "";
+}
+
+double last_solve_ms(Device* d) { return d->sol_ms; }
+
+}  // namespace femdev

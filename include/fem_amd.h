@@ -160,6 +160,18 @@ int fem_add_displacements(fem_model* m, size_t n, const uint32_t* numbers, const
 int fem_add_concentrated_loads(fem_model* m, size_t n, const uint32_t* numbers, const int32_t* dofs,
                                const double* values, size_t* n_ok);
 
+/* FEM::add_uniformly_distributed_line_load(beam, dof, value), n entries
+ *                                  methods_for_bc_data_handle.rs:58-100, beam.rs:775-797
+ * FEM::add_uniformly_distributed_surface_load(plate, dof, value), n entries
+ *                                  methods_for_bc_data_handle.rs:102-174, plate.rs:1145-1185
+ * The element's nodal loads (one point for beams, the four plate points) are
+ * computed on the device and added to the load vector in call order; the
+ * loads act along the global dof given. */
+int fem_add_uniformly_distributed_line_loads(fem_model* m, size_t n, const uint32_t* beam_numbers,
+                                             const int32_t* dofs, const double* values, size_t* n_ok);
+int fem_add_uniformly_distributed_surface_loads(fem_model* m, size_t n, const uint32_t* plate_numbers,
+                                                const int32_t* dofs, const double* values, size_t* n_ok);
+
 /* FEM::separate_stiffness_matrix_sparse_iterative()
  *                              methods_for_separate_stiffness_matrix.rs:217-320
  * plus compose_r_a_vector / compose_u_b_vector (:323-345), on the device, from
@@ -184,6 +196,47 @@ int fem_separated_vectors_to_host(fem_model* m, int64_t* k_aa_indexes, int64_t* 
                                   double* u_b);
 /* HIP-event time of the last fem_separate's kernels (ms); -1 when unavailable. */
 double fem_last_separate_ms(fem_model* m);
+
+/* ---- sparse global analysis ------------------------------------------- */
+
+/* FEM::find_ua_vector_iterative_pcg_jacobi_sparse (preconditioner 0) and
+ * find_ua_vector_iterative_pcg_block_jacobi_sparse (preconditioner 1), then
+ * find_r_r_vector_sparse        methods_for_global_analysis.rs:186-333
+ * on the last fem_separate: b = r_a - K_ab u_b, K_aa u_a = b by
+ * preconditioned CG from u_a = 0 (Jacobi, or the inverse of each node's
+ * block of K_aa, blocks as build_block_starts_from_k_aa_indexes :135-160),
+ * stopped when ||r|| <= max(rel_tol ||b||, abs_tol) or after max_iter
+ * updates; r_r = (K_ba u_a + K_bb u_b) - r_b.  Outputs: the number of
+ * updates and whether the stopping test was met.  Synchronizes `stream`. */
+int fem_solve_pcg(fem_model* m, int preconditioner, uint64_t max_iter, void* stream, uint64_t* iterations,
+                  int* converged);
+/* u_a (n_aa) and r_r (n_bb) of the last solve: borrowed device pointers
+ * (valid until the next solve) or a host copy. */
+int fem_solution_device(fem_model* m, const double** u_a, const double** r_r);
+int fem_solution_to_host(fem_model* m, double* u_a, double* r_r);
+/* HIP-event time of the last fem_solve_pcg (ms); -1 when unavailable. */
+double fem_last_solve_ms(fem_model* m);
+/* FEM::compose_global_analysis_result         methods_for_global_analysis.rs:335-358
+ * displacements[k_aa_indexes] = u_a, forces[k_bb_indexes] = r_r. */
+int fem_compose_global_result(fem_model* m);
+/* Global displacements of the last solve (6 per node: imposed values, u_a at
+ * the k_aa indexes); a borrowed device pointer valid until the next call. */
+int fem_solution_displacements(fem_model* m, void* stream, const double** u);
+/* FEM::extract_elements_analysis_result      methods_for_element_analysis.rs:27-58
+ * per element of each type, in insertion order, from device displacements
+ * u_device (6 per node):
+ *   truss (1):  ForceR                                     truss.rs:281-333
+ *   beam (10):  ForceR, ForceS, ForceT, MomentR, MomentS at node 1 / average /
+ *               node 2, MomentT at node 1 / average / node 2  beam.rs:803-993
+ *   plate (8):  MembraneForceR, S, RS, BendingMomentR, S, RS, ShearForceRT, ST
+ *                                                          plate.rs:1196-1409
+ * Outputs are device arrays (or NULL to skip a type).  Asynchronous on `stream`. */
+int fem_element_forces(fem_model* m, const double* u_device, double* truss_out, double* beam_out, double* plate_out,
+                       void* stream);
+/* FEM::extract_global_analysis_result         methods_for_global_analysis.rs:360-384
+ * 6 entries per node in insertion order: (node number, dof, displacement,
+ * load); arrays of 6 x n_nodes, any may be NULL. */
+int fem_global_result(fem_model* m, uint32_t* node_numbers, int32_t* dofs, double* displacements, double* loads);
 
 /* ---- batched assembly (no counterpart in the crate) ---------------------
  *

@@ -710,6 +710,72 @@ void quad_jacobian(const QuadLocal<V>& q, V r, V s, V J[4]) {
 // src/fem/structs/plate.rs:160-724
 // parts (test hook for reference gradients): bit 0 membrane, 1 bending,
 // 2 shear, 3 drilling; the default sums all four exactly as the crate does
+// B_mem, B_bend (3 x 24) and the MITC4 B_shear (2 x 24) of the local frame at
+// (r, s), and det J (plate.rs:160-511, quad:252-653)
+template <class V>
+void plate_b(const QuadLocal<V>& q, V r, V s, std::vector<V>& bm, std::vector<V>& bb, std::vector<V>& bs, V& det) {
+  const int n = 24;
+  V J[4];
+  quad_jacobian(q, r, s, J);
+  det = J[0] * J[3] - J[1] * J[2];
+  V inv[4] = {J[3] / det, V(-1) * J[1] / det, V(-1) * J[2] / det, J[0] / det};
+  V dr[4], ds[4];
+  quad_dh(r, s, dr, ds);
+  V dx[4], dy[4];
+  for (int a = 0; a < 4; ++a) {
+    dx[a] = inv[0] * dr[a] + inv[1] * ds[a];
+    dy[a] = inv[2] * dr[a] + inv[3] * ds[a];
+  }
+  // B_mem, B_bend (3 x 24)
+  bm.assign(3 * n, V(0));
+  bb.assign(3 * n, V(0));
+  bs.assign(2 * n, V(0));
+  for (int a = 0; a < 4; ++a) {
+    bm[0 * n + 6 * a + 0] = dx[a];
+    bm[1 * n + 6 * a + 1] = dy[a];
+    bm[2 * n + 6 * a + 0] = dy[a];
+    bm[2 * n + 6 * a + 1] = dx[a];
+    bb[0 * n + 6 * a + 4] = V(-1) * dx[a];
+    bb[1 * n + 6 * a + 3] = dy[a];
+    bb[2 * n + 6 * a + 3] = dx[a];
+    bb[2 * n + 6 * a + 4] = V(-1) * dy[a];
+  }
+  // B_shear (plate.rs:392-511), MITC4
+  V x1 = q.x1, y1 = q.y1, x2 = q.x2, y2 = q.y2, x3 = V(0), y3 = V(0), x4 = q.x4, y4 = q.y4;
+  V a_x = x1 - x2 - x3 + x4, b_x = x1 - x2 + x3 - x4, c_x = x1 + x2 - x3 - x4;
+  V a_y = y1 - y2 - y3 + y4, b_y = y1 - y2 + y3 - y4, c_y = y1 + y2 - y3 - y4;
+  auto sq = [](V v) { return v * v; };
+  V mr = std::sqrt(sq(c_x + r * b_x) + sq(c_y + r * b_y)) / (V(8) * det);
+  V ms = std::sqrt(sq(a_x + s * b_x) + sq(a_y + s * b_y)) / (V(8) * det);
+  V one = V(1);
+  V* R0 = &bs[0];
+  V* R1 = &bs[n];
+  R0[2] = (one + s) / V(2) * mr;
+  R0[3] = (one + s) * V(-1) * (y1 - y2) / V(4) * mr;
+  R0[4] = (one + s) * (x1 - x2) / V(4) * mr;
+  R0[8] = V(-1) * (one + s) / V(2) * mr;
+  R0[9] = (one + s) * V(-1) * (y1 - y2) / V(4) * mr;
+  R0[10] = (one + s) * (x1 - x2) / V(4) * mr;
+  R0[14] = V(-1) * (one - s) / V(2) * mr;
+  R0[15] = (one - s) * V(-1) * (y4 - y3) / V(4) * mr;
+  R0[16] = (one - s) * (x4 - x3) / V(4) * mr;
+  R0[20] = (one - s) / V(2) * mr;
+  R0[21] = (one - s) * V(-1) * (y4 - y3) / V(4) * mr;
+  R0[22] = (one - s) * (x4 - x3) / V(4) * mr;
+  R1[2] = (one + r) / V(2) * ms;
+  R1[3] = (one + r) * V(-1) * (y1 - y4) / V(4) * ms;
+  R1[4] = (one + r) * (x1 - x4) / V(4) * ms;
+  R1[8] = (one - r) / V(2) * ms;
+  R1[9] = (one - r) * V(-1) * (y2 - y3) / V(4) * ms;
+  R1[10] = (one - r) * (x2 - x3) / V(4) * ms;
+  R1[14] = V(-1) * (one - r) / V(2) * ms;
+  R1[15] = (one - r) * V(-1) * (y2 - y3) / V(4) * ms;
+  R1[16] = (one - r) * (x2 - x3) / V(4) * ms;
+  R1[20] = V(-1) * (one + r) / V(2) * ms;
+  R1[21] = (one + r) * V(-1) * (y1 - y4) / V(4) * ms;
+  R1[22] = (one + r) * (x1 - x4) / V(4) * ms;
+}
+
 template <class V>
 std::vector<V> plate_local_k(const Model<V>& m, const Plate<V>& e, unsigned parts = 15) {
   const int n = 24;
@@ -719,63 +785,9 @@ std::vector<V> plate_local_k(const Model<V>& m, const Plate<V>& e, unsigned part
   for (int ip = 0; ip < 4; ++ip) {
     V r = ips[ip][0], s = ips[ip][1];
     V alpha = V(1) * V(1);
-    V J[4];
-    quad_jacobian(q, r, s, J);
-    V det = J[0] * J[3] - J[1] * J[2];
-    V inv[4] = {J[3] / det, V(-1) * J[1] / det, V(-1) * J[2] / det, J[0] / det};
-    V dr[4], ds[4];
-    quad_dh(r, s, dr, ds);
-    V dx[4], dy[4];
-    for (int a = 0; a < 4; ++a) {
-      dx[a] = inv[0] * dr[a] + inv[1] * ds[a];
-      dy[a] = inv[2] * dr[a] + inv[3] * ds[a];
-    }
-    // B_mem, B_bend (3 x 24)
-    std::vector<V> bm(3 * n, V(0)), bb(3 * n, V(0)), bs(2 * n, V(0));
-    for (int a = 0; a < 4; ++a) {
-      bm[0 * n + 6 * a + 0] = dx[a];
-      bm[1 * n + 6 * a + 1] = dy[a];
-      bm[2 * n + 6 * a + 0] = dy[a];
-      bm[2 * n + 6 * a + 1] = dx[a];
-      bb[0 * n + 6 * a + 4] = V(-1) * dx[a];
-      bb[1 * n + 6 * a + 3] = dy[a];
-      bb[2 * n + 6 * a + 3] = dx[a];
-      bb[2 * n + 6 * a + 4] = V(-1) * dy[a];
-    }
-    // B_shear (plate.rs:392-511), MITC4
-    V x1 = q.x1, y1 = q.y1, x2 = q.x2, y2 = q.y2, x3 = V(0), y3 = V(0), x4 = q.x4, y4 = q.y4;
-    V a_x = x1 - x2 - x3 + x4, b_x = x1 - x2 + x3 - x4, c_x = x1 + x2 - x3 - x4;
-    V a_y = y1 - y2 - y3 + y4, b_y = y1 - y2 + y3 - y4, c_y = y1 + y2 - y3 - y4;
-    auto sq = [](V v) { return v * v; };
-    V mr = std::sqrt(sq(c_x + r * b_x) + sq(c_y + r * b_y)) / (V(8) * det);
-    V ms = std::sqrt(sq(a_x + s * b_x) + sq(a_y + s * b_y)) / (V(8) * det);
-    V one = V(1);
-    V* R0 = &bs[0];
-    V* R1 = &bs[n];
-    R0[2] = (one + s) / V(2) * mr;
-    R0[3] = (one + s) * V(-1) * (y1 - y2) / V(4) * mr;
-    R0[4] = (one + s) * (x1 - x2) / V(4) * mr;
-    R0[8] = V(-1) * (one + s) / V(2) * mr;
-    R0[9] = (one + s) * V(-1) * (y1 - y2) / V(4) * mr;
-    R0[10] = (one + s) * (x1 - x2) / V(4) * mr;
-    R0[14] = V(-1) * (one - s) / V(2) * mr;
-    R0[15] = (one - s) * V(-1) * (y4 - y3) / V(4) * mr;
-    R0[16] = (one - s) * (x4 - x3) / V(4) * mr;
-    R0[20] = (one - s) / V(2) * mr;
-    R0[21] = (one - s) * V(-1) * (y4 - y3) / V(4) * mr;
-    R0[22] = (one - s) * (x4 - x3) / V(4) * mr;
-    R1[2] = (one + r) / V(2) * ms;
-    R1[3] = (one + r) * V(-1) * (y1 - y4) / V(4) * ms;
-    R1[4] = (one + r) * (x1 - x4) / V(4) * ms;
-    R1[8] = (one - r) / V(2) * ms;
-    R1[9] = (one - r) * V(-1) * (y2 - y3) / V(4) * ms;
-    R1[10] = (one - r) * (x2 - x3) / V(4) * ms;
-    R1[14] = V(-1) * (one - r) / V(2) * ms;
-    R1[15] = (one - r) * V(-1) * (y2 - y3) / V(4) * ms;
-    R1[16] = (one - r) * (x2 - x3) / V(4) * ms;
-    R1[20] = V(-1) * (one + r) / V(2) * ms;
-    R1[21] = (one + r) * V(-1) * (y1 - y4) / V(4) * ms;
-    R1[22] = (one + r) * (x1 - x4) / V(4) * ms;
+    V det;
+    std::vector<V> bm, bb, bs;
+    plate_b(q, r, s, bm, bb, bs, det);
     // constitutive matrices (plate.rs:532-636)
     V nu = e.nu;
     V cm_mul = e.E * e.t / (V(1) - nu * nu);
@@ -1072,6 +1084,58 @@ bool add_concentrated_load(Model<V>& m, uint32_t number, int dof, V value) {
   return true;
 }
 
+// methods_for_bc_data_handle.rs:58-100, beam.rs:775-797: q L / 2-type nodal
+// loads from one integration point (r = 0, alpha = 2)
+template <class V>
+bool add_line_load(Model<V>& m, uint32_t number, int dof, V value) {
+  auto it = m.beams.find(number);
+  if (it == m.beams.end()) {
+    m.last_error = "Beam element with number " + std::to_string(number) + " does not exist!";
+    return false;
+  }
+  ensure_bc(m);
+  const auto& e = it->second;
+  const V r = V(0), alpha = V(2);
+  const V det = bar_jacobian(norm3(element_vector(m, e.n1, e.n2)), r);
+  const V f[2] = {V(0) + (bar_h1(r) * value) * (det * alpha), V(0) + (bar_h2(r) * value) * (det * alpha)};
+  const size_t i1 = m.nodes.at(e.n1).index * NODE_DOF + size_t(dof), i2 = m.nodes.at(e.n2).index * NODE_DOF + size_t(dof);
+  m.force[i1] = m.force[i1] + f[0];
+  m.force[i2] = m.force[i2] + f[1];
+  return true;
+}
+
+// methods_for_bc_data_handle.rs:102-174, plate.rs:1145-1185: nodal loads
+// h_a(r, s) p det J(r, s) alpha_r alpha_s summed over the four points
+template <class V>
+bool add_surface_load(Model<V>& m, uint32_t number, int dof, V value) {
+  auto it = m.plates.find(number);
+  if (it == m.plates.end()) {
+    m.last_error = "Plate element with number " + std::to_string(number) + " does not exist!";
+    return false;
+  }
+  ensure_bc(m);
+  const auto& e = it->second;
+  const QuadLocal<V> q = quad_local(m, e);
+  const V ips[4][2] = {{e.ip_rs[0], e.ip_rs[0]}, {e.ip_rs[1], e.ip_rs[0]}, {e.ip_rs[1], e.ip_rs[1]},
+                       {e.ip_rs[0], e.ip_rs[1]}};
+  V f[4] = {V(0), V(0), V(0), V(0)};
+  for (int ip = 0; ip < 4; ++ip) {
+    const V r = ips[ip][0], s = ips[ip][1];
+    V J[4];
+    quad_jacobian(q, r, s, J);
+    const V det = J[0] * J[3] - J[1] * J[2];
+    const V qf = V(0.25f);
+    const V h[4] = {qf * (V(1) + r) * (V(1) + s), qf * (V(1) - r) * (V(1) + s), qf * (V(1) - r) * (V(1) - s),
+                    qf * (V(1) + r) * (V(1) - s)};
+    for (int a = 0; a < 4; ++a) f[a] = f[a] + (h[a] * value) * (det * (V(1) * V(1)));
+  }
+  for (int a = 0; a < 4; ++a) {
+    const size_t ia = m.nodes.at(e.n[a]).index * NODE_DOF + size_t(dof);
+    m.force[ia] = m.force[ia] + f[a];
+  }
+  return true;
+}
+
 // methods_for_separate_stiffness_matrix.rs:217-320 on the exported CSR (call
 // export_csr first).  Entries are classified exactly as the crate does; the
 // triplets come out sorted by (row, column) instead of in HashMap order.
@@ -1138,6 +1202,128 @@ bool separate_sparse(Model<V>& m) {
     return false;
   }
   return true;
+}
+
+// ---------------------------------------------------------------- element analysis
+// local = T g with T = block-diagonal copies of R (compose_rotation_matrix),
+// as the crate's Matrix::multiply: a full row-by-column sum
+template <class V>
+std::vector<V> to_local(const std::array<V, 9>& R, const std::vector<V>& g) {
+  const int n = int(g.size());
+  std::vector<V> out(size_t(n), V(0));
+  for (int i = 0; i < n; ++i) {
+    V acc = V(0);
+    for (int k = 0; k < n; ++k) {
+      const V t = (i / 3 == k / 3) ? R[size_t((i % 3) * 3 + k % 3)] : V(0);
+      acc += t * g[size_t(k)];
+    }
+    out[size_t(i)] = acc;
+  }
+  return out;
+}
+template <class V>
+V row_dot(const std::vector<V>& b, const std::vector<V>& x, int off = 0) {
+  V acc = V(0);
+  for (size_t k = 0; k < x.size(); ++k) acc += b[size_t(off) + k] * x[k];
+  return acc;
+}
+
+// truss.rs:281-333: ForceR
+template <class V>
+std::vector<V> truss_forces(const Model<V>& m, const Truss<V>& e, const std::vector<V>& u) {
+  const size_t i1 = m.nodes.at(e.n1).index, i2 = m.nodes.at(e.n2).index;
+  std::vector<V> g(6);
+  for (int i = 0; i < 3; ++i) {
+    g[size_t(i)] = u[i1 * NODE_DOF + size_t(i)];
+    g[size_t(3 + i)] = u[i2 * NODE_DOF + size_t(i)];
+  }
+  auto loc = to_local(e.R, g);
+  const V L = norm3(element_vector(m, e.n1, e.n2));
+  const V r = V(0);
+  const V inv_j = V(1) / bar_jacobian(L, r);
+  std::vector<V> B = {V(0) + bar_dh1(r) * inv_j, V(0), V(0), V(0) + bar_dh2(r) * inv_j, V(0), V(0)};
+  const V area = V(0) + (e.has_a2 ? (e.A2 - e.A) / V(2) * r + e.A - (e.A2 - e.A) / V(2) * V(-1) : e.A);
+  const V strain = row_dot(B, loc);
+  return {strain * (e.E * area / V(1))};
+}
+
+// beam.rs:803-993: ForceR, ForceS, ForceT, MomentR, MomentS (node 1, average,
+// node 2), MomentT (node 1, average, node 2)
+template <class V>
+std::vector<V> beam_forces(const Model<V>& m, const Beam<V>& e, const std::vector<V>& u) {
+  const size_t i1 = m.nodes.at(e.n1).index, i2 = m.nodes.at(e.n2).index;
+  std::vector<V> g(12);
+  for (int i = 0; i < 6; ++i) {
+    g[size_t(i)] = u[i1 * NODE_DOF + size_t(i)];
+    g[size_t(6 + i)] = u[i2 * NODE_DOF + size_t(i)];
+  }
+  auto loc = to_local(e.R, g);
+  const V L = norm3(element_vector(m, e.n1, e.n2));
+  const V r = V(0);
+  const V inv_j = V(1) / bar_jacobian(L, r);
+  auto row = [&](int a, int b, int c = -1, int d = -1) {
+    std::vector<V> x(12, V(0));
+    x[size_t(a)] = bar_dh1(r) * inv_j;
+    x[size_t(b)] = bar_dh2(r) * inv_j;
+    if (c >= 0) {
+      x[size_t(c)] = x[size_t(c)] - bar_h1(r);
+      x[size_t(d)] = x[size_t(d)] - bar_h2(r);
+    }
+    for (auto& v : x) v = V(0) + v;  // summed over the single point
+    return x;
+  };
+  const V G = e.E / (V(2) * (V(1) + e.nu));
+  const V fr = row_dot(row(0, 6), loc) * (e.E * e.A / V(1));
+  const V fs = row_dot(row(1, 7, 5, 11), loc) * (G * e.A * e.k / V(1));
+  const V ft = row_dot(row(2, 8, 4, 10), loc) * (G * e.A * e.k / V(1));
+  const V mr = row_dot(row(3, 9), loc) * (G * e.it / V(1));
+  const V ms = row_dot(row(4, 10), loc) * (e.E * e.i22p / V(1));
+  const V mt = row_dot(row(5, 11), loc) * (e.E * e.i11p / V(1));
+  return {fr, fs, ft, mr, ms + L * ft / V(2), ms, ms - L * ft / V(2), mt + L * fs / V(2), mt, mt - L * fs / V(2)};
+}
+
+// plate.rs:1196-1409: membrane forces R, S, RS, bending moments R, S, RS,
+// shear forces RT, ST from B summed over the four nodal points
+template <class V>
+std::vector<V> plate_forces(const Model<V>& m, const Plate<V>& e, const std::vector<V>& u) {
+  std::vector<V> g(24);
+  for (int a = 0; a < 4; ++a) {
+    const size_t ia = m.nodes.at(e.n[a]).index;
+    for (int i = 0; i < 6; ++i) g[size_t(6 * a + i)] = u[ia * NODE_DOF + size_t(i)];
+  }
+  auto loc = to_local(e.R, g);
+  const QuadLocal<V> q = quad_local(m, e);
+  const int n = 24;
+  std::vector<V> sm(3 * n, V(0)), sb(3 * n, V(0)), ss(2 * n, V(0));
+  const V pts[4][2] = {{V(1), V(1)}, {V(-1), V(1)}, {V(-1), V(-1)}, {V(1), V(-1)}};
+  for (int k = 0; k < 4; ++k) {
+    std::vector<V> bm, bb, bs;
+    V det;
+    plate_b(q, pts[k][0], pts[k][1], bm, bb, bs, det);
+    for (int i = 0; i < 3 * n; ++i) { sm[size_t(i)] = sm[size_t(i)] + bm[size_t(i)]; sb[size_t(i)] = sb[size_t(i)] + bb[size_t(i)]; }
+    for (int i = 0; i < 2 * n; ++i) ss[size_t(i)] = ss[size_t(i)] + bs[size_t(i)];
+  }
+  const V nu = e.nu;
+  const V base[9] = {V(1), nu, V(0), nu, V(1), V(0), V(0), V(0), (V(1) - nu) / V(2)};
+  const V cmm = e.E / (V(1) - nu * nu);
+  const V cbm = e.E * e.t / (V(2) * (V(1) - nu * nu));
+  const V csm = e.E / (V(2) * (V(1) + nu));
+  V em[3], eb[3], es[2];
+  for (int i = 0; i < 3; ++i) { em[i] = row_dot(sm, loc, i * n); eb[i] = row_dot(sb, loc, i * n); }
+  for (int i = 0; i < 2; ++i) es[i] = row_dot(ss, loc, i * n);
+  V fm[3], fb[3], fsh[2];
+  for (int i = 0; i < 3; ++i) {
+    V am = V(0), ab = V(0);
+    for (int j = 0; j < 3; ++j) { am += (base[i * 3 + j] * cmm) * em[j]; ab += (base[i * 3 + j] * cbm) * eb[j]; }
+    fm[i] = am * (e.t / V(4));
+    fb[i] = ab * (e.t * e.t / V(24));
+  }
+  for (int i = 0; i < 2; ++i) {
+    V a = V(0);
+    for (int j = 0; j < 2; ++j) a += ((i == j ? V(1) : V(0)) * csm) * es[j];
+    fsh[i] = a * (e.t * e.k / V(4));
+  }
+  return {fm[0], fm[1], fm[2], fb[1], fb[0], fb[2], fsh[0], fsh[1]};
 }
 
 // ---------------------------------------------------------------- timed CPU baseline
@@ -1419,6 +1605,35 @@ int64_t om_structural_violations(AnyModel* am) {
   return am->f32 ? cnt(*am->f) : cnt(*am->d);
 }
 
+int64_t om_num_elements(AnyModel* am, int type) {
+  auto cnt = [&](auto& m) {
+    int64_t c = 0;
+    for (const auto& er : m.erange) c += er.type == type;
+    return c;
+  };
+  return am->f32 ? cnt(*am->f) : cnt(*am->d);
+}
+
+// element forces from global displacements u (6 x nodes_number) for the
+// elements of `type` in insertion order (truss 1, beam 10, plate 8 values each)
+int64_t om_element_forces(AnyModel* am, int type, const double* u, double* out) {
+  auto go = [&](auto& m, auto tag) -> int64_t {
+    using V = decltype(tag);
+    std::vector<V> uu(size_t(m.nodes_number) * NODE_DOF);
+    for (size_t i = 0; i < uu.size(); ++i) uu[i] = V(u[i]);
+    int64_t w = 0;
+    for (const auto& er : m.erange) {
+      if (er.type != type) continue;
+      std::vector<V> f = type == 0 ? truss_forces(m, m.trusses.at(er.number), uu)
+                         : type == 1 ? beam_forces(m, m.beams.at(er.number), uu)
+                                     : plate_forces(m, m.plates.at(er.number), uu);
+      for (V x : f) out[w++] = double(x);
+    }
+    return w;
+  };
+  return am->f32 ? go(*am->f, float(0)) : go(*am->d, double(0));
+}
+
 // CPU baseline: median seconds of `reps` value recomputes on `threads` OpenMP
 // threads; *mismatches = values differing from the sequential export.
 double om_time_values(AnyModel* am, int threads, int reps, int64_t* mismatches) {
@@ -1435,6 +1650,23 @@ int om_add_concentrated_load(AnyModel* am, uint32_t number, int dof, double valu
   bool ok = am->f32 ? add_concentrated_load<float>(*am->f, number, dof, float(value))
                     : add_concentrated_load<double>(*am->d, number, dof, value);
   return ok ? 0 : 1;
+}
+int om_add_line_load(AnyModel* am, uint32_t number, int dof, double value) {
+  bool ok = am->f32 ? add_line_load<float>(*am->f, number, dof, float(value)) : add_line_load<double>(*am->d, number, dof, value);
+  return ok ? 0 : 1;
+}
+int om_add_surface_load(AnyModel* am, uint32_t number, int dof, double value) {
+  bool ok = am->f32 ? add_surface_load<float>(*am->f, number, dof, float(value))
+                    : add_surface_load<double>(*am->d, number, dof, value);
+  return ok ? 0 : 1;
+}
+// the force vector (6 x nodes_number)
+void om_forces(AnyModel* am, double* out) {
+  auto cp = [&](auto& m) {
+    ensure_bc(m);
+    for (size_t i = 0; i < m.force.size(); ++i) out[i] = double(m.force[i]);
+  };
+  if (am->f32) cp(*am->f); else cp(*am->d);
 }
 // separation of the exported CSR (om_export first); out[6] = n_aa, n_bb, nnz of aa, ab, ba, bb
 int om_separate(AnyModel* am, int64_t out[6]) {

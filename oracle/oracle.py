@@ -63,10 +63,19 @@ def lib():
         L.om_add_displacement.argtypes = [vp, u32, ctypes.c_int, dbl]
         L.om_add_concentrated_load.restype = ctypes.c_int
         L.om_add_concentrated_load.argtypes = [vp, u32, ctypes.c_int, dbl]
+        L.om_add_line_load.restype = ctypes.c_int
+        L.om_add_line_load.argtypes = [vp, u32, ctypes.c_int, dbl]
+        L.om_add_surface_load.restype = ctypes.c_int
+        L.om_add_surface_load.argtypes = [vp, u32, ctypes.c_int, dbl]
+        L.om_forces.argtypes = [vp, vp]
         L.om_separate.restype = ctypes.c_int
         L.om_separate.argtypes = [vp, vp]
         L.om_separated_copy.argtypes = [vp, ctypes.c_int, vp, vp, vp]
         L.om_separated_vectors.argtypes = [vp, vp, vp, vp, vp]
+        L.om_num_elements.restype = ctypes.c_int64
+        L.om_num_elements.argtypes = [vp, ctypes.c_int]
+        L.om_element_forces.restype = ctypes.c_int64
+        L.om_element_forces.argtypes = [vp, ctypes.c_int, vp, vp]
         L.om_time_values.restype = ctypes.c_double
         L.om_time_values.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
         L.om_eval_element.restype = ctypes.c_int
@@ -158,6 +167,95 @@ def separate_csr(row_ptr, col_idx, val, constrained, disp=None, force=None, node
     if len(mats["aa"][2]) == 0:
         raise OracleError("Sparse separation: K_aa is empty (structure has no free stiffness?)")
     return mats, aa.astype(np.int64), bb.astype(np.int64), force[aa].copy(), disp[bb].copy()
+
+
+# ---------------------------------------------------------------- global analysis
+# methods_for_global_analysis.rs:28-133, 135-275.  pcg_jacobi_csr and
+# pcg_block_jacobi_csr live in the absent crate iterative_solvers_smpl 0.1.5;
+# they are restated as the textbook preconditioned conjugate gradient (Saad,
+# Iterative Methods for Sparse Linear Systems, 2nd ed., Algorithm 9.1) with
+# x0 = 0, stopped when ||r||_2 <= max(rel_tol * ||b||_2, abs_tol) and
+# counting the x updates.  The stopping rule and the block inverse are
+# assumptions (parity unpinned there); the tests pin the restatement with the
+# crate's own 1-iteration truss tests, direct solves and CG invariants.
+
+def csr_matvec(mat, x, n_rows):
+    rp, ci, v = mat
+    rows = np.repeat(np.arange(n_rows), np.diff(rp))
+    y = np.zeros(n_rows)
+    np.add.at(y, rows, v * x[ci])
+    return y
+
+
+def find_b_sparse(r_a, k_ab, u_b):
+    """methods_for_global_analysis.rs:28-47: b = r_a - K_ab u_b."""
+    return np.asarray(r_a, dtype=np.float64) - csr_matvec(k_ab, np.asarray(u_b, dtype=np.float64), len(r_a))
+
+
+def block_starts(k_aa_indexes):
+    """methods_for_global_analysis.rs:135-160: K_aa rows grouped by node."""
+    aa = np.asarray(k_aa_indexes)
+    if len(aa) == 0:
+        return np.zeros(0, dtype=np.int64)
+    node = aa // 6
+    return np.concatenate([[0], np.nonzero(np.diff(node))[0] + 1]).astype(np.int64)
+
+
+def pcg(k_aa, b, max_iter, rel_tol, abs_tol, starts=None):
+    """Preconditioned CG on K_aa x = b: Jacobi (starts is None) or block
+    Jacobi over [starts[i], starts[i + 1]).  Returns (x, iterations,
+    converged)."""
+    n = len(b)
+    rp, ci, v = k_aa
+    rows = np.repeat(np.arange(n), np.diff(rp))
+    if starts is None:
+        d = np.zeros(n)
+        on = rows == ci
+        d[rows[on]] = v[on]
+        prec = lambda r: r / d
+    else:
+        bounds = list(starts) + [n]
+        inv = []
+        for s0, s1 in zip(bounds[:-1], bounds[1:]):
+            blk = np.zeros((s1 - s0, s1 - s0))
+            for r in range(s0, s1):
+                for k in range(rp[r], rp[r + 1]):
+                    if s0 <= ci[k] < s1:
+                        blk[r - s0, ci[k] - s0] = v[k]
+            inv.append((s0, s1, np.linalg.inv(blk)))
+
+        def prec(r):
+            z = np.empty_like(r)
+            for s0, s1, bi in inv:
+                z[s0:s1] = bi @ r[s0:s1]
+            return z
+    x = np.zeros(n)
+    r = np.asarray(b, dtype=np.float64).copy()
+    stop = max(rel_tol * np.linalg.norm(b), abs_tol)
+    if np.linalg.norm(r) <= stop:
+        return x, 0, True
+    z = prec(r)
+    p = z.copy()
+    rz = r @ z
+    for it in range(1, max_iter + 1):
+        ap = np.zeros(n)
+        np.add.at(ap, rows, v * p[ci])
+        alpha = rz / (p @ ap)
+        x += alpha * p
+        r -= alpha * ap
+        if np.linalg.norm(r) <= stop:
+            return x, it, True
+        z = prec(r)
+        rz_new = r @ z
+        p = z + (rz_new / rz) * p
+        rz = rz_new
+    return x, max_iter, False
+
+
+def find_r_r_sparse(k_ba, u_a, k_bb, u_b, r_b):
+    """methods_for_global_analysis.rs:92-133: r_r = K_ba u_a + K_bb u_b - r_b."""
+    n_bb = len(u_b)
+    return csr_matvec(k_ba, np.asarray(u_a), n_bb) + csr_matvec(k_bb, np.asarray(u_b), n_bb) - np.asarray(r_b)
 
 
 class OracleModel:
@@ -261,6 +359,21 @@ class OracleModel:
         if lib().om_add_concentrated_load(self._h, number, dof, float(value)):
             raise OracleError(self._err())
 
+    def add_uniformly_distributed_line_load(self, number, dof, value):
+        """FEM::add_uniformly_distributed_line_load (methods_for_bc_data_handle.rs:58-100)."""
+        if lib().om_add_line_load(self._h, number, dof, float(value)):
+            raise OracleError(self._err())
+
+    def add_uniformly_distributed_surface_load(self, number, dof, value):
+        """FEM::add_uniformly_distributed_surface_load (methods_for_bc_data_handle.rs:102-174)."""
+        if lib().om_add_surface_load(self._h, number, dof, float(value)):
+            raise OracleError(self._err())
+
+    def forces(self, n_nodes):
+        out = np.zeros(6 * n_nodes)
+        lib().om_forces(self._h, _ptr(out))
+        return out
+
     def separate_sparse(self, structural=True):
         """separate_stiffness_matrix_sparse_iterative on the exported CSR:
         {"aa","ab","ba","bb": (row_ptr, col_idx, val)}, k_aa_indexes, k_bb_indexes,
@@ -282,6 +395,16 @@ class OracleModel:
         ra, ub = np.zeros(n_aa), np.zeros(n_bb)
         lib().om_separated_vectors(self._h, _ptr(aa), _ptr(bb), _ptr(ra), _ptr(ub))
         return mats, aa, bb, ra, ub
+
+    def element_forces(self, etype, u):
+        """extract_element_analysis_result (truss.rs:281-333, beam.rs:803-993,
+        plate.rs:1196-1409) for the elements of `etype` in insertion order,
+        from global displacements u (6 per node): (n, 1 | 10 | 8)."""
+        per = {0: 1, 1: 10, 2: 8}[etype]
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        out = np.zeros(per * max(1, lib().om_num_elements(self._h, int(etype))))
+        w = lib().om_element_forces(self._h, int(etype), _ptr(u), _ptr(out))
+        return out[:w].reshape(-1, per)
 
     def time_values(self, threads=1, reps=3):
         """Median seconds of a host value recompute (element matrices on
