@@ -84,11 +84,14 @@ def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream):
         model.backward(g, xyz=xyz, stream=stream)
 
     both_ms = time_steps(fwd_bwd, steps, warmup, stream)
+    fused_ms = time_steps(lambda: model.forward_backward(val, g, xyz=xyz, stream=stream), steps, warmup, stream)
     bb = backward_bytes(mesh, nnz)
     cb = backward_bytes(mesh, nnz, compliance=True)
-    return {"bwd_ms": bwd_ms, "compliance_ms": comp_ms, "fwd_plus_bwd_ms": both_ms,
-            "fwd_plus_bwd_over_fwd": both_ms / fwd_ms,
-            "sensitivities_per_s": mesh.n_elements / (both_ms * 1e-3),
+    best_ms = min(both_ms, fused_ms)
+    return {"bwd_ms": bwd_ms, "compliance_ms": comp_ms, "fwd_then_bwd_ms": both_ms,
+            "fwd_bwd_two_streams_ms": fused_ms, "fwd_plus_bwd_ms": best_ms,
+            "fwd_plus_bwd_over_fwd": best_ms / fwd_ms,
+            "sensitivities_per_s": mesh.n_elements / (best_ms * 1e-3),
             "bwd_roofline": {"bound": "hbm", "achieved": bb / (bwd_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                              "unit": "GB/s", "frac": bb / (bwd_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                              "algorithmic_bytes_per_launch": bb},

@@ -86,6 +86,7 @@ def lib():
         "fem_model_size": (ci, [vp, psz, psz, psz, psz]),
         "fem_backward": (ci, [vp, vp, vp, vp, vp, vp, vp]),
         "fem_compliance_sensitivity": (ci, [vp, vp, vp, vp, vp, vp, vp]),
+        "fem_forward_backward": (ci, [vp, vp, vp, vp, vp, vp, vp, vp]),
         "fem_last_backward_ms": (dbl, [vp]),
         "fem_update_params": (ci, [vp, ci, vp, vp]),
         "fem_update_nodes": (ci, [vp, vp, vp]),
@@ -510,6 +511,17 @@ class FEM:
         gradient or None).  See fem_backward in include/fem_amd.h."""
         self.build_pattern()
         return self._adjoint(lib().fem_backward, dl_dk, params, xyz, stream, self.csr_info()[1])
+
+    def forward_backward(self, val, dl_dk, params=True, xyz=True, stream=None):
+        """Assembles into `val` (as assemble_into) and returns the gradients of
+        L(K) from dl_dk (as backward) in one call; the two passes run at the
+        same time on two streams.  See fem_forward_backward."""
+        self.build_pattern()
+        nnz = self.csr_info()[1]
+        if val.numel() != nnz or not val.is_cuda or val.dtype.itemsize != 8 or not val.is_contiguous():
+            raise ValueError("expected a contiguous float64 CUDA tensor of %d values" % nnz)
+        fn = lambda h, g, *rest: lib().fem_forward_backward(h, ctypes.c_void_p(val.data_ptr()), g, *rest)
+        return self._adjoint(fn, dl_dk, params, xyz, stream, nnz)
 
     def compliance_sensitivity(self, u, params=True, xyz=True, stream=None):
         """-u_e^T (dK_e/dp) u_e summed per parameter, for nodal displacements

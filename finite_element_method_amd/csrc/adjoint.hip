@@ -42,6 +42,7 @@ struct AdjArgs {
   const uint32_t* bid;
   const uint64_t* bmask;
   const uint16_t* boff;
+  const uint32_t* kpack;  // flat plates: per (element, p) the block index k of each q, 8 bits each
   const int64_t* row_ptr;
   double* dparam;  // n x params_per, may be null
   double* xs;      // n x nodes_per x 3, may be null
@@ -229,6 +230,17 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
   double up[6];
   if (MODE == MODE_U) local_u(A.g, ap, R, up);
+  // flat dL/dK blocks: the 13 (14 on the diagonal) values of block k of node
+  // ap sit at fixed offsets from its first row (2 per row in the two membrane
+  // rows, 3 per row in the three bending rows, then the drilling entry)
+  int64_t g0 = 0;
+  int nb = 0;
+  uint32_t kp = 0;
+  if (MODE == MODE_DLDK && FLAT) {
+    g0 = A.row_ptr[6 * size_t(ap)];
+    nb = int((A.row_ptr[6 * size_t(ap) + 6] - g0) / 13);
+    kp = A.kpack[4 * e + p];
+  }
   double vp_[3], up_[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -240,14 +252,18 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   for (int q = 0; q < 4; ++q) {
     double gl[36];
     if (MODE == MODE_DLDK && FLAT) {
-      const uint32_t k = A.bid[16 * e + 4 * p + q];
-      int64_t rp[6];
-      uint32_t bo[6];
-#pragma unroll
-      for (int i = 0; i < 6; ++i) rp[i] = A.row_ptr[6 * size_t(ap) + i];
-      load_boff(A, k, bo);
+      const int k = int((kp >> (8 * q)) & 255u);
+      const double* gr = A.g + g0;
       FlatBlock g;
-      load_flat(A, rp, bo, p == q, g);
+      g.v[0] = gr[2 * k];
+      g.v[1] = gr[2 * k + 1];
+      g.v[2] = gr[2 * nb + 2 * k];
+      g.v[3] = gr[2 * nb + 2 * k + 1];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) g.v[4 + 3 * i + j] = gr[(4 + 3 * i) * nb + 3 * k + j];
+      g.v[13] = p == q ? gr[13 * nb] : 0.0;
       expand_flat(g, gl);
     } else if (MODE == MODE_DLDK) {
       gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
@@ -433,6 +449,10 @@ std::string upload_adjoint_maps(Device* d, const AdjointMaps& am) {
   if (!e.empty()) return e;
   for (int t = 0; t < 3; ++t) d->adj_xoff[t] = am.xoff[t];
   d->adj_flat = am.flat;
+  if (am.flat) {
+    e = dupload(d->adj_kpack, am.kpack.data(), am.kpack.size());
+    if (!e.empty()) return e;
+  }
   d->adj_nxs = am.n_xs_rows;
   return dalloc(d->adj_xs, size_t(3 * am.n_xs_rows));
 }
@@ -454,6 +474,7 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.bid = d->adj_bid[t];
     A.bmask = d->adj_bmask;
     A.boff = d->adj_boff;
+    A.kpack = d->adj_kpack;
     A.row_ptr = d->row_ptr;
     A.dparam = dparam[t];
     A.xs = (dxyz && t != BEAM) ? d->adj_xs + 3 * d->adj_xoff[t] : nullptr;

@@ -26,6 +26,23 @@ std::string stream_create(void*& s) {
   return "";
 }
 
+std::string event_create(void*& ev) {
+  hipEvent_t e = nullptr;
+  HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  ev = e;
+  return "";
+}
+
+void event_destroy(void* ev) {
+  if (ev) hipEventDestroy(static_cast<hipEvent_t>(ev));
+}
+
+std::string stream_wait(void* waiter, void* signaler, void* ev) {
+  HIP_TRY(hipEventRecord(static_cast<hipEvent_t>(ev), static_cast<hipStream_t>(signaler)));
+  HIP_TRY(hipStreamWaitEvent(static_cast<hipStream_t>(waiter), static_cast<hipEvent_t>(ev), 0));
+  return "";
+}
+
 void stream_destroy(void* s) {
   if (s) hipStreamDestroy(static_cast<hipStream_t>(s));
 }

@@ -15,6 +15,11 @@ constexpr double PI_F32 = 3.1415927410125732;  // std::f32::consts::PI widened (
 constexpr float ONE_THIRD_F32 = 1.0f / 3.0f;
 constexpr double KROT6 = 1.0;  // plate.rs:25
 
+// Frames (rotation matrices, the beam roll) are computed without contracting
+// a * b + c into fused multiply-adds, operation for operation as the crate
+// writes them: the beam roll is acos of a cosine that is often close to +-1,
+// where a last-bit difference in the cosine moves the angle by ~1e-8.
+#pragma clang fp contract(off)
 __host__ __device__ inline double cwt(double v, double abs_tol) {  // math_functions.rs:3-12
   return fabs(v) < abs_tol ? 0.0 : v;
 }
@@ -141,6 +146,8 @@ __host__ __device__ inline bool quad_rotation(const double p2[3], const double p
   double t[3] = {0.0, 0.0, norm3(n[0], n[1], n[2])};
   return rotation_to_align(n, t, abs_tol, R);
 }
+
+#pragma clang fp contract(fast)
 
 // Per-integration-point kinematics of the 4-node plate in its local frame
 // (node 3 at the origin): dh/dx, dh/dy (quad:613-653), det J (quad:477-503),

@@ -96,6 +96,7 @@ struct PlateBlockMaps {
   std::vector<uint32_t> blk;
 };
 std::string upload_plate_block_maps(Device* d, const PlateBlockMaps& pm, int wg_per_cu);
+std::string set_plate_block_grid(Device* d, int wg_per_cu);
 std::string assemble_plate_blocks(Device* d, double* val, void* stream, bool timed = true);
 
 // Bar owner path (bar_owner.hip): meshes of trusses and beams in which every
@@ -128,6 +129,10 @@ struct Timer;
 std::string stream_create(void*& s);
 void stream_destroy(void* s);
 std::string stream_sync(void* s);
+std::string event_create(void*& ev);
+void event_destroy(void* ev);
+// `waiter` waits for the work enqueued on `signaler` so far (through `ev`)
+std::string stream_wait(void* waiter, void* signaler, void* ev);
 std::string graph_begin(void* stream);
 std::string graph_end(void* stream, Graph*& g);
 std::string graph_launch(Graph* g, void* stream);
@@ -151,6 +156,7 @@ struct AdjointMaps {
   int64_t xoff[3] = {0, 0, 0};    // first gradient row of each element type (beams have none)
   int64_t n_xs_rows = 0;
   bool flat = false;              // plates only, identity frames, standard flat block masks
+  std::vector<uint32_t> kpack;    // flat: per (element, p) the block index within node p's row of each q
 };
 std::string upload_adjoint_maps(Device* d, const AdjointMaps& am);
 // mode 0: g = dL/dK (nnz values, CSR order); mode 1: g = nodal displacements

@@ -96,8 +96,9 @@ struct fem_model {
   int64_t nnz = 0;
   std::string path = "none";
   int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
-  int pf_wg_per_cu = 2;  // fused plate: resident workgroups per CU looping over patches (0 = one per patch)
-  int pf_kernel = 0;     // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block (plate_blocks)
+  int pf_wg_per_cu = -1;  // fused plate: resident workgroups per CU looping over patches (0 = one per
+                          // patch, -1 = the kernel's default: 2 for plate_fused, 0 for plate_blocks)
+  int pf_kernel = 1;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block (plate_blocks)
   bool force_generic = false;
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
@@ -115,6 +116,11 @@ struct fem_model {
   bool bc_dirty = true;
   bool own_vals = false;  // the library's own CSR values hold the current stiffness
   int sep_lanes = 0;      // separation: lanes per CSR row (tuning option, 0 = automatic)
+  // forward + backward: the backward runs on a second stream, forked from and
+  // joined to the caller's stream
+  void* side_stream = nullptr;
+  void* fork_ev = nullptr;
+  void* join_ev = nullptr;
 };
 
 // A batch of independent meshes assembled together (no counterpart in the
@@ -758,11 +764,11 @@ std::string build_pattern(fem_model* m) {
     if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
       femdev::PlateBlockMaps pbm;
       if (m->pf_kernel == 1 && build_plate_block_maps(m, bptr, blocks, row_ptr, pbm)) {
-        e = femdev::upload_plate_block_maps(m->dev, pbm, m->pf_wg_per_cu);
+        e = femdev::upload_plate_block_maps(m->dev, pbm, m->pf_wg_per_cu < 0 ? 0 : m->pf_wg_per_cu);
         if (!e.empty()) return e;
         m->path = "plate_blocks";
       } else {
-        e = femdev::upload_plate_maps(m->dev, pm, m->pf_wg_per_cu);
+        e = femdev::upload_plate_maps(m->dev, pm, m->pf_wg_per_cu < 0 ? 2 : m->pf_wg_per_cu);
         if (!e.empty()) return e;
         m->path = "plate_fused";
       }
@@ -923,6 +929,22 @@ int ensure_adjoint(fem_model* m) {
     for (size_t e = 0; e < S.size() && am.flat; ++e)
       for (int pq = 0; pq < 16; ++pq)
         if (m->pat_bmask[am.bid[PLATE][16 * e + pq]] != ((pq / 4 == pq % 4) ? MSELF : MOFF)) am.flat = false;
+    // the fixed in-row layout needs 13 nb + 1 values per node and nb < 256
+    for (size_t a = 0; a < N && am.flat; ++a) {
+      const uint64_t nb = m->pat_bptr[a + 1] - m->pat_bptr[a];
+      if (nb && (nb > 255 || m->row_ptr_host[6 * a + 6] - m->row_ptr_host[6 * a] != int64_t(13 * nb + 1)))
+        am.flat = false;
+    }
+    if (am.flat) {
+      am.kpack.resize(S.size() * 4);
+      for (size_t e = 0; e < S.size(); ++e)
+        for (int p = 0; p < 4; ++p) {
+          uint32_t w = 0;
+          const uint64_t k0 = m->pat_bptr[S.conn[4 * e + p]];
+          for (int q = 0; q < 4; ++q) w |= uint32_t(am.bid[PLATE][16 * e + 4 * p + q] - k0) << (8 * q);
+          am.kpack[4 * e + p] = w;
+        }
+    }
   }
   am.bmask = m->pat_bmask;
   am.boff = m->pat_boff;
@@ -1253,6 +1275,9 @@ int fem_reset(fem_model* m, uint32_t nodes_number) {
 
 void fem_destroy(fem_model* m) {
   if (!m) return;
+  femdev::stream_destroy(m->side_stream);
+  femdev::event_destroy(m->fork_ev);
+  femdev::event_destroy(m->join_ev);
   femdev::destroy_device(m->dev);
   delete m;
 }
@@ -1535,6 +1560,28 @@ int fem_backward(fem_model* m, const double* dl_dk, double* dl_dparam_truss, dou
 int fem_compliance_sensitivity(fem_model* m, const double* u, double* s_truss, double* s_beam, double* s_plate,
                                double* s_xyz, void* stream) {
   return run_backward(m, 1, u, s_truss, s_beam, s_plate, s_xyz, stream);
+}
+
+int fem_forward_backward(fem_model* m, double* val_device, const double* dl_dk, double* dl_dparam_truss,
+                         double* dl_dparam_beam, double* dl_dparam_plate, double* dl_dxyz, void* stream) {
+  if (!m) return FEM_ERR_ARGUMENT;
+  if (!m->side_stream) {
+    std::string e = femdev::stream_create(m->side_stream);
+    if (e.empty()) e = femdev::event_create(m->fork_ev);
+    if (e.empty()) e = femdev::event_create(m->join_ev);
+    if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  }
+  int rc = ensure_adjoint(m);
+  if (rc) return rc;
+  // the backward does not read the forward's values: both passes run at once
+  auto e = femdev::stream_wait(m->side_stream, stream, m->fork_ev);
+  if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  rc = run_backward(m, 0, dl_dk, dl_dparam_truss, dl_dparam_beam, dl_dparam_plate, dl_dxyz, m->side_stream);
+  if (rc) return rc;
+  rc = fem_assemble_into(m, val_device, stream);
+  if (rc) return rc;
+  e = femdev::stream_wait(stream, m->side_stream, m->join_ev);
+  return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
 }
 
 int fem_model_size(const fem_model* m, size_t* n_nodes, size_t* n_truss, size_t* n_beam, size_t* n_plate) {
@@ -1911,7 +1958,8 @@ int fem_batch_autotune(fem_batch* b) {
       const bool plate_only = g->set[PLATE].size() && !g->set[TRUSS].size() && !g->set[BEAM].size();
       const int patch_opts[2] = {32, 64};
       for (int pn : patch_opts) {
-        if (!plate_only && pn != 64) continue;
+        // patch_nodes shapes only the row-staging plate kernel
+        if ((!plate_only || g->pf_kernel == 1) && pn != 64) continue;
         if (g->pf_nodes != pn) {
           g->pf_nodes = pn;
           g->pattern_valid = false;
@@ -1928,7 +1976,16 @@ int fem_batch_autotune(fem_batch* b) {
             if (rc) return rc;
             if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
           }
-        } else if (g->path == "bar_owner" || g->path == "plate_blocks") {
+        } else if (g->path == "plate_blocks") {
+          for (int wg : {0, 4}) {
+            auto e = femdev::set_plate_block_grid(g->dev, wg);
+            if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+            double ms = 0;
+            rc = time_group(b, g, 3, ms);
+            if (rc) return rc;
+            if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
+          }
+        } else if (g->path == "bar_owner") {
           cfg = {pn, g->pf_wg_per_cu, 256};  // fixed workgroup shape
         } else {
           for (int bs : {128, 256, 512}) {
@@ -1964,6 +2021,10 @@ int fem_batch_autotune(fem_batch* b) {
       if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
       b->config += "group " + std::to_string(k) + ": plate_fused patch_nodes=" + std::to_string(best[k][0]) +
                    " wg_per_cu=" + std::to_string(best[k][1]) + "; ";
+    } else if (g->path == "plate_blocks") {
+      auto e = femdev::set_plate_block_grid(g->dev, best[k][1]);
+      if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+      b->config += "group " + std::to_string(k) + ": plate_blocks wg_per_cu=" + std::to_string(best[k][1]) + "; ";
     } else {
       if (g->path == "generic") femdev::set_generic_block(g->dev, best[k][2]);
       b->config += "group " + std::to_string(k) + ": " + g->path +

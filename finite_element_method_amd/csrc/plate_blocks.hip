@@ -196,6 +196,10 @@ std::string upload_plate_block_maps(Device* d, const PlateBlockMaps& pm, int wg_
   if (!(e = dupload(d->pb_conn, pm.pconn.data(), pm.pconn.size())).empty()) return e;
   if (!(e = dupload(d->pb_eid, pm.peid.data(), pm.peid.size())).empty()) return e;
   if (!(e = dupload(d->pb_blk, pm.blk.data(), pm.blk.size())).empty()) return e;
+  return set_plate_block_grid(d, wg_per_cu);
+}
+
+std::string set_plate_block_grid(Device* d, int wg_per_cu) {
   int cus = 0;
   HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, d->dev));
   d->pb_grid = wg_per_cu <= 0 ? d->pb_patches : std::max(1, std::min(d->pb_patches, cus * wg_per_cu));

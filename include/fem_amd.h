@@ -135,6 +135,13 @@ int fem_backward(fem_model* m, const double* dl_dk, double* dl_dparam_truss, dou
  * the global matrix) every output receives sum_e -u_e^T (dK_e/dp) u_e. */
 int fem_compliance_sensitivity(fem_model* m, const double* u, double* s_truss, double* s_beam, double* s_plate,
                                double* s_xyz, void* stream);
+/* Forward and backward in one call (no counterpart in the crate): the
+ * values into val_device as fem_assemble_into, and the gradients of
+ * L(K) from dl_dk as fem_backward.  The backward does not depend on the
+ * values, so it runs on a second stream of the library, forked from and
+ * joined back to `stream`; the results are the same as the two calls'. */
+int fem_forward_backward(fem_model* m, double* val_device, const double* dl_dk, double* dl_dparam_truss,
+                         double* dl_dparam_beam, double* dl_dparam_plate, double* dl_dxyz, void* stream);
 
 /* Timing of the last backward / sensitivity call (HIP events, ms); -1 when unavailable. */
 double fem_last_backward_ms(fem_model* m);
@@ -275,9 +282,13 @@ int fem_batch_mesh_csr(fem_batch* b, size_t mesh, int64_t* n_rows, int64_t* nnz,
 int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int32_t* col_idx, double* val);
 
 /* Tuning options (no counterpart in the crate; they change speed, never values):
- *   "plate_patch_nodes"  owned nodes per workgroup of the fused plate kernel: 4..64
- *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernel,
- *                        looping over the patches: 0..64 (0 = one workgroup per patch)
+ *   "plate_kernel"       fused kernel of flat plate meshes: 1 = one lane per
+ *                        owned node-pair block (plate_blocks, default),
+ *                        0 = row staging (plate_fused); bitwise equal values
+ *   "plate_patch_nodes"  owned nodes per workgroup of the row-staging kernel: 4..64
+ *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernels,
+ *                        looping over the patches: 0..64 (0 = one workgroup
+ *                        per patch; default 0 for plate_blocks, 2 for plate_fused)
  *   "force_generic"      1 = always use the generic value path
  *   "separate_lanes"     lanes per CSR row in fem_separate: 4, 8, 16, 32
  *                        (0 = automatic, from the mean row length)
@@ -285,7 +296,8 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
  * assembly). */
 int fem_set_option(fem_model* m, const char* key, int64_t value);
 
-/* Name of the value path used by the last assembly ("plate_fused" or "generic"). */
+/* Name of the value path of the last assembly: "plate_blocks", "plate_fused",
+ * "bar_owner" or "generic". */
 const char* fem_assembly_path(fem_model* m);
 
 #ifdef __cplusplus

@@ -233,3 +233,25 @@ def test_autograd_function(gpu):
         lm = float((assemble(m, plate_params=pp) * w).sum())
     fd = (lp - lm) / (2 * h)
     assert abs(fd - p.grad[e, 2].item()) < 1e-5 * abs(p.grad[e, 2].item())
+
+
+@pytest.mark.parametrize("kind", ["plate", "truss"])
+def test_forward_backward_matches_separate_calls(gpu, kind):
+    """fem_forward_backward (both passes at once on two streams) gives the
+    values of fem_assemble_into and the gradients of fem_backward bit for bit."""
+    import torch
+
+    mesh = meshgen.plate_grid(24, seed=3) if kind == "plate" else meshgen.truss_lattice(5, seed=4)
+    m = FEM.from_mesh(mesh)
+    m.build_pattern()
+    nnz = m.csr_info()[1]
+    g = torch.randn(nnz, dtype=torch.float64, device="cuda", generator=torch.Generator(device="cuda").manual_seed(1))
+    v1 = torch.empty(nnz, dtype=torch.float64, device="cuda")
+    m.assemble_into(v1)
+    p1, x1 = m.backward(g)
+    v2 = torch.empty_like(v1)
+    p2, x2 = m.forward_backward(v2, g)
+    torch.cuda.synchronize()
+    assert torch.equal(v1, v2)
+    assert p1.keys() == p2.keys() and all(torch.equal(p1[k].nan_to_num(), p2[k].nan_to_num()) for k in p1)
+    assert torch.equal(x1, x2)

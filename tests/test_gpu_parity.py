@@ -15,15 +15,17 @@ CASES = {
     "beam_frame_6": lambda: meshgen.beam_frame(6, seed=4),
     "beam_frame_4x5x3": lambda: meshgen.beam_frame(4, 5, 3, seed=5),
     "beam_frame_5": lambda: meshgen.beam_frame(5, seed=4),
+    "beam_frame_3_seed1": lambda: meshgen.beam_frame(3, seed=1),
     "plate_8": lambda: meshgen.plate_grid(8, seed=1),
     "plate_17x9": lambda: meshgen.plate_grid(17, 9, seed=2),
     "plate_64_shuffled": lambda: meshgen.plate_grid(64, seed=7, shuffle=True),
     "plate_12_tilted": lambda: meshgen.plate_grid(12, seed=8, rotation=_tilt()),
 }
 
-EXPECTED_PATH = {"plate_8": "plate_fused", "plate_17x9": "plate_fused", "plate_64_shuffled": "plate_fused",
+EXPECTED_PATH = {"plate_8": "plate_blocks", "plate_17x9": "plate_blocks", "plate_64_shuffled": "plate_blocks",
                  "plate_12_tilted": "generic", "beam_frame_6": "bar_owner", "truss_lattice_5": "bar_owner",
-                 "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner", "beam_frame_5": "bar_owner"}
+                 "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner", "beam_frame_5": "bar_owner",
+                 "beam_frame_3_seed1": "bar_owner"}
 
 
 def _tilt():
