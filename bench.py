@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the forward value kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE counter runs of this script (scaled by a 1 GiB copy kernel that reads
 # FETCH_SIZE = 0.5 x its bytes and WRITE_SIZE = 1.0 x)
-MEASURED_TRAFFIC = {("plate_512", "plate_fused"): 333.1e6}
+MEASURED_TRAFFIC = {("plate_512", "plate_blocks"): 348.3e6}
 
 WORKLOADS = {
     "plate_512": ("plate_grid", dict(nx=512, seed=0)),
@@ -134,6 +134,56 @@ def separation(model, mesh, nnz, steps, warmup):
                          "algorithmic_bytes_per_launch": alg}}
 
 
+def analysis(model, mesh, sep, steps, warmup, pcg_iters):
+    """Global analysis on the separation above: block-Jacobi PCG capped at
+    `pcg_iters` updates (cost per update; the rate is what matters, not
+    convergence), then element forces from the resulting displacements.
+    PCG time is the HIP-event time of the whole solve, including the host
+    read of r.r after every update; forces are timed with events on the
+    launch stream."""
+    import torch
+
+    n_aa, nnz_aa = sep["n_aa"], sep["nnz"]["aa"]
+    model.solve_pcg("block_jacobi", max_iter=pcg_iters)
+    ms, iters = [], 0
+    for _ in range(3):
+        iters = model.solve_pcg("block_jacobi", max_iter=pcg_iters)[2]
+        ms.append(model.last_solve_ms())
+    per_it = min(ms) / max(iters, 1)
+    # compulsory bytes of one update of a fused PCG: K_aa (values, columns, row
+    # pointers) once; p read and K p written by the product; x, p, r, K p read
+    # and x, r, z written by the update; z, p read and p written; the node
+    # blocks of the preconditioner (36 doubles per 6 dofs) read once
+    spmv = 12 * nnz_aa + 8 * (n_aa + 1)
+    it_bytes = spmv + 8 * 12 * n_aa + 48 * n_aa
+    stream = torch.cuda.current_stream()
+    for _ in range(warmup):
+        model.element_forces(stream=stream)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    e0.record(stream)
+    for _ in range(steps):
+        model.element_forces(stream=stream)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    f_ms = e0.elapsed_time(e1) / steps
+    nn = mesh.conn.shape[1]
+    n_params = {"truss": 3, "beam": 11, "plate": 4}[mesh.kind]
+    n_out = {"truss": 1, "beam": 10, "plate": 8}[mesh.kind]
+    # displacements once, connectivity, parameters, frames and forces once
+    f_bytes = 48 * mesh.n_nodes + mesh.n_elements * (4 * nn + 8 * n_params + 72 + 8 * n_out)
+
+    def roof(b, t_ms):
+        a = b / (t_ms * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": a / HBM_PEAK_GBS,
+                "algorithmic_bytes_per_launch": b}
+
+    return {"pcg": {"preconditioner": "block_jacobi", "updates": iters, "ms_per_update": per_it,
+                    "updates_per_s": 1e3 / per_it, "roofline": roof(it_bytes, per_it)},
+            "element_forces": {"ms": f_ms, "elements_per_s": mesh.n_elements / (f_ms * 1e-3),
+                               "roofline": roof(f_bytes, f_ms)}}
+
+
 def run_batched(args):
     """Batched workload: 256 independent meshes (type uniform over truss
     lattice / beam frame / plate, element counts log-uniform in [1e3, 2e5]),
@@ -230,6 +280,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the backward / sensitivity timings")
     ap.add_argument("--no-separate", action="store_true", help="skip the sparse separation timing")
+    ap.add_argument("--pcg-iters", type=int, default=50,
+                    help="PCG updates per timed solve after the separation (0 = skip the analysis leg)")
     ap.add_argument("--separate-lanes", type=int, default=0, help="separation: lanes per CSR row (0 = automatic)")
     ap.add_argument("--plate-kernel", type=int, default=-1, help="fused plate kernel: 0 = row staging, 1 = block lanes")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -320,6 +372,8 @@ def main():
         out["sensitivity"] = sensitivity(model, mesh, nnz, val, ms, args.steps, args.warmup, stream)
     if not args.no_separate:
         out["separation"] = separation(model, mesh, nnz, args.steps, args.warmup)
+        if args.pcg_iters > 0:
+            out["analysis"] = analysis(model, mesh, out["separation"], args.steps, args.warmup, args.pcg_iters)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(mesh.kind)
     print(json.dumps(out), flush=True)
