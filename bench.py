@@ -283,7 +283,7 @@ def main():
     ap.add_argument("--pcg-iters", type=int, default=50,
                     help="PCG updates per timed solve after the separation (0 = skip the analysis leg)")
     ap.add_argument("--separate-lanes", type=int, default=0, help="separation: lanes per CSR row (0 = automatic)")
-    ap.add_argument("--plate-kernel", type=int, default=-1, help="fused plate kernel: 0 = row staging, 1 = block lanes")
+    ap.add_argument("--plate-kernel", type=int, default=-1, help="fused plate kernel: 0 = row staging, 1 = block lanes, 2 = corner lanes")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="run one elementwise copy of this many MiB first (a known-traffic kernel that "
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")

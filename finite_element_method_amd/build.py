@@ -11,7 +11,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libfem_amd.so")
 SOURCES = ["fem_host.cpp", "fem_device.hip", "plate_fused.hip", "adjoint.hip", "batch.hip", "bar_owner.hip",
-           "separate.hip", "plate_blocks.hip", "solve.hip",
+           "separate.hip", "plate_blocks.hip", "plate_corners.hip", "solve.hip",
            "forces.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FEM_AMD_ARCH", "gfx950")

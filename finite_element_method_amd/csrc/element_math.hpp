@@ -203,6 +203,11 @@ struct PlateMat {
   double cb00, cb01, cb22;  // bending:  E t^3/(12(1-nu^2)) * [...]
   double cs;                // shear:    E t k/(2(1+nu)) (alpha = 1)
 };
+// The fused plate kernels must give the same bits for the same element, and
+// whether a multiply and an add contract into an FMA depends on the code
+// around an inlined helper.  So the plate helpers below are compiled without
+// contraction, and the FMAs they want are written out.
+#pragma clang fp contract(off)
 // plate.rs:532-636
 __host__ __device__ inline PlateMat plate_material(double E, double nu, double t, double k) {
   PlateMat m;
@@ -214,6 +219,7 @@ __host__ __device__ inline PlateMat plate_material(double E, double nu, double t
   m.cs = E * t * k / (2.0 * (1.0 + nu));
   return m;
 }
+#pragma clang fp contract(fast)
 
 // Local-frame 6x6 block (p, q) of the plate stiffness accumulated over one
 // integration point: membrane (u, v), bending (theta_x, theta_y) and MITC4
@@ -308,6 +314,7 @@ __host__ __device__ inline void plate_row_add_ip(const PlateIP& g, const PlateMa
 // constants.  Same values as the per-point sums up to rounding.
 // xl = {x1, y1, x2, y2, x4, y4}: local coordinates relative to node 3.
 
+#pragma clang fp contract(off)
 // per-point data: w = det J (alpha = 1), dh/dx, dh/dy (quad:613-653), alpha, beta
 struct PlatePoint {
   double w, dx[4], dy[4], al[4], be[4];
@@ -374,12 +381,12 @@ __host__ __device__ inline void plate_row_accumulate(const PlatePoint& g, double
                                                      double acc[24]) {
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
-    acc[q] += wdx * g.dx[q];
-    acc[4 + q] += wdx * g.dy[q];
-    acc[8 + q] += wdy * g.dx[q];
-    acc[12 + q] += wdy * g.dy[q];
-    acc[16 + q] += wal * g.al[q];
-    acc[20 + q] += wbe * g.be[q];
+    acc[q] = __builtin_fma(wdx, g.dx[q], acc[q]);
+    acc[4 + q] = __builtin_fma(wdx, g.dy[q], acc[4 + q]);
+    acc[8 + q] = __builtin_fma(wdy, g.dx[q], acc[8 + q]);
+    acc[12 + q] = __builtin_fma(wdy, g.dy[q], acc[12 + q]);
+    acc[16 + q] = __builtin_fma(wal, g.al[q], acc[16 + q]);
+    acc[20 + q] = __builtin_fma(wbe, g.be[q], acc[20 + q]);
   }
 }
 
@@ -388,19 +395,19 @@ __host__ __device__ inline void plate_row_accumulate(const PlatePoint& g, double
 __host__ __device__ inline void plate_block_entries(double Mxx, double Mxy, double Myx, double Myy, double A, double B,
                                                     const PlateMat& m, const double vp[3], const double up[3],
                                                     const double vq[3], const double uq[3], double o[13]) {
-  o[0] = m.cm00 * Mxx + m.cm22 * Myy;
-  o[1] = m.cm01 * Mxy + m.cm22 * Myx;
-  o[2] = m.cm01 * Myx + m.cm22 * Mxy;
-  o[3] = m.cm00 * Myy + m.cm22 * Mxx;
-  const double bxx = m.cb00 * Myy + m.cb22 * Mxx;
-  const double bxy = -(m.cb01 * Myx + m.cb22 * Mxy);
-  const double byx = -(m.cb01 * Mxy + m.cb22 * Myx);
-  const double byy = m.cb00 * Mxx + m.cb22 * Myy;
+  o[0] = __builtin_fma(m.cm00, Mxx, m.cm22 * Myy);
+  o[1] = __builtin_fma(m.cm01, Mxy, m.cm22 * Myx);
+  o[2] = __builtin_fma(m.cm01, Myx, m.cm22 * Mxy);
+  o[3] = __builtin_fma(m.cm00, Myy, m.cm22 * Mxx);
+  const double bxx = __builtin_fma(m.cb00, Myy, m.cb22 * Mxx);
+  const double bxy = -__builtin_fma(m.cb01, Myx, m.cb22 * Mxy);
+  const double byx = -__builtin_fma(m.cb01, Mxy, m.cb22 * Myx);
+  const double byy = __builtin_fma(m.cb00, Mxx, m.cb22 * Myy);
 #pragma unroll
   for (int i = 0; i < 3; ++i)
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-      double sh = (A * vp[i]) * vq[j] + (B * up[i]) * uq[j];
+      double sh = __builtin_fma(A * vp[i], vq[j], (B * up[i]) * uq[j]);
       double b = 0.0;
       if (i == 1 && j == 1) b = bxx;
       else if (i == 1 && j == 2) b = bxy;
@@ -448,6 +455,7 @@ __host__ __device__ inline void plate_row_folded(const double xl[6], const Plate
   }
   plate_row_finish<BS>(acc, m, vp, up, sv, out);
 }
+#pragma clang fp contract(fast)
 
 // Gauss abscissa of the 2x2 rule in the crate's point order (g,g), (-g,g), (-g,-g), (g,-g)
 __host__ __device__ inline double plate_gauss() { return sqrt(double(ONE_THIRD_F32)); }

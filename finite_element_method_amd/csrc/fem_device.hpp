@@ -99,6 +99,32 @@ std::string upload_plate_block_maps(Device* d, const PlateBlockMaps& pm, int wg_
 std::string set_plate_block_grid(Device* d, int wg_per_cu);
 std::string assemble_plate_blocks(Device* d, double* val, void* stream, bool timed = true);
 
+// Corner-lane fused plate path (plate_corners.hip): same preconditions as the
+// maps above, every node in at most four elements, every off-diagonal node
+// pair in at most two, and each element corner sharing at most two of its
+// blocks.  Uses the same device buffers as the block-lane path.
+constexpr int PC_NODES = 64;       // owned nodes of a patch (four lanes each)
+constexpr int PC_MAX_ELEMS = 64;   // elements touching a patch
+struct PlateCornerMaps {
+  int n_patches = 0;
+  std::vector<uint32_t> hdr;    // per patch: elem_off, n_elems
+  std::vector<uint32_t> pconn;  // 4 per patch element: node indices
+  std::vector<uint32_t> peid;   // per patch element: element id (insertion order within the patch)
+  // per patch 4 PC_NODES lanes (node slot n: lanes 4 n .. 4 n + 3, the node's
+  // elements in insertion order), 4 words each:
+  //   CSR position of row 6a of the owned node a;
+  //   patch element (127: idle lane) | p << 7 | q1 << 9 | q2 << 11 | q3 << 13 | nb << 15 |
+  //     deg << 20 (lane 0 only) | lane 0 << 23 | store slot 1 << 24 | pull into slot 2 << 25 |
+  //     source lane in the quad << 26;
+  //   k0 | k1 << 5 | k2 << 10 | k3 << 15 (block of each slot within a's rows);
+  //   0
+  // Slot 0 is the corner itself (q0 = p); slot 1 is the block this lane
+  // exports (if any), slot 2 the block it completes with the pulled one.
+  std::vector<uint32_t> lane;
+};
+std::string upload_plate_corner_maps(Device* d, const PlateCornerMaps& pm, int wg_per_cu);
+std::string assemble_plate_corners(Device* d, double* val, void* stream, bool timed = true);
+
 // Bar owner path (bar_owner.hip): meshes of trusses and beams in which every
 // off-diagonal node pair belongs to one element.  One thread per (node, row).
 struct BarMaps {

@@ -98,7 +98,8 @@ struct fem_model {
   int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
   int pf_wg_per_cu = -1;  // fused plate: resident workgroups per CU looping over patches (0 = one per
                           // patch, -1 = the kernel's default: 2 for plate_fused, 0 for plate_blocks)
-  int pf_kernel = 1;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block (plate_blocks)
+  int pf_kernel = 1;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block
+                          // (plate_blocks), 2 = one lane per element corner (plate_corners)
   bool force_generic = false;
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
@@ -517,16 +518,20 @@ bool build_plate_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std
 // Called only after build_plate_maps accepted the mesh, which checked the
 // preconditions (plates only, identity frames, standard block masks, valence
 // and row lengths).
-bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
-                            const std::vector<int64_t>& row_ptr, femdev::PlateBlockMaps& pm) {
+// Plate incidence shared by the fused-path map builders: per node its
+// elements (element | corner << 30) in insertion order, and the nodes that
+// have elements in Morton order of their coordinates (grouped by node_part
+// when the batch set one).
+void plate_node_incidence(const fem_model* m, std::vector<uint32_t>& deg, std::vector<uint32_t>& nptr,
+                          std::vector<uint32_t>& nel, std::vector<std::pair<uint64_t, uint32_t>>& order) {
   const auto& S = m->set[PLATE];
   const size_t N = m->nodes_number, E = S.size();
-  std::vector<uint32_t> deg(N + 1, 0);
+  deg.assign(N + 1, 0);
   for (size_t e = 0; e < E; ++e)
     for (int p = 0; p < 4; ++p) deg[S.conn[4 * e + p]]++;
-  std::vector<uint32_t> nptr(N + 1, 0);
+  nptr.assign(N + 1, 0);
   for (size_t a = 0; a < N; ++a) nptr[a + 1] = nptr[a] + deg[a];
-  std::vector<uint32_t> nel(nptr[N]);
+  nel.assign(nptr[N], 0);
   {
     std::vector<uint32_t> f(nptr.begin(), nptr.end() - 1);
     for (size_t e = 0; e < E; ++e)
@@ -535,7 +540,7 @@ bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, con
   double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
   for (size_t a = 0; a < N; ++a)
     for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], m->xyz[3 * a + k]); hi[k] = std::max(hi[k], m->xyz[3 * a + k]); }
-  std::vector<std::pair<uint64_t, uint32_t>> order;
+  order.clear();
   order.reserve(N);
   for (size_t a = 0; a < N; ++a) {
     if (!deg[a]) continue;
@@ -556,6 +561,15 @@ bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, con
   } else {
     std::sort(order.begin(), order.end());
   }
+}
+
+bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
+                            const std::vector<int64_t>& row_ptr, femdev::PlateBlockMaps& pm) {
+  const auto& S = m->set[PLATE];
+  const size_t E = S.size();
+  std::vector<uint32_t> deg, nptr, nel;
+  std::vector<std::pair<uint64_t, uint32_t>> order;
+  plate_node_incidence(m, deg, nptr, nel, order);
   std::vector<uint32_t> cur, cur_elems;
   uint32_t cur_blocks = 0;
   std::vector<int32_t> elem_local(E, -1);
@@ -625,6 +639,151 @@ bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, con
     }
     cur.push_back(a);
     cur_blocks += nb;
+    for (uint32_t r = 0; r < deg[a]; ++r) {
+      const uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
+      if (!in_patch[e]) { in_patch[e] = 1; cur_elems.push_back(e); }
+    }
+  }
+  for (uint32_t e : cur_elems) in_patch[e] = 0;
+  return flush();
+}
+
+// Corner-lane maps (plate_corners.hip).  Returns false when the mesh does not
+// fit the kernel's exchange scheme (the caller then uses the block-lane maps).
+bool build_plate_corner_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
+                             const std::vector<int64_t>& row_ptr, femdev::PlateCornerMaps& pm) {
+  const auto& S = m->set[PLATE];
+  const size_t E = S.size();
+  std::vector<uint32_t> deg, nptr, nel;
+  std::vector<std::pair<uint64_t, uint32_t>> order;
+  plate_node_incidence(m, deg, nptr, nel, order);
+  for (const auto& o : order)
+    if (deg[o.second] > 4 || bptr[o.second + 1] - bptr[o.second] > 31) return false;
+  std::vector<uint32_t> cur, cur_elems;
+  std::vector<int32_t> elem_local(E, -1);
+  std::vector<uint8_t> in_patch(E, 0);
+  auto block_of = [&](uint32_t a, uint32_t b) -> int {
+    const PairRef* k0 = blocks.data() + bptr[a];
+    const PairRef* k1 = blocks.data() + bptr[a + 1];
+    const PairRef* kb = std::lower_bound(k0, k1, b, [](const PairRef& x, uint32_t v) { return x.b < v; });
+    return (kb != k1 && kb->b == b) ? int(kb - k0) : -1;
+  };
+  // lane words of node a (4 lanes) into w[16]
+  auto node_lanes = [&](uint32_t a, uint32_t* w) -> bool {
+    const uint32_t d = deg[a];
+    const uint32_t* el = &nel[nptr[a]];
+    const int nb = int(bptr[a + 1] - bptr[a]);
+    const int kself = block_of(a, a);
+    if (kself < 0) return false;
+    int kq[4][4], pj[4];
+    std::vector<std::array<int, 2>> contrib(size_t(nb) * 2, {-1, -1});  // up to two (lane, q) per block
+    std::vector<int> ccount(nb, 0);
+    for (uint32_t j = 0; j < d; ++j) {
+      const uint32_t e = el[j] & 0x3FFFFFFFu;
+      pj[j] = int(el[j] >> 30);
+      for (int q = 0; q < 4; ++q) {
+        const int k = block_of(a, S.conn[4 * size_t(e) + q]);
+        if (k < 0) return false;
+        kq[j][q] = k;
+        if (q == pj[j]) continue;
+        if (ccount[k] == 2) return false;  // more than two elements share the pair
+        contrib[2 * size_t(k) + ccount[k]++] = {int(j), q};
+      }
+    }
+    for (int k = 0; k < nb; ++k)
+      if (k != kself && !ccount[k]) return false;
+    // shared blocks as edges between lanes; orient every path and cycle so that
+    // each lane exports at most one block and completes at most one
+    struct Edge { int u, v, k; };
+    std::vector<Edge> edges;
+    int ldeg[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nb; ++k)
+      if (k != kself && ccount[k] == 2) {
+        edges.push_back({contrib[2 * k][0], contrib[2 * k + 1][0], k});
+        ldeg[contrib[2 * k][0]]++;
+        ldeg[contrib[2 * k + 1][0]]++;
+      }
+    for (int j = 0; j < 4; ++j)
+      if (ldeg[j] > 2) return false;
+    int own_q[4] = {-1, -1, -1, -1}, prov_q[4] = {-1, -1, -1, -1}, src[4] = {0, 0, 0, 0};
+    std::vector<char> used(edges.size(), 0);
+    auto q_of = [&](int k, int j) { return contrib[2 * k][0] == j ? contrib[2 * k][1] : contrib[2 * k + 1][1]; };
+    auto walk = [&](int v) -> bool {
+      for (;;) {
+        int i = -1;
+        for (size_t x = 0; x < edges.size() && i < 0; ++x)
+          if (!used[x] && (edges[x].u == v || edges[x].v == v)) i = int(x);
+        if (i < 0) return true;
+        used[i] = 1;
+        const int w2 = edges[i].u == v ? edges[i].v : edges[i].u;
+        if (own_q[v] >= 0 || prov_q[w2] >= 0) return false;
+        own_q[v] = q_of(edges[i].k, v);
+        src[v] = w2;
+        prov_q[w2] = q_of(edges[i].k, w2);
+        v = w2;
+      }
+    };
+    for (int j = 0; j < 4; ++j)
+      if (ldeg[j] == 1 && !walk(j)) return false;
+    for (int j = 0; j < 4; ++j)
+      if (!walk(j)) return false;
+    for (uint32_t j = 0; j < 4; ++j) {
+      uint32_t* o = w + 4 * j;
+      if (j >= d) {
+        o[0] = 0; o[1] = 127u; o[2] = 0; o[3] = 0;
+        continue;
+      }
+      const int p = pj[j];
+      int keep[3], nk = 0;
+      for (int q = 0; q < 4; ++q)
+        if (q != p && q != own_q[j] && q != prov_q[j]) keep[nk++] = q;
+      int qs[4] = {p, 0, 0, 0}, u = 0;
+      qs[1] = prov_q[j] >= 0 ? prov_q[j] : keep[u++];
+      qs[2] = own_q[j] >= 0 ? own_q[j] : keep[u++];
+      qs[3] = keep[u++];
+      if (u != nk) return false;
+      const uint32_t e = el[j] & 0x3FFFFFFFu;
+      o[0] = uint32_t(row_ptr[6 * size_t(a)]);
+      o[1] = uint32_t(elem_local[e]) | uint32_t(p) << 7 | uint32_t(qs[1]) << 9 | uint32_t(qs[2]) << 11 |
+             uint32_t(qs[3]) << 13 | uint32_t(nb) << 15 | (j == 0 ? d : 0u) << 20 | uint32_t(j == 0) << 23 |
+             uint32_t(prov_q[j] < 0) << 24 | uint32_t(own_q[j] >= 0) << 25 | uint32_t(src[j]) << 26;
+      o[2] = uint32_t(kq[j][qs[0]]) | uint32_t(kq[j][qs[1]]) << 5 | uint32_t(kq[j][qs[2]]) << 10 |
+             uint32_t(kq[j][qs[3]]) << 15;
+      o[3] = 0;
+    }
+    return true;
+  };
+  auto flush = [&]() -> bool {
+    if (cur.empty()) return true;
+    std::sort(cur.begin(), cur.end());
+    std::sort(cur_elems.begin(), cur_elems.end());  // insertion order
+    for (size_t i = 0; i < cur_elems.size(); ++i) elem_local[cur_elems[i]] = int32_t(i);
+    const uint32_t elem_off = uint32_t(pm.peid.size());
+    const size_t lane0 = pm.lane.size();
+    pm.lane.resize(lane0 + 4 * size_t(4 * femdev::PC_NODES), 0);
+    for (size_t l = 0; l < size_t(4 * femdev::PC_NODES); ++l) pm.lane[lane0 + 4 * l + 1] = 127u;
+    for (size_t n = 0; n < cur.size(); ++n)
+      if (!node_lanes(cur[n], &pm.lane[lane0 + 16 * n])) return false;
+    for (uint32_t e : cur_elems) {
+      pm.peid.push_back(e);
+      for (int q = 0; q < 4; ++q) pm.pconn.push_back(S.conn[4 * size_t(e) + q]);
+      elem_local[e] = -1;
+    }
+    pm.hdr.insert(pm.hdr.end(), {elem_off, uint32_t(cur_elems.size())});
+    pm.n_patches++;
+    cur.clear();
+    cur_elems.clear();
+    return true;
+  };
+  for (auto& o : order) {
+    const uint32_t a = o.second;
+    uint32_t new_elems = 0;
+    for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_patch[nel[nptr[a] + r] & 0x3FFFFFFFu];
+    if (cur.size() == size_t(femdev::PC_NODES) || cur_elems.size() + new_elems > size_t(femdev::PC_MAX_ELEMS)) {
+      for (uint32_t e : cur_elems) in_patch[e] = 0;
+      if (!flush()) return false;
+    }
+    cur.push_back(a);
     for (uint32_t r = 0; r < deg[a]; ++r) {
       const uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
       if (!in_patch[e]) { in_patch[e] = 1; cur_elems.push_back(e); }
@@ -762,8 +921,13 @@ std::string build_pattern(fem_model* m) {
   {
     femdev::PlateMaps pm;
     if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
+      femdev::PlateCornerMaps pcm;
       femdev::PlateBlockMaps pbm;
-      if (m->pf_kernel == 1 && build_plate_block_maps(m, bptr, blocks, row_ptr, pbm)) {
+      if (m->pf_kernel == 2 && build_plate_corner_maps(m, bptr, blocks, row_ptr, pcm)) {
+        e = femdev::upload_plate_corner_maps(m->dev, pcm, m->pf_wg_per_cu < 0 ? 0 : m->pf_wg_per_cu);
+        if (!e.empty()) return e;
+        m->path = "plate_corners";
+      } else if (m->pf_kernel >= 1 && build_plate_block_maps(m, bptr, blocks, row_ptr, pbm)) {
         e = femdev::upload_plate_block_maps(m->dev, pbm, m->pf_wg_per_cu < 0 ? 0 : m->pf_wg_per_cu);
         if (!e.empty()) return e;
         m->path = "plate_blocks";
@@ -1094,6 +1258,7 @@ int batch_ready(fem_batch* b) {
 std::string launch_values(fem_model* g, double* val, void* stream, bool timed) {
   return g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, val, stream, timed)
          : g->path == "plate_blocks" ? femdev::assemble_plate_blocks(g->dev, val, stream, timed)
+         : g->path == "plate_corners" ? femdev::assemble_plate_corners(g->dev, val, stream, timed)
          : g->path == "bar_owner" ? femdev::assemble_bar_owner(g->dev, val, stream, timed)
                                   : femdev::assemble_generic(g->dev, val, stream, timed);
 }
@@ -1512,6 +1677,7 @@ int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
   if (rc) return rc;
   std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
                   : m->path == "plate_blocks" ? femdev::assemble_plate_blocks(m->dev, val_device, stream)
+                  : m->path == "plate_corners" ? femdev::assemble_plate_corners(m->dev, val_device, stream)
                   : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream)
                                            : femdev::assemble_generic(m->dev, val_device, stream);
   if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
@@ -1838,7 +2004,7 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
   } else if (k == "force_generic") {
     m->force_generic = value != 0;
   } else if (k == "plate_kernel") {
-    if (value != 0 && value != 1) return fail(m, FEM_ERR_ARGUMENT, "plate_kernel must be 0 or 1");
+    if (value < 0 || value > 2) return fail(m, FEM_ERR_ARGUMENT, "plate_kernel must be 0, 1 or 2");
     m->pf_kernel = int(value);
   } else if (k == "separate_lanes") {
     if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
@@ -1959,7 +2125,7 @@ int fem_batch_autotune(fem_batch* b) {
       const int patch_opts[2] = {32, 64};
       for (int pn : patch_opts) {
         // patch_nodes shapes only the row-staging plate kernel
-        if ((!plate_only || g->pf_kernel == 1) && pn != 64) continue;
+        if ((!plate_only || g->pf_kernel >= 1) && pn != 64) continue;
         if (g->pf_nodes != pn) {
           g->pf_nodes = pn;
           g->pattern_valid = false;
@@ -1976,7 +2142,7 @@ int fem_batch_autotune(fem_batch* b) {
             if (rc) return rc;
             if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
           }
-        } else if (g->path == "plate_blocks") {
+        } else if (g->path == "plate_blocks" || g->path == "plate_corners") {
           for (int wg : {0, 4}) {
             auto e = femdev::set_plate_block_grid(g->dev, wg);
             if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
@@ -2021,10 +2187,10 @@ int fem_batch_autotune(fem_batch* b) {
       if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
       b->config += "group " + std::to_string(k) + ": plate_fused patch_nodes=" + std::to_string(best[k][0]) +
                    " wg_per_cu=" + std::to_string(best[k][1]) + "; ";
-    } else if (g->path == "plate_blocks") {
+    } else if (g->path == "plate_blocks" || g->path == "plate_corners") {
       auto e = femdev::set_plate_block_grid(g->dev, best[k][1]);
       if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
-      b->config += "group " + std::to_string(k) + ": plate_blocks wg_per_cu=" + std::to_string(best[k][1]) + "; ";
+      b->config += "group " + std::to_string(k) + ": " + g->path + " wg_per_cu=" + std::to_string(best[k][1]) + "; ";
     } else {
       if (g->path == "generic") femdev::set_generic_block(g->dev, best[k][2]);
       b->config += "group " + std::to_string(k) + ": " + g->path +

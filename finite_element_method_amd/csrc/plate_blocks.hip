@@ -9,7 +9,7 @@
 //   step 2: one lane per owned node-pair block (a, b) walks the elements that
 //           couple a and b in insertion order (plate.rs:513-724 folded over
 //           the four points), sums their 13-entry blocks and stores them into
-//           the CSR rows of a.  No atomics and no staging: every value is
+//           the CSR rows of a (8 wide stores per block, plate_layout.hpp).  No atomics and no staging: every value is
 //           written once, by the lane that summed it.
 // Every element frame is exactly the identity (checked when the maps are
 // built); the drilling entry of an owned node is the sum of KROT6 over its
@@ -21,6 +21,7 @@
 #include "device_state.hpp"
 #include "element_math.hpp"
 #include "fem_device.hpp"
+#include "plate_layout.hpp"
 
 namespace femdev {
 
@@ -145,12 +146,12 @@ __global__ __launch_bounds__(PB_THREADS, 4) void plate_blocks_kernel(BlockArgs A
           const double2 dp = gi[1 + 2 * p], ap = gi[2 + 2 * p];
           const double2 dq = gi[1 + 2 * q], aq = gi[2 + 2 * q];
           const double wdx = w * dp.x, wdy = w * dp.y, wal = w * ap.x, wbe = w * ap.y;
-          Mxx += wdx * dq.x;
-          Mxy += wdx * dq.y;
-          Myx += wdy * dq.x;
-          Myy += wdy * dq.y;
-          Aa += wal * aq.x;
-          Bb += wbe * aq.y;
+          Mxx = __builtin_fma(wdx, dq.x, Mxx);
+          Mxy = __builtin_fma(wdx, dq.y, Mxy);
+          Myx = __builtin_fma(wdy, dq.x, Myx);
+          Myy = __builtin_fma(wdy, dq.y, Myy);
+          Aa = __builtin_fma(wal, aq.x, Aa);
+          Bb = __builtin_fma(wbe, aq.y, Bb);
         }
         const double2* ed = L.edat + cle * (PB_EDAT / 2);
         const double2 e0 = ed[0], e1 = ed[1], e2 = ed[2], e3 = ed[3];
@@ -167,15 +168,8 @@ __global__ __launch_bounds__(PB_THREADS, 4) void plate_blocks_kernel(BlockArgs A
 #pragma unroll
         for (int i = 0; i < 13; ++i) bs[i] = (m == 0) ? o[i] : bs[i] + o[i];
       }
-      const int pos[13] = {2 * k, 2 * k + 1, 2 * nb + 2 * k, 2 * nb + 2 * k + 1,
-                           4 * nb + 3 * k, 4 * nb + 3 * k + 1, 4 * nb + 3 * k + 2,
-                           7 * nb + 3 * k, 7 * nb + 3 * k + 1, 7 * nb + 3 * k + 2,
-                           10 * nb + 3 * k, 10 * nb + 3 * k + 1, 10 * nb + 3 * k + 2};
       const uint32_t base = g0 * 8u;  // 32-bit byte offsets (checked when the maps are built)
-#pragma unroll
-      for (int i = 0; i < 13; ++i)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, bs[i]), rsrc, base + 8u * uint32_t(pos[i]), 0,
-                                              0);
+      plate_store_block(rsrc, base, k, nb, bs);
       if (self) {
         // drilling row: KROT6 of every element of the node, in insertion order
         double dr = fem::KROT6;

@@ -209,12 +209,12 @@ __device__ __forceinline__ bool pf_process(PatchRegs& cur, PatchRegs& nxt, int p
         const double dxq = h ? gv[3 + c] : gv[1 + c], dyq = h ? gv[7 + c] : gv[5 + c];
         const double alq = h ? gv[11 + c] : gv[9 + c], beq = h ? gv[15 + c] : gv[13 + c];
         (void)q;
-        acc[6 * c + 0] += wdx * dxq;
-        acc[6 * c + 1] += wdx * dyq;
-        acc[6 * c + 2] += wdy * dxq;
-        acc[6 * c + 3] += wdy * dyq;
-        acc[6 * c + 4] += wal * alq;
-        acc[6 * c + 5] += wbe * beq;
+        acc[6 * c + 0] = __builtin_fma(wdx, dxq, acc[6 * c + 0]);
+        acc[6 * c + 1] = __builtin_fma(wdx, dyq, acc[6 * c + 1]);
+        acc[6 * c + 2] = __builtin_fma(wdy, dxq, acc[6 * c + 2]);
+        acc[6 * c + 3] = __builtin_fma(wdy, dyq, acc[6 * c + 3]);
+        acc[6 * c + 4] = __builtin_fma(wal, alq, acc[6 * c + 4]);
+        acc[6 * c + 5] = __builtin_fma(wbe, beq, acc[6 * c + 5]);
       }
     }
     if (h == 0) L.stage_p[row] = p;

@@ -284,7 +284,9 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
 /* Tuning options (no counterpart in the crate; they change speed, never values):
  *   "plate_kernel"       fused kernel of flat plate meshes: 1 = one lane per
  *                        owned node-pair block (plate_blocks, default),
- *                        0 = row staging (plate_fused); bitwise equal values
+ *                        0 = row staging (plate_fused), 2 = one lane per
+ *                        element corner (plate_corners; meshes whose nodes
+ *                        have at most four elements); bitwise equal values
  *   "plate_patch_nodes"  owned nodes per workgroup of the row-staging kernel: 4..64
  *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernels,
  *                        looping over the patches: 0..64 (0 = one workgroup
@@ -297,7 +299,7 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
 int fem_set_option(fem_model* m, const char* key, int64_t value);
 
 /* Name of the value path of the last assembly: "plate_blocks", "plate_fused",
- * "bar_owner" or "generic". */
+ * "plate_corners", "bar_owner" or "generic". */
 const char* fem_assembly_path(fem_model* m);
 
 #ifdef __cplusplus

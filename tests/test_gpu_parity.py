@@ -87,17 +87,19 @@ def test_rotation_getters(gpu):
 
 @pytest.mark.parametrize("name", ["plate_8", "plate_17x9", "plate_64_shuffled"])
 def test_plate_kernels_agree(gpu, name):
-    """The two fused plate kernels (row staging, block lanes) give bitwise
-    identical values, and both match the oracle."""
+    """The three fused plate kernels (row staging, block lanes, corner lanes)
+    give bitwise identical values, and all match the oracle."""
     mesh = CASES[name]()
     ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)
     out = {}
-    for kern, path in ((0, "plate_fused"), (1, "plate_blocks")):
+    for kern, path in ((0, "plate_fused"), (1, "plate_blocks"), (2, "plate_corners")):
         m = FEM.from_mesh(mesh)
         m.set_option("plate_kernel", kern)
         out[kern] = m.stiffness_csr()
+        print(name, "kernel", kern, m.assembly_path(), flush=True)
         assert m.assembly_path() == path
         r = compare(out[kern], ref)
         assert r["pattern_equal"] and r["max_ratio"] <= 1.0, (kern, r)
-    for a, b in zip(out[0], out[1]):
-        assert np.array_equal(a, b)
+    for k in (1, 2):
+        for a, b in zip(out[0], out[k]):
+            assert np.array_equal(a, b), k
