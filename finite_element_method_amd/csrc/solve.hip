@@ -31,8 +31,11 @@ constexpr int SV_PARTS = 1024;    // blocks of the reductions (fixed: reproducib
 constexpr int SPMV_LANES = 8;     // lanes per CSR row
 constexpr int BJ_MAX = 6;         // dofs per node block
 
-// scalar slots (device): 0 r.z, 1 p.Ap, 2 alpha, 3 beta, 4 r.r, 5 (r.z)_new, 6 b.b, 7 status
-enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_RZN, S_BB, S_STATUS, S_COUNT };
+// scalar slots (device): 0 r.z, 1 p.Ap, 2 alpha, 3 beta, 4 r.r, 5 (r.z)_new, 6 b.b, 7 status,
+// 8 updates done, 9 stop (converged or capped: every later kernel of the loop
+// returns at once), 10 converged
+enum { S_RZ = 0, S_PAP, S_ALPHA, S_BETA, S_RR, S_RZN, S_BB, S_STATUS, S_ITER, S_DONE, S_CONV, S_COUNT };
+constexpr int BJ_PER_WAVE = 10;   // node blocks per wave in the block-Jacobi kernels (6 lanes each)
 
 template <class T>
 std::string dgrow(T*& p, size_t& cap, size_t n) {
@@ -62,6 +65,21 @@ __global__ void sv_spmv(const int64_t* __restrict__ rp, const int32_t* __restric
   }
 }
 
+// Ap = K p inside the update loop (returns at once after the stop flag)
+__global__ void sv_spmv_it(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                           const double* __restrict__ v, const double* __restrict__ p, double* __restrict__ ap,
+                           int64_t n, const double* s) {
+  if (s[S_DONE] != 0.0) return;
+  const int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SPMV_LANES;
+  const int lig = threadIdx.x & (SPMV_LANES - 1);
+  if (r >= n) return;
+  double acc = 0.0;
+  for (int64_t e = rp[r] + lig; e < rp[r + 1]; e += SPMV_LANES) acc = __builtin_fma(v[e], p[ci[e]], acc);
+#pragma unroll
+  for (int o = SPMV_LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, SPMV_LANES);
+  if (lig == 0) ap[r] = acc;
+}
+
 // block tree sum of per-thread values into part[blockIdx.x]
 __device__ inline void block_sum2(double a, double b, double* part_a, double* part_b) {
   __shared__ double sa[SV_B], sb[SV_B];
@@ -81,21 +99,22 @@ __device__ inline void block_sum2(double a, double b, double* part_a, double* pa
   }
 }
 
-// partials of (x, y) and optionally (u, w); grid SV_PARTS x SV_B, strided
-__global__ void sv_dot2(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ u,
-                        const double* __restrict__ w, int64_t n, double* part_a, double* part_b) {
-  double a = 0.0, b = 0.0;
-  for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B) {
-    a += x[i] * y[i];
-    if (u) b += u[i] * w[i];
-  }
-  block_sum2(a, b, part_a, u ? part_b : nullptr);
+// partials of p.Ap; grid SV_PARTS x SV_B, strided
+__global__ void sv_dot_it(const double* __restrict__ x, const double* __restrict__ y, int64_t n, const double* s,
+                          double* part) {
+  if (s[S_DONE] != 0.0) return;
+  double a = 0.0;
+  for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B)
+    a = __builtin_fma(x[i], y[i], a);
+  block_sum2(a, 0.0, part, nullptr);
 }
 
 // final sums of the partials into scalars, then the PCG scalar step:
 // mode 0: s[S_RZ] = a, s[S_BB] = b (start);  mode 1: s[S_PAP] = a, alpha = rz / pAp;
 // mode 2: s[S_RZN] = a, s[S_RR] = b, beta = rzn / rz, rz = rzn
-__global__ void sv_final(const double* __restrict__ part_a, const double* __restrict__ part_b, double* s, int mode) {
+__global__ void sv_final(const double* __restrict__ part_a, const double* __restrict__ part_b, double* s, int mode,
+                         double stop, double max_iter) {
+  if (mode != 0 && s[S_DONE] != 0.0) return;
   __shared__ double sa[SV_PARTS], sb[SV_PARTS];
   for (int i = threadIdx.x; i < SV_PARTS; i += blockDim.x) {
     sa[i] = part_a[i];
@@ -122,6 +141,11 @@ __global__ void sv_final(const double* __restrict__ part_a, const double* __rest
     s[S_RR] = sb[0];
     s[S_BETA] = sa[0] / s[S_RZ];
     s[S_RZ] = sa[0];
+    const double it = s[S_ITER] + 1.0;
+    const bool conv = sqrt(sb[0]) <= stop;
+    s[S_ITER] = it;
+    s[S_CONV] = conv ? 1.0 : 0.0;
+    s[S_DONE] = (conv || it >= max_iter) ? 1.0 : 0.0;
   }
 }
 
@@ -217,11 +241,12 @@ __global__ void sv_update_jacobi(const double* __restrict__ d, const double* __r
                                  const double* __restrict__ ap, const double* __restrict__ s, int64_t n,
                                  double* __restrict__ x, double* __restrict__ r, double* __restrict__ z, double* part_a,
                                  double* part_b) {
+  if (s[S_DONE] != 0.0) return;
   const double alpha = s[S_ALPHA];
   double ra = 0.0, rb = 0.0;
   for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * ap[i];
+    x[i] = __builtin_fma(alpha, p[i], x[i]);
+    const double ri = __builtin_fma(-alpha, ap[i], r[i]);
     r[i] = ri;
     const double zi = ri / d[i];
     z[i] = zi;
@@ -231,57 +256,98 @@ __global__ void sv_update_jacobi(const double* __restrict__ d, const double* __r
   block_sum2(ra, rb, part_a, part_b);
 }
 
-// block Jacobi variants: one thread per node block
-__device__ inline void bj_apply(const double* __restrict__ inv, int64_t b, int m, const double* rr, double* zz) {
-  const double* o = inv + 36 * b;
-  for (int i = 0; i < m; ++i) {
-    double t = 0.0;
-    for (int j = 0; j < m; ++j) t += o[6 * i + j] * rr[j];
-    zz[i] = t;
+// block Jacobi: six lanes per node block (lane = row of the block), ten
+// blocks per wave; z = M^-1 r with the row of the inverse read as three
+// 16-byte loads and the block's r gathered by lane shuffles.  Lanes beyond
+// a block's size, and the last four lanes of each wave, only take part in the
+// shuffles.
+struct BjLane {
+  int64_t k;  // node block
+  int row, s0, m;
+  bool act;   // row < m of an existing block
+};
+__device__ inline BjLane bj_lane(const int32_t* __restrict__ starts, int64_t nblk, int64_t wave, int lane) {
+  BjLane o;
+  o.k = wave * BJ_PER_WAVE + lane / 6;
+  o.row = lane % 6;
+  o.s0 = 0;
+  o.m = 0;
+  if (lane < 6 * BJ_PER_WAVE && o.k < nblk) {
+    o.s0 = starts[o.k];
+    o.m = starts[o.k + 1] - o.s0;
   }
+  o.act = o.row < o.m;
+  return o;
+}
+__device__ inline double bj_row_apply(const double* __restrict__ inv, const BjLane& L, int lane, double ri) {
+  double2 w0 = make_double2(0.0, 0.0), w1 = w0, w2 = w0;
+  if (L.act) {
+    const double2* row = reinterpret_cast<const double2*>(inv + 36 * L.k + 6 * L.row);
+    w0 = row[0];
+    w1 = row[1];
+    w2 = row[2];
+  }
+  const double w[6] = {w0.x, w0.y, w1.x, w1.y, w2.x, w2.y};
+  double zi = 0.0;
+  const int first = lane - L.row;
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    const double rj = __shfl(ri, (first + j) & 63);  // every lane takes part
+    if (j < L.m) zi = __builtin_fma(w[j], rj, zi);
+  }
+  return zi;
 }
 __global__ void sv_start_block(const double* __restrict__ b, const double* __restrict__ inv,
                                const int32_t* __restrict__ starts, int64_t nblk, double* __restrict__ x,
                                double* __restrict__ r, double* __restrict__ z, double* __restrict__ p, double* part_a,
                                double* part_b) {
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = int64_t(SV_PARTS) * (SV_B / 64);
   double ra = 0.0, rb = 0.0;
-  for (int64_t k = int64_t(blockIdx.x) * SV_B + threadIdx.x; k < nblk; k += int64_t(SV_PARTS) * SV_B) {
-    const int s0 = starts[k], m = starts[k + 1] - s0;
-    double rr[BJ_MAX], zz[BJ_MAX];
-    for (int i = 0; i < m; ++i) {
-      rr[i] = b[s0 + i];
-      x[s0 + i] = 0.0;
-      r[s0 + i] = rr[i];
+  for (int64_t w = int64_t(blockIdx.x) * (SV_B / 64) + (threadIdx.x >> 6); w * BJ_PER_WAVE < nblk; w += waves) {
+    const BjLane L = bj_lane(starts, nblk, w, lane);
+    double ri = 0.0;
+    if (L.act) {
+      const int64_t i = L.s0 + L.row;
+      ri = b[i];
+      x[i] = 0.0;
+      r[i] = ri;
     }
-    bj_apply(inv, k, m, rr, zz);
-    for (int i = 0; i < m; ++i) {
-      z[s0 + i] = zz[i];
-      p[s0 + i] = zz[i];
-      ra += rr[i] * zz[i];
-      rb += rr[i] * rr[i];
+    const double zi = bj_row_apply(inv, L, lane, ri);
+    if (L.act) {
+      const int64_t i = L.s0 + L.row;
+      z[i] = zi;
+      p[i] = zi;
+      ra = __builtin_fma(ri, zi, ra);
+      rb = __builtin_fma(ri, ri, rb);
     }
   }
   block_sum2(ra, rb, part_a, part_b);
 }
+// update: x += alpha p, r -= alpha Ap, z = M^-1 r; partials of (r, z) and (r, r)
 __global__ void sv_update_block(const double* __restrict__ inv, const int32_t* __restrict__ starts, int64_t nblk,
                                 const double* __restrict__ p, const double* __restrict__ ap,
                                 const double* __restrict__ s, double* __restrict__ x, double* __restrict__ r,
                                 double* __restrict__ z, double* part_a, double* part_b) {
+  if (s[S_DONE] != 0.0) return;
   const double alpha = s[S_ALPHA];
+  const int lane = threadIdx.x & 63;
+  const int64_t waves = int64_t(SV_PARTS) * (SV_B / 64);
   double ra = 0.0, rb = 0.0;
-  for (int64_t k = int64_t(blockIdx.x) * SV_B + threadIdx.x; k < nblk; k += int64_t(SV_PARTS) * SV_B) {
-    const int s0 = starts[k], m = starts[k + 1] - s0;
-    double rr[BJ_MAX], zz[BJ_MAX];
-    for (int i = 0; i < m; ++i) {
-      x[s0 + i] += alpha * p[s0 + i];
-      rr[i] = r[s0 + i] - alpha * ap[s0 + i];
-      r[s0 + i] = rr[i];
+  for (int64_t w = int64_t(blockIdx.x) * (SV_B / 64) + (threadIdx.x >> 6); w * BJ_PER_WAVE < nblk; w += waves) {
+    const BjLane L = bj_lane(starts, nblk, w, lane);
+    double ri = 0.0;
+    if (L.act) {
+      const int64_t i = L.s0 + L.row;
+      x[i] = __builtin_fma(alpha, p[i], x[i]);
+      ri = __builtin_fma(-alpha, ap[i], r[i]);
+      r[i] = ri;
     }
-    bj_apply(inv, k, m, rr, zz);
-    for (int i = 0; i < m; ++i) {
-      z[s0 + i] = zz[i];
-      ra += rr[i] * zz[i];
-      rb += rr[i] * rr[i];
+    const double zi = bj_row_apply(inv, L, lane, ri);
+    if (L.act) {
+      z[L.s0 + L.row] = zi;
+      ra = __builtin_fma(ri, zi, ra);
+      rb = __builtin_fma(ri, ri, rb);
     }
   }
   block_sum2(ra, rb, part_a, part_b);
@@ -290,9 +356,10 @@ __global__ void sv_update_block(const double* __restrict__ inv, const int32_t* _
 // p = z + beta p
 __global__ void sv_direction(const double* __restrict__ z, const double* __restrict__ s, int64_t n,
                              double* __restrict__ p) {
+  if (s[S_DONE] != 0.0) return;
   const double beta = s[S_BETA];
   for (int64_t i = int64_t(blockIdx.x) * SV_B + threadIdx.x; i < n; i += int64_t(SV_PARTS) * SV_B)
-    p[i] = z[i] + beta * p[i];
+    p[i] = __builtin_fma(beta, p[i], z[i]);
 }
 
 // r_b: forces at the k_bb indexes
@@ -321,7 +388,7 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
   if (!(e = dgrow(d->sol_rb, d->sol_cap[7], unb)).empty()) return e;
   if (!(e = dgrow(d->sol_part, d->sol_cap[8], size_t(2 * SV_PARTS))).empty()) return e;
   if (!(e = dgrow(d->sol_s, d->sol_cap[9], size_t(S_COUNT))).empty()) return e;
-  if (!d->sol_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&d->sol_host), 8 * sizeof(double)));
+  if (!d->sol_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&d->sol_host), 16 * sizeof(double)));
   double *b = d->sol_vec[0], *x = d->sol_vec[1], *r = d->sol_vec[2], *z = d->sol_vec[3], *p = d->sol_vec[4],
          *ap = d->sol_vec[5];
   double *pa = d->sol_part, *pb = d->sol_part + SV_PARTS;
@@ -342,7 +409,7 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
   // preconditioner
   int64_t nblk = 0;
   double* zero_s = d->sol_host;
-  for (int k = 0; k < S_COUNT && k < 8; ++k) zero_s[k] = 0.0;
+  for (int k = 0; k < S_COUNT; ++k) zero_s[k] = 0.0;
   HIP_TRY(hipMemcpyAsync(d->sol_s, zero_s, sizeof(double) * S_COUNT, hipMemcpyHostToDevice, s));
   if (precond == 0) {
     if (!(e = dgrow(d->sol_diag, d->sol_cap[10], una)).empty()) return e;
@@ -379,7 +446,7 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
   else
     sv_start_block<<<SV_PARTS, B, 0, s>>>(b, d->sol_inv, d->sol_starts, nblk, x, r, z, p, pa, pb);
   HIP_TRY(hipGetLastError());
-  sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 0);
+  sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 0, 0.0, 0.0);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_s, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
@@ -387,24 +454,35 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
   const double stop = std::max(rel_tol * std::sqrt(d->sol_host[S_BB]), abs_tol);
   int it = 0;
   bool conv = std::sqrt(d->sol_host[S_RR]) <= stop;
-  while (!conv && it < max_iter) {
-    ++it;
-    sv_spmv<<<grid_for(na * SPMV_LANES, B), B, 0, s>>>(rp[0], ci[0], v[0], p, nullptr, 1.0, nullptr, ap, na);
-    sv_dot2<<<SV_PARTS, B, 0, s>>>(p, ap, nullptr, nullptr, na, pa, pb);
-    sv_final<<<1, 1024, 0, s>>>(pa, nullptr, d->sol_s, 1);
-    if (precond == 0)
-      sv_update_jacobi<<<SV_PARTS, B, 0, s>>>(d->sol_diag, p, ap, d->sol_s, na, x, r, z, pa, pb);
-    else
-      sv_update_block<<<SV_PARTS, B, 0, s>>>(d->sol_inv, d->sol_starts, nblk, p, ap, d->sol_s, x, r, z, pa, pb);
-    sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 2);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipMemcpyAsync(d->sol_host + S_RR, d->sol_s + S_RR, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    conv = std::sqrt(d->sol_host[S_RR]) <= stop;
-    if (!conv) {
-      sv_direction<<<SV_PARTS, B, 0, s>>>(z, d->sol_s, na, p);
+  if (!conv && max_iter > 0) {
+    // Updates are queued in growing batches; the last scalar kernel of an
+    // update raises the stop flag on convergence or at max_iter, and every
+    // kernel queued after that returns at once.  The count stays exact and the
+    // host waits once per batch, not once per update.
+    const unsigned spmv_blocks = grid_for(na * SPMV_LANES, B);
+    int queued = 0, batch = 1;
+    for (;;) {
+      const int nq = std::min(batch, max_iter - queued);
+      for (int q = 0; q < nq; ++q) {
+        sv_spmv_it<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, d->sol_s);
+        sv_dot_it<<<SV_PARTS, B, 0, s>>>(p, ap, na, d->sol_s, pa);
+        sv_final<<<1, 1024, 0, s>>>(pa, nullptr, d->sol_s, 1, stop, double(max_iter));
+        if (precond == 0)
+          sv_update_jacobi<<<SV_PARTS, B, 0, s>>>(d->sol_diag, p, ap, d->sol_s, na, x, r, z, pa, pb);
+        else
+          sv_update_block<<<SV_PARTS, B, 0, s>>>(d->sol_inv, d->sol_starts, nblk, p, ap, d->sol_s, x, r, z, pa, pb);
+        sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 2, stop, double(max_iter));
+        sv_direction<<<SV_PARTS, B, 0, s>>>(z, d->sol_s, na, p);
+      }
       HIP_TRY(hipGetLastError());
+      queued += nq;
+      HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_s, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipStreamSynchronize(s));
+      if (d->sol_host[S_DONE] != 0.0 || queued >= max_iter) break;
+      batch = std::min(batch * 2, 64);
     }
+    it = int(d->sol_host[S_ITER]);
+    conv = d->sol_host[S_CONV] != 0.0;
   }
   // r_r = (K_ba u_a + K_bb u_b) - r_b
   if (nb) {

@@ -4,6 +4,7 @@ restatement: b = r_a - K_ab u_b, PCG (Jacobi / node-block Jacobi), r_r."""
 import numpy as np
 import pytest
 import scipy.sparse as sp
+import scipy.sparse.linalg as spla
 
 from finite_element_method_amd import FEM, DOFParameter, meshgen
 from oracle.oracle import OracleModel, block_starts, find_b_sparse, find_r_r_sparse, pcg
@@ -54,8 +55,13 @@ CASES = {
 def test_pcg_matches_oracle(gpu, name, pre):
     """On the GPU's own separated matrices: the residual meets the stopping
     rule, the iteration count matches the restatement's (within 2: the sums
-    round differently), u_a agrees to 1e-6 relative, and r_r follows from u_a
-    as find_r_r_sparse says (1e-12)."""
+    round differently), u_a is as accurate as the restatement's PCG (its
+    distance to the direct solution at most twice the restatement's, plus
+    rounding) and close to it, and r_r follows from u_a as find_r_r_sparse
+    says (1e-12).  Two runs of PCG that sum their dot products in different
+    orders drift apart over hundreds of updates by up to about the stopping
+    tolerance times the conditioning, so the two approximate solutions are
+    compared through their accuracy, not entry by entry."""
     mesh = CASES[name]()
     om = meshgen.load_oracle(mesh, OracleModel)
     diag = bc_cases.diagonal(*om.csr())
@@ -72,7 +78,10 @@ def test_pcg_matches_oracle(gpu, name, pre):
     assert np.linalg.norm(b - K @ ua) <= 1.01 * max(m.rel_tol * np.linalg.norm(b), m.abs_tol)
     x, it_ref, ok = pcg(mats["aa"], b, 20000, m.rel_tol, m.abs_tol, block_starts(aa) if pre == "block_jacobi" else None)
     assert ok and abs(it - it_ref) <= 2, (it, it_ref)
-    assert np.linalg.norm(ua - x) <= 1e-6 * np.linalg.norm(x)
+    exact = spla.spsolve(K.tocsc(), b)
+    err, err_ref = np.linalg.norm(ua - exact), np.linalg.norm(x - exact)
+    assert err <= 2.0 * err_ref + 1e-12 * np.linalg.norm(exact), (err, err_ref)
+    assert np.linalg.norm(ua - x) <= 1e-4 * np.linalg.norm(x)
     r_b = np.zeros(len(bb))
     idx = {int(v): i for i, v in enumerate(mesh.node_numbers)}
     for nn, d, val in zip(*loads):
