@@ -260,15 +260,21 @@ def cpu_baseline(kind):
     om = meshgen.load_oracle(sample, OracleModel)
     om.csr(structural=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    one, bad1 = om.time_values(1, 1)
-    par, badp = om.time_values(threads, 3)
+    one, bad1 = om.time_values(1, reps=1)
+    par, badp = om.time_values(threads, reps=5, warmup=1)
     if bad1 or badp:
         raise RuntimeError("CPU baseline values differ from the sequential export")
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": sample.n_elements / par, "unit": "elements/s", "cores": threads, "kind": "port",
-            "single_thread_value": sample.n_elements / one,
+            "single_thread_value": sample.n_elements / one, "cpu_model": model, "nproc": os.cpu_count(),
             "sample": desc + ", oracle value recompute (element k, T^T k T, insertion-order CSR sums), "
-                      "median of 3 on %d OpenMP threads (%.3f s), one run on one thread (%.3f s)"
-                      % (threads, par, one)}
+                      "median of 5 after 1 warm-up on %d OpenMP threads (%.3f s), one run on one thread "
+                      "(%.3f s)" % (threads, par, one)}
 
 
 def main():

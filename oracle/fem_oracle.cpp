@@ -1335,7 +1335,7 @@ std::vector<V> plate_forces(const Model<V>& m, const Plate<V>& e, const std::vec
 // result does not depend on the thread count.  Returns the median seconds of
 // `reps` runs and the number of values that differ from export_csr(mode 1).
 template <class V>
-double time_values(Model<V>& m, int threads, int reps, int64_t* mismatches) {
+double time_values(Model<V>& m, int threads, int warmup, int reps, int64_t* mismatches) {
   // contributions of each structural CSR value, in insertion order (as export_csr mode 1)
   std::vector<size_t> order;
   order.reserve(m.t_row.size());
@@ -1353,7 +1353,7 @@ double time_values(Model<V>& m, int threads, int reps, int64_t* mismatches) {
   std::vector<V> tv(m.t_val.size());
   std::vector<double> val(ns);
   std::vector<double> times;
-  for (int rep = 0; rep < reps; ++rep) {
+  for (int rep = -warmup; rep < reps; ++rep) {  // negative reps: untimed warm-up
     auto t0 = std::chrono::steady_clock::now();
 #pragma omp parallel for num_threads(threads) schedule(dynamic, 256)
     for (int64_t ei = 0; ei < int64_t(m.erange.size()); ++ei) {
@@ -1385,7 +1385,7 @@ double time_values(Model<V>& m, int threads, int reps, int64_t* mismatches) {
       val[size_t(s)] = double(acc);
     }
     auto t1 = std::chrono::steady_clock::now();
-    times.push_back(std::chrono::duration<double>(t1 - t0).count());
+    if (rep >= 0) times.push_back(std::chrono::duration<double>(t1 - t0).count());
   }
   std::sort(times.begin(), times.end());
   int64_t bad = 0;
@@ -1636,8 +1636,9 @@ int64_t om_element_forces(AnyModel* am, int type, const double* u, double* out) 
 
 // CPU baseline: median seconds of `reps` value recomputes on `threads` OpenMP
 // threads; *mismatches = values differing from the sequential export.
-double om_time_values(AnyModel* am, int threads, int reps, int64_t* mismatches) {
-  return am->f32 ? time_values(*am->f, threads, reps, mismatches) : time_values(*am->d, threads, reps, mismatches);
+double om_time_values(AnyModel* am, int threads, int warmup, int reps, int64_t* mismatches) {
+  return am->f32 ? time_values(*am->f, threads, warmup, reps, mismatches)
+                 : time_values(*am->d, threads, warmup, reps, mismatches);
 }
 
 // boundary conditions; dof 0..5 = X, Y, Z, ThX, ThY, ThZ.  Return 0 or 1 (message in om_last_error).

@@ -77,7 +77,7 @@ def lib():
         L.om_element_forces.restype = ctypes.c_int64
         L.om_element_forces.argtypes = [vp, ctypes.c_int, vp, vp]
         L.om_time_values.restype = ctypes.c_double
-        L.om_time_values.argtypes = [vp, ctypes.c_int, ctypes.c_int, vp]
+        L.om_time_values.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, vp]
         L.om_eval_element.restype = ctypes.c_int
         L.om_eval_element.argtypes = [ctypes.c_int, vp, vp, dbl, dbl, ctypes.c_uint, vp]
         _lib = L
@@ -406,12 +406,13 @@ class OracleModel:
         w = lib().om_element_forces(self._h, int(etype), _ptr(u), _ptr(out))
         return out[:w].reshape(-1, per)
 
-    def time_values(self, threads=1, reps=3):
-        """Median seconds of a host value recompute (element matrices on
-        `threads` OpenMP threads, insertion-order sums) and the number of
-        values that differ from the sequential CSR export (0 expected)."""
+    def time_values(self, threads=1, reps=3, warmup=0):
+        """Median seconds (steady clock) of `reps` host value recomputes
+        (element matrices on `threads` OpenMP threads, insertion-order sums)
+        after `warmup` untimed ones, and the number of values that differ
+        from the sequential CSR export (0 expected)."""
         bad = np.zeros(1, dtype=np.int64)
-        sec = lib().om_time_values(self._h, int(threads), int(reps), _ptr(bad))
+        sec = lib().om_time_values(self._h, int(threads), int(warmup), int(reps), _ptr(bad))
         return sec, int(bad[0])
 
     def structural_violations(self):
