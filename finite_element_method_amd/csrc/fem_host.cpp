@@ -140,6 +140,9 @@ struct fem_batch {
   std::vector<fem_model*> groups;
   std::vector<Mesh> meshes;
   void* stream = nullptr;
+  // group k > 0 is captured on side[k - 1], forked from and joined to `stream`
+  // by events, so the graph runs the groups as parallel branches
+  std::vector<void*> side, fork_ev, join_ev;
   femdev::Graph* graph = nullptr;
   femdev::Timer* timer = nullptr;
   bool ready = false, timed = false;
@@ -1267,9 +1270,35 @@ std::string batch_launch_groups_one(fem_model* g, void* stream) {
   return launch_values(g, femdev::values(g->dev), stream, false);
 }
 
-std::string batch_launch_groups(fem_batch* b, void* stream, bool timed) {
-  for (auto* g : b->groups) {
-    auto e = launch_values(g, femdev::values(g->dev), stream, timed);
+// the groups' launches as parallel branches of one captured graph
+std::string batch_launch_branches(fem_batch* b) {
+  const size_t ng = b->groups.size();
+  while (b->side.size() + 1 < ng) {
+    void *s = nullptr, *f = nullptr, *j = nullptr;
+    auto e = femdev::stream_create(s);
+    if (e.empty()) e = femdev::event_create(f);
+    if (e.empty()) e = femdev::event_create(j);
+    if (!e.empty()) {
+      femdev::stream_destroy(s);
+      femdev::event_destroy(f);
+      femdev::event_destroy(j);
+      return e;
+    }
+    b->side.push_back(s);
+    b->fork_ev.push_back(f);
+    b->join_ev.push_back(j);
+  }
+  for (size_t k = 1; k < ng; ++k) {
+    auto e = femdev::stream_wait(b->side[k - 1], b->stream, b->fork_ev[k - 1]);
+    if (!e.empty()) return e;
+  }
+  for (size_t k = 0; k < ng; ++k) {
+    auto* g = b->groups[k];
+    auto e = launch_values(g, femdev::values(g->dev), k ? b->side[k - 1] : b->stream, false);
+    if (!e.empty()) return e;
+  }
+  for (size_t k = 1; k < ng; ++k) {
+    auto e = femdev::stream_wait(b->stream, b->side[k - 1], b->join_ev[k - 1]);
     if (!e.empty()) return e;
   }
   return "";
@@ -1280,7 +1309,7 @@ int batch_capture(fem_batch* b) {
   b->graph = nullptr;
   auto e = femdev::graph_begin(b->stream);
   if (e.empty()) {
-    e = batch_launch_groups(b, b->stream, false);
+    e = batch_launch_branches(b);
     femdev::Graph* g = nullptr;
     auto e2 = femdev::graph_end(b->stream, g);
     if (e.empty()) e = e2;
@@ -2091,6 +2120,9 @@ void fem_batch_destroy(fem_batch* b) {
   femdev::graph_destroy(b->graph);
   femdev::timer_destroy(b->timer);
   for (auto* g : b->groups) fem_destroy(g);
+  for (void* s : b->side) femdev::stream_destroy(s);
+  for (void* ev : b->fork_ev) femdev::event_destroy(ev);
+  for (void* ev : b->join_ev) femdev::event_destroy(ev);
   femdev::stream_destroy(b->stream);
   delete b;
 }
