@@ -158,10 +158,12 @@ def test_beam_backward_matches_reference(gpu):
         m.backward(_cuda(g), xyz=True)
 
 
-def test_backward_properties_at_size(gpu):
+@pytest.mark.parametrize("nx,seed", [(256, 31), (1024, 0)])
+def test_backward_properties_at_size(gpu, nx, seed):
     """Linearity in dL/dK, translation invariance of dL/dX, E-homogeneity
-    tied to the assembled K, bitwise determinism -- on a 256 x 256 plate."""
-    mesh = meshgen.plate_grid(256, seed=31)
+    tied to the assembled K, bitwise determinism -- on a 256 x 256 plate and
+    on the 1024 x 1024 differentiation workload."""
+    mesh = meshgen.plate_grid(nx, seed=seed)
     m = FEM.from_mesh(mesh)
     m.assemble()
     _, nnz = m.csr_info()
@@ -189,9 +191,10 @@ def test_backward_properties_at_size(gpu):
     assert abs(lhs - rhs) < 1e-10 * np.sum(np.abs(g * val)), (lhs, rhs)
 
 
-def test_compliance_energy_identity_at_size(gpu):
+@pytest.mark.parametrize("nx,seed", [(128, 41), (1024, 0)])
+def test_compliance_energy_identity_at_size(gpu, nx, seed):
     """sum_e E_e dc/dE_e = -(u^T K u - drilling energy) for flat plates."""
-    mesh = meshgen.plate_grid(128, seed=41)
+    mesh = meshgen.plate_grid(nx, seed=seed)
     m = FEM.from_mesh(mesh)
     rp, ci, val = m.stiffness_csr()
     rng = np.random.default_rng(42)
