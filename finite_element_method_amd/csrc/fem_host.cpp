@@ -1270,8 +1270,8 @@ std::string batch_launch_groups_one(fem_model* g, void* stream) {
   return launch_values(g, femdev::values(g->dev), stream, false);
 }
 
-// the groups' launches as parallel branches of one captured graph
-std::string batch_launch_branches(fem_batch* b) {
+// side streams and fork / join events for groups 1.. (created before a capture)
+std::string batch_ensure_branches(fem_batch* b) {
   const size_t ng = b->groups.size();
   while (b->side.size() + 1 < ng) {
     void *s = nullptr, *f = nullptr, *j = nullptr;
@@ -1288,6 +1288,12 @@ std::string batch_launch_branches(fem_batch* b) {
     b->fork_ev.push_back(f);
     b->join_ev.push_back(j);
   }
+  return "";
+}
+
+// the groups' launches as parallel branches of one captured graph
+std::string batch_launch_branches(fem_batch* b) {
+  const size_t ng = b->groups.size();
   for (size_t k = 1; k < ng; ++k) {
     auto e = femdev::stream_wait(b->side[k - 1], b->stream, b->fork_ev[k - 1]);
     if (!e.empty()) return e;
@@ -1307,7 +1313,9 @@ std::string batch_launch_branches(fem_batch* b) {
 int batch_capture(fem_batch* b) {
   femdev::graph_destroy(b->graph);
   b->graph = nullptr;
-  auto e = femdev::graph_begin(b->stream);
+  auto e = batch_ensure_branches(b);
+  if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+  e = femdev::graph_begin(b->stream);
   if (e.empty()) {
     e = batch_launch_branches(b);
     femdev::Graph* g = nullptr;
