@@ -285,7 +285,7 @@ __device__ inline void truss_block(const double* __restrict__ xyz, const uint32_
 
 // ------------------------------------------------------------------ generic path
 template <int TYPE>
-__global__ void generic_items_kernel(const double* __restrict__ xyz, const uint32_t* __restrict__ conn,
+__global__ __launch_bounds__(512) void generic_items_kernel(const double* __restrict__ xyz, const uint32_t* __restrict__ conn,
                                      const double* __restrict__ params, const double* __restrict__ Rall,
                                      const double* __restrict__ derived, const uint16_t* __restrict__ eclass,
                                      const int64_t* __restrict__ ebase, const uint64_t* __restrict__ cmask,

@@ -90,7 +90,7 @@ __global__ void beam_forces_kernel(const uint32_t* __restrict__ conn, const doub
   o[9] = mt - L * fs / 2.0;
 }
 
-__global__ void plate_forces_kernel(const uint32_t* __restrict__ conn, const double* __restrict__ prm,
+__global__ __launch_bounds__(256) void plate_forces_kernel(const uint32_t* __restrict__ conn, const double* __restrict__ prm,
                                     const double* __restrict__ Rs, const double* __restrict__ xyz,
                                     const double* __restrict__ u, size_t n, double* __restrict__ out) {
   const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
