@@ -365,10 +365,30 @@ __global__ __launch_bounds__(256) void truss_adjoint_kernel(AdjArgs A) {
 }
 
 // ---------------------------------------------------------------- beams
+// every lane of a quad receives lane L's value
+template <int L>
+__device__ inline double quad_bcast(double v) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  const u32x2 x = __builtin_bit_cast(u32x2, v);
+  u32x2 r;
+  r.x = uint32_t(__builtin_amdgcn_mov_dpp(int(x.x), L * 0x55, 0xF, 0xF, false));
+  r.y = uint32_t(__builtin_amdgcn_mov_dpp(int(x.y), L * 0x55, 0xF, 0xF, false));
+  return __builtin_bit_cast(double, r);
+}
+
+// dL/dK: four lanes per beam, lane pq = block (p, q), so the four blocks'
+// scattered gathers run in parallel; lane 0 adds the term sums in block order
+// (pq = 0, 1, 2, 3) and writes the gradients.  Compliance: one lane per beam
+// (its two nodal displacements serve all four blocks).
+template <int MODE>
+constexpr int beam_lanes() { return MODE == MODE_DLDK ? 4 : 1; }
+
 template <int MODE>
 __global__ __launch_bounds__(256) void beam_adjoint_kernel(AdjArgs A) {
-  const size_t e = size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x;
-  if (e >= A.n) return;
+  constexpr int LANES = beam_lanes<MODE>();
+  const size_t e = (size_t(xcd_block(blockIdx.x, gridDim.x)) * blockDim.x + threadIdx.x) / LANES;
+  const int pq0 = LANES == 4 ? int(threadIdx.x & 3u) : 0;
+  if (e >= A.n) return;  // whole quads: the four lanes share e
   const uint32_t nd[2] = {A.conn[2 * e], A.conn[2 * e + 1]};
   double d[3];
 #pragma unroll
@@ -382,7 +402,7 @@ __global__ __launch_bounds__(256) void beam_adjoint_kernel(AdjArgs A) {
   const double* prm = A.params + 11 * e;
   double S[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
-  for (int pq = 0; pq < 4; ++pq) {
+  for (int pq = pq0; pq < pq0 + 4 / LANES; ++pq) {
     const int p = pq >> 1, q = pq & 1;
     double gl[36];
     if (MODE == MODE_DLDK) {
@@ -398,6 +418,14 @@ __global__ __launch_bounds__(256) void beam_adjoint_kernel(AdjArgs A) {
         for (int j = 0; j < 6; ++j) gl[i * 6 + j] = -(up[i] * uq[j]);
     }
     fem::beam_term_sums(gl, (p == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j, (q == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j, S);
+  }
+  if (LANES == 4) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+      const double s1 = quad_bcast<1>(S[k]), s2 = quad_bcast<2>(S[k]), s3 = quad_bcast<3>(S[k]);
+      S[k] = ((S[k] + s1) + s2) + s3;
+    }
+    if (pq0 != 0) return;
   }
   double i11p, i22p, angle;
   fem::beam_principal(prm[3], prm[4], prm[5], A.rel_tol, i11p, i22p, angle);
@@ -491,8 +519,10 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
         if (mode == MODE_DLDK) truss_adjoint_kernel<MODE_DLDK><<<blocks, 256, 0, s>>>(A);
         else truss_adjoint_kernel<MODE_U><<<blocks, 256, 0, s>>>(A);
       } else {
-        if (mode == MODE_DLDK) beam_adjoint_kernel<MODE_DLDK><<<blocks, 256, 0, s>>>(A);
-        else beam_adjoint_kernel<MODE_U><<<blocks, 256, 0, s>>>(A);
+        if (mode == MODE_DLDK)
+          beam_adjoint_kernel<MODE_DLDK><<<unsigned((beam_lanes<MODE_DLDK>() * n + 255) / 256), 256, 0, s>>>(A);
+        else
+          beam_adjoint_kernel<MODE_U><<<unsigned((beam_lanes<MODE_U>() * n + 255) / 256), 256, 0, s>>>(A);
       }
     }
     HIP_TRY(hipGetLastError());
