@@ -169,8 +169,37 @@ __device__ inline void load_boff(const AdjArgs& A, uint32_t k, uint32_t bo[6]) {
 // dL/dK mode every block is also standard: then the frame-rotation term
 // vanishes identically (the pattern has no in-plane / out-of-plane
 // couplings) and the blocks have fixed layouts.
+// point data of the block's 64 elements: lane p of an element computes point p
+constexpr int PA_POINT = 9;  // double2 per (element, point): plate_point's 17 values and a pad
+__device__ inline void store_point(double2* dst, const fem::PlatePoint& g) {
+  const double v[18] = {g.w, g.dx[0], g.dx[1], g.dx[2], g.dx[3], g.dy[0], g.dy[1], g.dy[2], g.dy[3],
+                        g.al[0], g.al[1], g.al[2], g.al[3], g.be[0], g.be[1], g.be[2], g.be[3], 0.0};
+#pragma unroll
+  for (int k = 0; k < PA_POINT; ++k) dst[k] = make_double2(v[2 * k], v[2 * k + 1]);
+}
+__device__ inline fem::PlatePoint load_point(const double2* src) {
+  double v[18];
+#pragma unroll
+  for (int k = 0; k < PA_POINT; ++k) {
+    const double2 x = src[k];
+    v[2 * k] = x.x;
+    v[2 * k + 1] = x.y;
+  }
+  fem::PlatePoint g;
+  g.w = v[0];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    g.dx[a] = v[1 + a];
+    g.dy[a] = v[5 + a];
+    g.al[a] = v[9 + a];
+    g.be[a] = v[13 + a];
+  }
+  return g;
+}
+
 template <int MODE, bool FLAT>
 __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
+  __shared__ double2 pts[64 * 4 * PA_POINT];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const size_t e0 = size_t(b) * 64 + (threadIdx.x >> 2);
   const int p = int(threadIdx.x & 3u);
@@ -206,14 +235,20 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   fem::PlateShear sv;
   fem::plate_shear_vectors(xl, sv);
   const uint32_t ap = p == 0 ? nd[0] : (p == 1 ? nd[1] : (p == 2 ? nd[2] : nd[3]));
-  // pass 1: integrals of row p
+  // pass 1: integrals of row p; the point data of point p come from lane p
+  double2* my_pts = pts + (threadIdx.x >> 2) * 4 * PA_POINT;
+  {
+    fem::PlatePoint g;
+    fem::plate_point(xl, p, g);
+    store_point(my_pts + p * PA_POINT, g);
+  }
+  __syncthreads();
   double acc[24];
 #pragma unroll
   for (int i = 0; i < 24; ++i) acc[i] = 0.0;
 #pragma unroll 1
   for (int ip = 0; ip < 4; ++ip) {
-    fem::PlatePoint g;
-    fem::plate_point(xl, ip, g);
+    const fem::PlatePoint g = load_point(my_pts + ip * PA_POINT);
     const double wdx = g.w * fem::sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p);
     const double wdy = g.w * fem::sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
     const double wal = g.w * fem::sel4(g.al[0], g.al[1], g.al[2], g.al[3], p);
@@ -291,9 +326,23 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
 #pragma unroll
       for (int j = 0; j < 4; ++j) c[4 * k + j] = j == q ? cq[k] : c[4 * k + j];
   }
-  // pass 3: reverse sweep over the integration points
+  // pass 3: reverse sweep over the integration points.  The sweep is linear in
+  // the seven adjoints of a point's forward quantities: the four rows' values
+  // are added first, and lane p sweeps point p.
+  {
+    double mine[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 1
-  for (int ip = 0; ip < 4; ++ip) fem::plate_point_adjoint(xl, ip, p, c, gxl);
+    for (int ip = 0; ip < 4; ++ip) {
+      double s7[7];
+      fem::plate_point_scalars(load_point(my_pts + ip * PA_POINT), ip, p, c, s7);
+#pragma unroll
+      for (int k = 0; k < 7; ++k) {
+        const double tot = quad_sum(s7[k]);
+        mine[k] = ip == p ? tot : mine[k];
+      }
+    }
+    fem::plate_point_reverse(xl, p, mine, gxl);
+  }
   // element sums over the four lanes
 #pragma unroll
   for (int i = 0; i < 7; ++i) P[i] = quad_sum(P[i]);

@@ -110,6 +110,91 @@ __host__ __device__ inline void plate_point_adjoint(const double xl[6], int ip, 
   gxl[5] += kuy * ms_ - kry * pr;
 }
 
+// The same sweep in two parts.  The second part is linear in the seven
+// adjoints of the point's forward quantities (det, the inverse Jacobian, the
+// two MITC4 multipliers), so the four row lanes of an element can add their
+// seven values first and run the second part once per point.
+//
+// Part 1: the seven adjoints s7 = (g_det, g_i00, g_i01, g_i10, g_i11, g_mr,
+// g_ms) of row p at point ip, from that point's data g (plate_point).
+__host__ __device__ inline void plate_point_scalars(const PlatePoint& g, int ip, int p, const double c[24],
+                                                    double s7[7]) {
+  const double gs = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? gs : -gs;
+  const double s = (ip < 2) ? gs : -gs;
+  const double q4 = 0.25;
+  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
+  const double dr[4] = {q4 + q4 * s, -(q4 + q4 * s), -(q4 - q4 * s), q4 - q4 * s};
+  const double ds[4] = {q4 + q4 * r, q4 - q4 * r, -(q4 - q4 * r), -(q4 + q4 * r)};
+  const double fa[4] = {ps, ps, ms_, ms_}, fb[4] = {pr, mr_, mr_, pr};
+  const double det = g.w;
+  const double dxp = sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p), dyp = sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
+  const double alp = sel4(g.al[0], g.al[1], g.al[2], g.al[3], p), bep = sel4(g.be[0], g.be[1], g.be[2], g.be[3], p);
+  double SX = 0.0, SY = 0.0, SA = 0.0, SB = 0.0;
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    SX += c[q] * g.dx[q] + c[4 + q] * g.dy[q];
+    SY += c[8 + q] * g.dx[q] + c[12 + q] * g.dy[q];
+    SA += c[16 + q] * g.al[q];
+    SB += c[20 + q] * g.be[q];
+  }
+  double g_det = ((dxp * SX + dyp * SY) + alp * SA) + bep * SB;
+  double g_i00 = 0.0, g_i01 = 0.0, g_i10 = 0.0, g_i11 = 0.0, g_mr = 0.0, g_ms = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const bool self = a == p;
+    const double gdx = det * ((dxp * c[a] + dyp * c[8 + a]) + (self ? SX : 0.0));
+    const double gdy = det * ((dxp * c[4 + a] + dyp * c[12 + a]) + (self ? SY : 0.0));
+    const double gal = det * (alp * c[16 + a] + (self ? SA : 0.0));
+    const double gbe = det * (bep * c[20 + a] + (self ? SB : 0.0));
+    g_i00 += gdx * dr[a];
+    g_i01 += gdx * ds[a];
+    g_i10 += gdy * dr[a];
+    g_i11 += gdy * ds[a];
+    g_mr += gal * fa[a];
+    g_ms += gbe * fb[a];
+  }
+  s7[0] = g_det; s7[1] = g_i00; s7[2] = g_i01; s7[3] = g_i10; s7[4] = g_i11; s7[5] = g_mr; s7[6] = g_ms;
+}
+
+// Part 2: from the seven adjoints of point ip (summed over rows) to xl.
+__host__ __device__ inline void plate_point_reverse(const double xl[6], int ip, const double s7[7], double gxl[6]) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double gs = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? gs : -gs;
+  const double s = (ip < 2) ? gs : -gs;
+  const double q4 = 0.25;
+  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
+  const double usx = (x1 * ps - x2 * ps) + x4 * ms_, usy = (y1 * ps - y2 * ps) + y4 * ms_;
+  const double urx = (x1 * pr + x2 * mr_) - x4 * pr, ury = (y1 * pr + y2 * mr_) - y4 * pr;
+  const double j00 = usx * q4, j01 = usy * q4, j10 = urx * q4, j11 = ury * q4;
+  const double det = j00 * j11 - j01 * j10;
+  const double rdet = 1.0 / det;
+  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+  const double sqr = sqrt(urx * urx + ury * ury), sqs = sqrt(usx * usx + usy * usy);
+  const double m_r = sqr * 0.125 * rdet, m_s = sqs * 0.125 * rdet;
+  double g_det = s7[0];
+  const double g_i00 = s7[1], g_i01 = s7[2], g_i10 = s7[3], g_i11 = s7[4], g_mr = s7[5], g_ms = s7[6];
+  g_det -= (g_mr * m_r + g_ms * m_s) * rdet;
+  const double g_sqr = g_mr * 0.125 * rdet, g_sqs = g_ms * 0.125 * rdet;
+  double g_j11 = g_i00 * rdet, g_j01 = -g_i01 * rdet, g_j10 = -g_i10 * rdet, g_j00 = g_i11 * rdet;
+  g_det -= (((g_i00 * i00 + g_i01 * i01) + g_i10 * i10) + g_i11 * i11) * rdet;
+  g_j00 += g_det * j11;
+  g_j11 += g_det * j00;
+  g_j01 -= g_det * j10;
+  g_j10 -= g_det * j01;
+  const double kux = g_j00 * q4 + (sqs > 0.0 ? g_sqs * usx / sqs : 0.0);
+  const double kuy = g_j01 * q4 + (sqs > 0.0 ? g_sqs * usy / sqs : 0.0);
+  const double krx = g_j10 * q4 + (sqr > 0.0 ? g_sqr * urx / sqr : 0.0);
+  const double kry = g_j11 * q4 + (sqr > 0.0 ? g_sqr * ury / sqr : 0.0);
+  gxl[0] += kux * ps + krx * pr;
+  gxl[1] += kuy * ps + kry * pr;
+  gxl[2] += -kux * ps + krx * mr_;
+  gxl[3] += -kuy * ps + kry * mr_;
+  gxl[4] += kux * ms_ - krx * pr;
+  gxl[5] += kuy * ms_ - kry * pr;
+}
+
 // adjoint of the shear vectors of node a (components 1 and 2; component 0 is
 // the constant +-1/2): gv = dL/dv_a, gu = dL/du_a, mapped onto xl through
 //   e12 = (-(y1-y2), x1-x2)/4, e43 = (-y4, x4)/4, e14 = (-(y1-y4), x1-x4)/4, e23 = (-y2, x2)/4
