@@ -9,8 +9,11 @@
 // the element).  The diagonal block (a, a) is summed in registers over all
 // incident elements (insertion order, starting from the first contribution);
 // an off-diagonal block (a, b) has exactly one contributing element (checked
-// when the maps are built) and is written directly.  Every CSR value is
-// written once: no contribution buffer, no atomics, bitwise reproducible.
+// when the maps are built).  The rows are formed in LDS and the workgroup's
+// owned rows, one contiguous CSR range, are then written out by consecutive
+// lanes (a scattered row-by-row write costs one memory request per value).
+// Every CSR value is written once: no contribution buffer, no atomics,
+// bitwise reproducible.
 //
 // Beam blocks use the rank-one form of beam.rs:495-605: with T = diag(R, R),
 //   row i of T^T (sum_t s_t v_t,p v_t,q^T) T = sum_t s_t (T^T v_t,p)_i (T^T v_t,q)^T.
@@ -30,6 +33,9 @@ constexpr int BO_THREADS = 256;
 constexpr int BO_NODES = 42;     // owned nodes per workgroup (6 row lanes each)
 constexpr int BO_ENTRIES = 256;  // (owned node, incident element) entries per workgroup
 constexpr int BO_DATA = 18;      // doubles per staged entry
+constexpr int BO_VALUES = 4096;  // CSR values of the owned rows per workgroup
+constexpr int TO_NODES = 85;     // truss kernel: owned nodes per workgroup (3 row lanes each)
+constexpr int TO_ENTRIES = 512;  // truss kernel: entries per workgroup
 
 struct BarArgs {
   const double* __restrict__ xyz;
@@ -53,7 +59,22 @@ struct BarLds {
   uint16_t off[BO_ENTRIES][2][6];      // per-row offsets of those blocks
   uint32_t type[BO_ENTRIES];
   uint32_t first[BO_NODES + 1];        // entry range of each owned node
+  double out[BO_VALUES];               // the owned rows
 };
+static_assert(2 * sizeof(BarLds) <= 160 * 1024, "two workgroups per CU");
+static_assert(BO_NODES == BAR_WG_NODES && BO_ENTRIES == BAR_WG_ENTRIES && BO_VALUES == BAR_WG_VALUES, "range cut");
+
+// Truss-only ranges: rows 3..5 are empty and an entry needs five constants,
+// so the layout is compact (four workgroups per CU) and a range holds twice
+// the entries.
+struct TrussLds {
+  double data[TO_ENTRIES][5];      // r (3), kpp, kpq
+  uint64_t mask[TO_ENTRIES][2];    // masks of the entry's blocks (p, p) and (p, 1 - p)
+  uint16_t off[TO_ENTRIES][2][3];  // offsets of rows 0..2 of those blocks
+  uint32_t first[TO_NODES + 1];
+};
+static_assert(4 * sizeof(TrussLds) <= 160 * 1024, "four workgroups per CU");
+static_assert(TO_NODES == TRUSS_WG_NODES && TO_ENTRIES == TRUSS_WG_ENTRIES && 3 * TO_NODES <= BO_THREADS, "range cut");
 
 // T^T v_t for the beam's rank-one term t with nodal shape value b
 // (v: u = b e0; v = b e1 - e5/2; w = b e2 - e4/2; tu = b e3; tv = b e4; tw = b e5)
@@ -70,12 +91,9 @@ __device__ inline void beam_term_vec(int t, double b, const double R[9], double 
   }
 }
 
-// stage one (owned node, incident element) entry: the element constants
-// (truss.rs:132-189, beam.rs:495-605) and its two blocks' masks and offsets
-__device__ inline void stage_entry(const BarArgs& A, BarLds& L, int slot, uint32_t u, int i_row_unused) {
-  (void)i_row_unused;
-  const int type = int(u >> 31), p = int(u & 1u);
-  const size_t e = (u >> 1) & 0x3FFFFFFFu;
+// Jacobian of element e of `type` and the nodal shape derivatives b_p, b_q
+// of its end p and the other end
+__device__ inline double bar_shape(const BarArgs& A, int type, size_t e, int p, double& bp, double& bq) {
   const uint32_t n0 = A.conn[type][2 * e], n1 = A.conn[type][2 * e + 1];
   double d[3];
 #pragma unroll
@@ -83,18 +101,38 @@ __device__ inline void stage_entry(const BarArgs& A, BarLds& L, int slot, uint32
   const double Lg = fem::norm3(d[0], d[1], d[2]);
   const double J = fem::bar_jacobian(Lg);
   const double inv_j = 1.0 / J;
-  const double bp = (p == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j, bq = (p == 0 ? 0.0 + 0.5 : 0.0 - 0.5) * inv_j;
-  const double* Rp = A.R[type] + 9 * e;
+  bp = (p == 0 ? 0.0 - 0.5 : 0.0 + 0.5) * inv_j;
+  bq = (p == 0 ? 0.0 + 0.5 : 0.0 - 0.5) * inv_j;
+  return J;
+}
+
+// truss constants (truss.rs:132-189): direction r and the scales of the
+// (p, p) and (p, q) blocks, k_pq = kpq r r^T
+__device__ inline void truss_constants(const BarArgs& A, size_t e, int p, double o[5]) {
+  double bp, bq;
+  const double J = bar_shape(A, 0, e, p, bp, bq);
+  const double* Rp = A.R[0] + 9 * e;
+  const double* prm = A.params[0] + 3 * e;
+  const double E = prm[0], Ar = prm[1], A2 = prm[2];
+  const double area = (A2 == A2) ? (A2 - Ar) / 2.0 * 0.0 + Ar - (A2 - Ar) / 2.0 * -1.0 : Ar;
+  const double scale = area * E * J * 2.0;
+  o[0] = Rp[0]; o[1] = Rp[1]; o[2] = Rp[2];
+  o[3] = (bp * bp) * scale;
+  o[4] = (bp * bq) * scale;
+}
+
+// stage one (owned node, incident element) entry: the element constants
+// (truss.rs:132-189, beam.rs:495-605) and its two blocks' masks and offsets
+__device__ inline void stage_entry(const BarArgs& A, BarLds& L, int slot, uint32_t u) {
+  const int type = int(u >> 31), p = int(u & 1u);
+  const size_t e = (u >> 1) & 0x3FFFFFFFu;
   double* o = L.data + slot * BO_DATA;
   if (type == 0) {
-    const double* prm = A.params[0] + 3 * e;
-    const double E = prm[0], Ar = prm[1], A2 = prm[2];
-    const double area = (A2 == A2) ? (A2 - Ar) / 2.0 * 0.0 + Ar - (A2 - Ar) / 2.0 * -1.0 : Ar;
-    const double scale = area * E * J * 2.0;
-    o[0] = Rp[0]; o[1] = Rp[1]; o[2] = Rp[2];
-    o[3] = (bp * bp) * scale;
-    o[4] = (bp * bq) * scale;
+    truss_constants(A, e, p, o);
   } else {
+    double bp, bq;
+    const double J = bar_shape(A, 1, e, p, bp, bq);
+    const double* Rp = A.R[1] + 9 * e;
     const double* prm = A.params[1] + 11 * e;
     const double E = prm[0], nu = prm[1], Ar = prm[2], it = prm[6], k = prm[7];
     const double G = E / (2.0 * (1.0 + nu));
@@ -156,40 +194,105 @@ __device__ inline void entry_rows(const BarLds& L, int slot, int i, double self[
   }
 }
 
-__device__ inline void write_row(double* val, uint64_t mask, const uint16_t* off, int i, int64_t r0, const double v[6]) {
+// row i of one block into the staged rows (r0: the row's start within them)
+__device__ inline void write_row(BarLds& L, uint64_t mask, const uint16_t* off, int i, int r0, const double v[6]) {
   const uint32_t mi = uint32_t(mask >> (6 * i)) & 63u;
-  double* dst = val + (r0 + off[i]);
+  const int d0 = r0 + int(off[i]);
 #pragma unroll
   for (int j = 0; j < 6; ++j)
-    if ((mi >> j) & 1u) dst[__builtin_popcount(mi & ((1u << j) - 1u))] = v[j];
+    if ((mi >> j) & 1u) L.out[d0 + __builtin_popcount(mi & ((1u << j) - 1u))] = v[j];
 }
 
 // One workgroup per range of owned nodes: stage the incident entries (one
-// lane each, all loads in parallel), then one lane per (owned node, row).
+// lane each, all loads in parallel), form the rows with one lane per (owned
+// node, row), then write the range out with consecutive lanes.
 __global__ __launch_bounds__(BO_THREADS) void bar_owner_kernel(BarArgs A) {
   __shared__ BarLds L;
   const int t = threadIdx.x;
   const uint2 rg = A.ranges[blockIdx.x];
   const uint32_t a0 = rg.x, na = rg.y;
   const uint32_t e0 = A.inc_ptr[a0], ne = A.inc_ptr[a0 + na] - e0;
+  const int64_t base = A.row_ptr[6 * size_t(a0)];
+  const int nv = int(A.row_ptr[6 * size_t(a0 + na)] - base);  // <= BO_VALUES (cut on the host)
   if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
-  if (t < int(ne)) stage_entry(A, L, t, A.inc[e0 + t], 0);
+  if (t < int(ne)) stage_entry(A, L, t, A.inc[e0 + t]);
   __syncthreads();
   const int j = t / 6, i = t % 6;
+  if (j < int(na)) {
+    const size_t a = size_t(a0) + j;
+    const int r0 = int(A.row_ptr[6 * a + i] - base);
+    const int s0 = int(L.first[j]), s1 = int(L.first[j + 1]);
+    if (A.row_ptr[6 * a + i + 1] - base > r0) {
+      double acc[6];
+      for (int sl = s0; sl < s1; ++sl) {
+        double self[6], nb[6];
+        entry_rows(L, sl, i, self, nb);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] = sl == s0 ? self[q] : acc[q] + self[q];
+        write_row(L, L.mask[sl][1], L.off[sl][1], i, r0, nb);
+      }
+      if (s1 > s0) write_row(L, L.mask[s0][0], L.off[s0][0], i, r0, acc);
+    }
+  }
+  __syncthreads();
+  double* __restrict__ dst = A.val + base;
+  for (int k = t; k < nv; k += BO_THREADS) dst[k] = L.out[k];
+}
+
+// row i < 3 of a truss block (columns 0..2) straight to the CSR
+__device__ inline void write_row3(double* val, uint64_t mask, int off, int i, int64_t r0, const double v[3]) {
+  const uint32_t mi = uint32_t(mask >> (6 * i)) & 7u;
+  double* dst = val + (r0 + off);
+#pragma unroll
+  for (int j = 0; j < 3; ++j)
+    if ((mi >> j) & 1u) dst[__builtin_popcount(mi & ((1u << j) - 1u))] = v[j];
+}
+
+// Truss-only ranges: the same sums as bar_owner_kernel (diagonal block summed
+// in insertion order from the first contribution, off-diagonal blocks written
+// by their one element), one lane per (owned node, row 0..2).
+__global__ __launch_bounds__(BO_THREADS) void truss_owner_kernel(BarArgs A) {
+  __shared__ TrussLds L;
+  const int t = threadIdx.x;
+  const uint2 rg = A.ranges[blockIdx.x];
+  const uint32_t a0 = rg.x, na = rg.y;
+  const uint32_t e0 = A.inc_ptr[a0], ne = A.inc_ptr[a0 + na] - e0;
+  if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
+  for (int sl = t; sl < int(ne); sl += BO_THREADS) {
+    const uint32_t u = A.inc[e0 + sl];
+    const int p = int(u & 1u);
+    const size_t e = (u >> 1) & 0x3FFFFFFFu;
+    truss_constants(A, e, p, L.data[sl]);
+    const uint32_t ks = A.bid[0][4 * e + 3 * p], kn = A.bid[0][4 * e + 2 * p + (1 - p)];
+    L.mask[sl][0] = A.bmask[ks];
+    L.mask[sl][1] = A.bmask[kn];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      L.off[sl][0][i] = A.boff[6 * size_t(ks) + i];
+      L.off[sl][1][i] = A.boff[6 * size_t(kn) + i];
+    }
+  }
+  __syncthreads();
+  const int j = t / 3, i = t % 3;
   if (j >= int(na)) return;
   const size_t a = size_t(a0) + j;
   const int64_t r0 = A.row_ptr[6 * a + i];
   if (A.row_ptr[6 * a + i + 1] == r0) return;
   const int s0 = int(L.first[j]), s1 = int(L.first[j + 1]);
-  double acc[6];
+  double acc[3];
   for (int sl = s0; sl < s1; ++sl) {
-    double self[6], nb[6];
-    entry_rows(L, sl, i, self, nb);
+    const double* o = L.data[sl];
+    const double ri = o[i];
+    double nb[3];
 #pragma unroll
-    for (int q = 0; q < 6; ++q) acc[q] = sl == s0 ? self[q] : acc[q] + self[q];
-    write_row(A.val, L.mask[sl][1], L.off[sl][1], i, r0, nb);
+    for (int q = 0; q < 3; ++q) {
+      const double self = (ri * o[3]) * o[q];
+      nb[q] = (ri * o[4]) * o[q];
+      acc[q] = sl == s0 ? self : acc[q] + self;
+    }
+    write_row3(A.val, L.mask[sl][1], L.off[sl][1][i], i, r0, nb);
   }
-  if (s1 > s0) write_row(A.val, L.mask[s0][0], L.off[s0][0], i, r0, acc);
+  if (s1 > s0) write_row3(A.val, L.mask[s0][0], L.off[s0][0][i], i, r0, acc);
 }
 
 }  // namespace
@@ -202,14 +305,15 @@ std::string upload_bar_maps(Device* d, const BarMaps& bm) {
   if (!(e = dupload(d->bo_boff, bm.boff.data(), bm.boff.size())).empty()) return e;
   if (!(e = dupload(d->bo_inc_ptr, bm.inc_ptr.data(), bm.inc_ptr.size())).empty()) return e;
   if (!(e = dupload(d->bo_ranges, bm.ranges.data(), bm.ranges.size())).empty()) return e;
-  d->bo_n_ranges = bm.ranges.size() / 2;
+  d->bo_n_ranges = bm.n_bar_ranges;
+  d->bo_n_truss_ranges = bm.ranges.size() / 2 - bm.n_bar_ranges;
   return dupload(d->bo_inc, bm.inc.data(), bm.inc.size());
 }
 
 std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (timed) HIP_TRY(hipEventRecord(d->ev0, s));
-  if (d->bo_n_ranges) {
+  if (d->bo_n_ranges + d->bo_n_truss_ranges) {
     BarArgs A;
     A.xyz = d->xyz;
     for (int t = 0; t < 2; ++t) {
@@ -226,8 +330,15 @@ std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed)
     A.ranges = reinterpret_cast<const uint2*>(d->bo_ranges);
     A.row_ptr = d->row_ptr;
     A.val = val;
-    bar_owner_kernel<<<unsigned(d->bo_n_ranges), BO_THREADS, 0, s>>>(A);
-    HIP_TRY(hipGetLastError());
+    if (d->bo_n_ranges) {
+      bar_owner_kernel<<<unsigned(d->bo_n_ranges), BO_THREADS, 0, s>>>(A);
+      HIP_TRY(hipGetLastError());
+    }
+    if (d->bo_n_truss_ranges) {
+      A.ranges += d->bo_n_ranges;
+      truss_owner_kernel<<<unsigned(d->bo_n_truss_ranges), BO_THREADS, 0, s>>>(A);
+      HIP_TRY(hipGetLastError());
+    }
   }
   if (timed) {
     HIP_TRY(hipEventRecord(d->ev1, s));

@@ -45,7 +45,7 @@ struct Device {
   uint64_t* bo_bmask = nullptr;
   uint16_t* bo_boff = nullptr;
   uint32_t *bo_inc_ptr = nullptr, *bo_inc = nullptr, *bo_ranges = nullptr;
-  size_t bo_n_ranges = 0;
+  size_t bo_n_ranges = 0, bo_n_truss_ranges = 0;  // bar-kernel ranges, then truss-kernel ranges
   // adjoint maps
   uint32_t* adj_bid[3] = {nullptr, nullptr, nullptr};
   uint64_t* adj_bmask = nullptr;

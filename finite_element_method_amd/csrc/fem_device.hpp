@@ -133,9 +133,14 @@ struct BarMaps {
   std::vector<uint16_t> boff;     // per block and row: first entry of the block within the row
   std::vector<uint32_t> inc_ptr;  // N + 1
   std::vector<uint32_t> inc;      // per node, insertion order: type << 31 | element << 1 | p
-  std::vector<uint32_t> ranges;   // per workgroup: first node, node count (<= 42 nodes, <= 256 entries)
+  // per workgroup: first node, node count.  The first n_bar_ranges go to the
+  // bar kernel (<= 42 nodes, <= 256 entries, <= 4096 values), the rest, whose
+  // nodes touch trusses only, to the truss kernel (<= 85 nodes, <= 512 entries)
+  std::vector<uint32_t> ranges;
+  size_t n_bar_ranges = 0;
 };
-constexpr int BAR_WG_NODES = 42, BAR_WG_ENTRIES = 256;
+constexpr int BAR_WG_NODES = 42, BAR_WG_ENTRIES = 256, BAR_WG_VALUES = 4096;
+constexpr int TRUSS_WG_NODES = 85, TRUSS_WG_ENTRIES = 512;
 std::string upload_bar_maps(Device* d, const BarMaps& bm);
 std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed = true);
 

@@ -977,18 +977,34 @@ std::string build_pattern(fem_model* m) {
         const uint32_t* c = &m->set[r.t].conn[size_t(r.pos) * 2];
         for (int p = 0; p < 2; ++p) bm.inc[f[c[p]]++] = uint32_t(r.t) << 31 | r.pos << 1 | uint32_t(p);
       }
-      // workgroup ranges: consecutive nodes, <= 42 nodes and <= 256 entries each
+      // workgroup ranges of consecutive nodes.  Nodes whose incident elements
+      // are all trusses go to the truss kernel (<= 85 nodes, <= 512 entries);
+      // the others to the bar kernel (<= 42 nodes, <= 256 entries, <= 4096
+      // CSR values, since it stages the rows in LDS).
+      auto truss_only = [&](uint32_t a) {
+        for (uint32_t k = bm.inc_ptr[a]; k < bm.inc_ptr[a + 1]; ++k)
+          if (bm.inc[k] >> 31) return false;
+        return true;
+      };
+      std::vector<uint32_t> truss_ranges;
       uint32_t a0 = 0;
       while (a0 < N && ok) {
+        const bool tonly = truss_only(a0);
+        const uint32_t max_nodes = uint32_t(tonly ? femdev::TRUSS_WG_NODES : femdev::BAR_WG_NODES);
+        const uint32_t max_entries = uint32_t(tonly ? femdev::TRUSS_WG_ENTRIES : femdev::BAR_WG_ENTRIES);
         uint32_t na = 0;
-        while (a0 + na < N && na < uint32_t(femdev::BAR_WG_NODES) &&
-               bm.inc_ptr[a0 + na + 1] - bm.inc_ptr[a0] <= uint32_t(femdev::BAR_WG_ENTRIES))
+        while (a0 + na < N && na < max_nodes && (na == 0 || truss_only(a0 + na) == tonly) &&
+               bm.inc_ptr[a0 + na + 1] - bm.inc_ptr[a0] <= max_entries &&
+               (tonly || row_ptr[6 * size_t(a0 + na + 1)] - row_ptr[6 * size_t(a0)] <= int64_t(femdev::BAR_WG_VALUES)))
           na++;
-        if (na == 0) ok = false;  // one node with more than 256 incident elements
-        bm.ranges.push_back(a0);
-        bm.ranges.push_back(na);
+        if (na == 0) ok = false;  // one node with too many incident elements or CSR values
+        auto& dst = tonly ? truss_ranges : bm.ranges;
+        dst.push_back(a0);
+        dst.push_back(na);
         a0 += na;
       }
+      bm.n_bar_ranges = bm.ranges.size() / 2;
+      bm.ranges.insert(bm.ranges.end(), truss_ranges.begin(), truss_ranges.end());
     }
     if (ok && blocks.size() < (size_t(1) << 32)) {
       e = femdev::upload_bar_maps(m->dev, bm);

@@ -103,3 +103,53 @@ def test_plate_kernels_agree(gpu, name):
     for k in (1, 2):
         for a, b in zip(out[0], out[k]):
             assert np.array_equal(a, b), k
+
+
+def _mixed_bars(model_cls):
+    """A 5^3 beam frame and a 5^3 truss lattice beside it, truss diagonals
+    inside the frame (tapered and not) and trusses bridging the two, added to
+    `model_cls` in alternating batches: nodes that touch beams only, trusses
+    only, and both; both kinds of workgroup range in one model."""
+    fr = meshgen.beam_frame(5, seed=6)
+    tl = meshgen.truss_lattice(5, seed=7)
+    off = fr.n_nodes
+    numbers = np.concatenate([fr.node_numbers, tl.node_numbers + off])
+    xyz = np.concatenate([fr.xyz, tl.xyz + np.array([10.0, 0.0, 0.0])])
+    rng = np.random.default_rng(9)
+    used = {tuple(sorted(map(int, c))) for c in fr.conn} | {tuple(sorted(map(int, c + off))) for c in tl.conn}
+    extra = []
+    while len(extra) < 120:  # 60 frame diagonals, then 60 bridges
+        a = int(rng.integers(1, off + 1))
+        b = int(rng.integers(1, off + 1)) if len(extra) < 60 else int(rng.integers(off + 1, off + tl.n_nodes + 1))
+        key = tuple(sorted((a, b)))
+        if a != b and key not in used:
+            used.add(key)
+            extra.append(key)
+    extra = np.array(extra, dtype=np.uint32)
+    A = rng.uniform(1e-4, 5e-4, len(extra))
+    A2 = np.where(rng.random(len(extra)) < 0.5, A * 1.5, np.nan)
+    m = model_cls(fr.rel_tol, fr.abs_tol, len(numbers))
+    m.add_nodes(numbers, xyz)
+    nb, nt = fr.n_elements, tl.n_elements
+    beam_no = fr.numbers
+    truss_no = tl.numbers + nb
+    extra_no = np.arange(1, len(extra) + 1, dtype=np.uint32) + nb + nt
+    h = nb // 2
+    m.add_beams(beam_no[:h], fr.conn[:h], fr.props[:h])
+    m.add_trusses(extra_no[:60], extra[:60], 2.1e11, A[:60], A2[:60])
+    m.add_trusses(truss_no, tl.conn + off, tl.props[:, 0], tl.props[:, 1], tl.props[:, 2])
+    m.add_beams(beam_no[h:], fr.conn[h:], fr.props[h:])
+    m.add_trusses(extra_no[60:], extra[60:], 2.1e11, A[60:], A2[60:])
+    return m
+
+
+def test_mixed_bars_match_oracle(gpu):
+    ref = _mixed_bars(OracleModel).csr(structural=True)
+    m = _mixed_bars(FEM)
+    got = m.stiffness_csr()
+    r = compare(got, ref)
+    print("mixed bars", m.assembly_path(), r, flush=True)
+    assert m.assembly_path() == "bar_owner"
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r
+    again = m.stiffness_csr()[2]
+    assert np.array_equal(got[2].view(np.uint64), again.view(np.uint64))
