@@ -71,7 +71,9 @@ struct TrussLds {
   double data[TO_ENTRIES][5];      // r (3), kpp, kpq
   uint64_t mask[TO_ENTRIES][2];    // masks of the entry's blocks (p, p) and (p, 1 - p)
   uint16_t off[TO_ENTRIES][2][3];  // offsets of rows 0..2 of those blocks
+  uint8_t node[TO_ENTRIES];        // owned node of the entry
   uint32_t first[TO_NODES + 1];
+  uint32_t row[TO_NODES][3];       // start of rows 0..2 of each owned node, from the range's first value
 };
 static_assert(4 * sizeof(TrussLds) <= 160 * 1024, "four workgroups per CU");
 static_assert(TO_NODES == TRUSS_WG_NODES && TO_ENTRIES == TRUSS_WG_ENTRIES && 3 * TO_NODES <= BO_THREADS, "range cut");
@@ -248,51 +250,89 @@ __device__ inline void write_row3(double* val, uint64_t mask, int off, int i, in
     if ((mi >> j) & 1u) dst[__builtin_popcount(mi & ((1u << j) - 1u))] = v[j];
 }
 
+// one truss entry's constants, masks and row offsets (registers)
+struct TrussEntry {
+  double o[5];
+  uint64_t m[2];
+  uint16_t off[2][3];
+};
+
+__device__ inline void truss_entry(const BarArgs& A, uint32_t u, TrussEntry& E) {
+  const int p = int(u & 1u);
+  const size_t e = (u >> 1) & 0x3FFFFFFFu;
+  truss_constants(A, e, p, E.o);
+  const uint32_t ks = A.bid[0][4 * e + 3 * p], kn = A.bid[0][4 * e + 2 * p + (1 - p)];
+  E.m[0] = A.bmask[ks];
+  E.m[1] = A.bmask[kn];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    E.off[0][i] = A.boff[6 * size_t(ks) + i];
+    E.off[1][i] = A.boff[6 * size_t(kn) + i];
+  }
+}
+
+__device__ inline void put_entry(TrussLds& L, int sl, const TrussEntry& E) {
+#pragma unroll
+  for (int k = 0; k < 5; ++k) L.data[sl][k] = E.o[k];
+  L.mask[sl][0] = E.m[0];
+  L.mask[sl][1] = E.m[1];
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    L.off[sl][0][i] = E.off[0][i];
+    L.off[sl][1][i] = E.off[1][i];
+  }
+}
+
 // Truss-only ranges: the same sums as bar_owner_kernel (diagonal block summed
 // in insertion order from the first contribution, off-diagonal blocks written
-// by their one element), one lane per (owned node, row 0..2).
+// by their one element).  Staging: two entries per lane, loads of both in
+// flight together.  Off-diagonal rows: one task per (row, entry), so that
+// consecutive lanes write neighbouring blocks of one row.  Diagonal rows: one
+// lane per (owned node, row 0..2).
 __global__ __launch_bounds__(BO_THREADS) void truss_owner_kernel(BarArgs A) {
   __shared__ TrussLds L;
   const int t = threadIdx.x;
   const uint2 rg = A.ranges[blockIdx.x];
   const uint32_t a0 = rg.x, na = rg.y;
-  const uint32_t e0 = A.inc_ptr[a0], ne = A.inc_ptr[a0 + na] - e0;
+  const uint32_t e0 = A.inc_ptr[a0];
+  const int ne = int(A.inc_ptr[a0 + na] - e0);  // <= TO_ENTRIES (cut on the host)
+  const int64_t base = A.row_ptr[6 * size_t(a0)];
   if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
-  for (int sl = t; sl < int(ne); sl += BO_THREADS) {
-    const uint32_t u = A.inc[e0 + sl];
-    const int p = int(u & 1u);
-    const size_t e = (u >> 1) & 0x3FFFFFFFu;
-    truss_constants(A, e, p, L.data[sl]);
-    const uint32_t ks = A.bid[0][4 * e + 3 * p], kn = A.bid[0][4 * e + 2 * p + (1 - p)];
-    L.mask[sl][0] = A.bmask[ks];
-    L.mask[sl][1] = A.bmask[kn];
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-      L.off[sl][0][i] = A.boff[6 * size_t(ks) + i];
-      L.off[sl][1][i] = A.boff[6 * size_t(kn) + i];
-    }
+  if (t < int(na)) {
+    const int f0 = int(A.inc_ptr[a0 + t] - e0), f1 = int(A.inc_ptr[a0 + t + 1] - e0);
+    for (int sl = f0; sl < f1; ++sl) L.node[sl] = uint8_t(t);
+  }
+  if (t < 3 * int(na)) L.row[t / 3][t % 3] = uint32_t(A.row_ptr[6 * size_t(a0 + t / 3) + t % 3] - base);
+  if (ne > 0) {
+    TrussEntry E0, E1;
+    truss_entry(A, A.inc[e0 + min(t, ne - 1)], E0);
+    truss_entry(A, A.inc[e0 + min(t + BO_THREADS, ne - 1)], E1);
+    if (t < ne) put_entry(L, t, E0);
+    if (t + BO_THREADS < ne) put_entry(L, t + BO_THREADS, E1);
   }
   __syncthreads();
+  double* __restrict__ val = A.val + base;
+  for (int k = t; k < 3 * ne; k += BO_THREADS) {
+    const int i = k / ne, sl = k - i * ne;
+    const double* o = L.data[sl];
+    const double s = o[i] * o[4];
+    double nb[3];
+#pragma unroll
+    for (int q = 0; q < 3; ++q) nb[q] = s * o[q];
+    write_row3(val, L.mask[sl][1], L.off[sl][1][i], i, L.row[L.node[sl]][i], nb);
+  }
   const int j = t / 3, i = t % 3;
   if (j >= int(na)) return;
-  const size_t a = size_t(a0) + j;
-  const int64_t r0 = A.row_ptr[6 * a + i];
-  if (A.row_ptr[6 * a + i + 1] == r0) return;
   const int s0 = int(L.first[j]), s1 = int(L.first[j + 1]);
+  if (s1 == s0) return;
   double acc[3];
   for (int sl = s0; sl < s1; ++sl) {
     const double* o = L.data[sl];
-    const double ri = o[i];
-    double nb[3];
+    const double s = o[i] * o[3];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) {
-      const double self = (ri * o[3]) * o[q];
-      nb[q] = (ri * o[4]) * o[q];
-      acc[q] = sl == s0 ? self : acc[q] + self;
-    }
-    write_row3(A.val, L.mask[sl][1], L.off[sl][1][i], i, r0, nb);
+    for (int q = 0; q < 3; ++q) acc[q] = sl == s0 ? s * o[q] : acc[q] + s * o[q];
   }
-  if (s1 > s0) write_row3(A.val, L.mask[s0][0], L.off[s0][0][i], i, r0, acc);
+  write_row3(val, L.mask[s0][0], L.off[s0][0][i], i, L.row[j][i], acc);
 }
 
 }  // namespace
