@@ -47,6 +47,7 @@ struct AdjArgs {
   double* dparam;  // n x params_per, may be null
   double* xs;      // n x nodes_per x 3, may be null
   double rel_tol;
+  uint32_t g_bytes;  // dL/dK: bytes of g when below 4 GiB (wide range-checked loads), else 0
 };
 
 // logical block of hardware block b: the hardware deals blocks to the eight
@@ -60,16 +61,44 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
 // global-frame 6x6 gradient block of node pair block k, rows of node a.  The
 // block's entries of row i are contiguous in the CSR row: load the run, then
 // place it by the row mask.
+// With g_bytes set the run is read in 16-byte pairs (and one 8-byte tail) by
+// buffer loads, range-checked against g: half the memory requests of one load
+// per value, and the requests, not the bytes, bound this gather.
 __device__ inline void gather_block(const AdjArgs& A, uint32_t k, uint32_t a, double G[36]) {
+  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const uint64_t um = A.bmask[k];
+  const __amdgpu_buffer_rsrc_t rsrc =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(A.g), 0, int(A.g_bytes), 0x00020000);
+  uint32_t bo[3];  // the block's six row offsets, two per word (boff rows start 4-byte aligned)
+#pragma unroll
+  for (int m = 0; m < 3; ++m) bo[m] = reinterpret_cast<const uint32_t*>(A.boff + 6 * size_t(k))[m];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
     const uint32_t mi = uint32_t(um >> (6 * i)) & 63u;
     const int n = __builtin_popcount(mi);
-    const double* src = A.g + (A.row_ptr[6 * size_t(a) + i] + A.boff[6 * size_t(k) + i]);
+    const uint32_t off_i = (bo[i >> 1] >> (16 * (i & 1))) & 0xFFFFu;
+    const int64_t pos = A.row_ptr[6 * size_t(a) + i] + off_i;
     double v[6];
+    if (A.g_bytes) {
+      const uint32_t byte = uint32_t(pos) * 8u;
 #pragma unroll
-    for (int c = 0; c < 6; ++c) v[c] = c < n ? src[c] : 0.0;
+      for (int c = 0; c < 6; c += 2) {
+        v[c] = 0.0;
+        v[c + 1] = 0.0;
+        if (c + 1 < n) {
+          const double2 w = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rsrc, byte + 8u * c, 0, 0));
+          v[c] = w.x;
+          v[c + 1] = w.y;
+        } else if (c < n) {
+          v[c] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(rsrc, byte + 8u * c, 0, 0));
+        }
+      }
+    } else {
+      const double* src = A.g + pos;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) v[c] = c < n ? src[c] : 0.0;
+    }
 #pragma unroll
     for (int j = 0; j < 6; ++j) {
       const int c = __builtin_popcount(mi & ((1u << j) - 1u));
@@ -556,6 +585,7 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.dparam = dparam[t];
     A.xs = (dxyz && t != BEAM) ? d->adj_xs + 3 * d->adj_xoff[t] : nullptr;
     A.rel_tol = d->rel_tol;
+    A.g_bytes = (mode == MODE_DLDK && size_t(d->nnz) * 8 < (size_t(1) << 32)) ? uint32_t(size_t(d->nnz) * 8) : 0u;
     if (t == PLATE) {
       const unsigned blocks = unsigned((n + 63) / 64);
       if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<MODE_DLDK, true><<<blocks, 256, 0, s>>>(A);
