@@ -563,7 +563,8 @@ std::string upload_adjoint_maps(Device* d, const AdjointMaps& am) {
   return dalloc(d->adj_xs, size_t(3 * am.n_xs_rows));
 }
 
-std::string run_adjoint(Device* d, int mode, const double* g, double* const dparam[3], double* dxyz, void* stream) {
+std::string run_adjoint(Device* d, int mode, const double* g, double* const dparam[3], double* dxyz, void* stream,
+                        bool wide) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipEventRecord(d->ev2, s));
   for (int t = 0; t < 3; ++t) {
@@ -585,7 +586,9 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.dparam = dparam[t];
     A.xs = (dxyz && t != BEAM) ? d->adj_xs + 3 * d->adj_xoff[t] : nullptr;
     A.rel_tol = d->rel_tol;
-    A.g_bytes = (mode == MODE_DLDK && size_t(d->nnz) * 8 < (size_t(1) << 32)) ? uint32_t(size_t(d->nnz) * 8) : 0u;
+    A.g_bytes = (mode == MODE_DLDK && wide && size_t(d->nnz) * 8 < (size_t(1) << 32))
+                    ? uint32_t(size_t(d->nnz) * 8)
+                    : 0u;
     if (t == PLATE) {
       const unsigned blocks = unsigned((n + 63) / 64);
       if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<MODE_DLDK, true><<<blocks, 256, 0, s>>>(A);

@@ -193,7 +193,9 @@ std::string upload_adjoint_maps(Device* d, const AdjointMaps& am);
 // mode 0: g = dL/dK (nnz values, CSR order); mode 1: g = nodal displacements
 // (6 N) and the outputs are -u_e^T (dK_e/dp) u_e.  dparam[t]: n_t x params_per
 // (or null); dxyz: N x 3 (or null; trusses and plates only).
-std::string run_adjoint(Device* d, int mode, const double* g, double* const dparam[3], double* dxyz, void* stream);
+// wide: dL/dK block gathers with 16-byte range-checked loads when the values span under 4 GiB
+std::string run_adjoint(Device* d, int mode, const double* g, double* const dparam[3], double* dxyz, void* stream,
+                        bool wide = true);
 double last_backward_ms(Device* d);
 
 // Device-to-device replacement of one element type's parameters or of the node

@@ -101,6 +101,7 @@ struct fem_model {
   int pf_kernel = 1;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block
                           // (plate_blocks), 2 = one lane per element corner (plate_corners)
   bool force_generic = false;
+  bool adj_wide = true;  // adjoint block gathers: 16-byte loads when the values fit 32-bit offsets
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
   std::vector<uint64_t> pat_bptr;
@@ -1178,7 +1179,7 @@ int run_backward(fem_model* m, int mode, const double* g, double* dp_truss, doub
   int rc = ensure_adjoint(m);
   if (rc) return rc;
   double* dp[3] = {dp_truss, dp_beam, dp_plate};
-  auto e = femdev::run_adjoint(m->dev, mode, g, dp, dxyz, stream);
+  auto e = femdev::run_adjoint(m->dev, mode, g, dp, dxyz, stream, m->adj_wide);
   return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
 }
 
@@ -2064,6 +2065,9 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
       return fail(m, FEM_ERR_ARGUMENT, "separate_lanes must be 0 (automatic), 4, 8, 16 or 32");
     m->sep_lanes = int(value);
     return FEM_OK;  // speed only; the pattern stays
+  } else if (k == "adjoint_wide_loads") {
+    m->adj_wide = value != 0;
+    return FEM_OK;  // speed only
   } else {
     return fail(m, FEM_ERR_ARGUMENT, "unknown option " + k);
   }
