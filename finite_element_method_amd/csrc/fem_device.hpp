@@ -234,8 +234,10 @@ struct SolveResult {
   bool converged = false;
   double residual = 0.0, stop = 0.0;
 };
+// wide: the update loop's SpMV reads four values per lane and step with 16-byte range-checked loads
+// when K_aa's values span under 4 GiB
 std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
-                      SolveResult& res);
+                      SolveResult& res, bool wide = true);
 std::string solution(Device* d, const double** u_a, const double** r_r, int64_t* n_aa, int64_t* n_bb);
 double last_solve_ms(Device* d);
 

@@ -101,7 +101,7 @@ struct fem_model {
   int pf_kernel = 1;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block
                           // (plate_blocks), 2 = one lane per element corner (plate_corners)
   bool force_generic = false;
-  bool adj_wide = true;  // adjoint block gathers: 16-byte loads when the values fit 32-bit offsets
+  bool wide_loads = true;  // backward block gathers and PCG SpMV: 16-byte loads when the values fit 32-bit offsets
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
   std::vector<uint64_t> pat_bptr;
@@ -1179,7 +1179,7 @@ int run_backward(fem_model* m, int mode, const double* g, double* dp_truss, doub
   int rc = ensure_adjoint(m);
   if (rc) return rc;
   double* dp[3] = {dp_truss, dp_beam, dp_plate};
-  auto e = femdev::run_adjoint(m->dev, mode, g, dp, dxyz, stream, m->adj_wide);
+  auto e = femdev::run_adjoint(m->dev, mode, g, dp, dxyz, stream, m->wide_loads);
   return e.empty() ? FEM_OK : fail(m, FEM_ERR_DEVICE, e);
 }
 
@@ -1954,7 +1954,7 @@ int fem_solve_pcg(fem_model* m, int preconditioner, uint64_t max_iter, void* str
     return fail(m, FEM_ERR_ARGUMENT, "preconditioner must be 0 (Jacobi) or 1 (node-block Jacobi)");
   if (max_iter > uint64_t(INT32_MAX)) return fail(m, FEM_ERR_ARGUMENT, "max_iter too large");
   femdev::SolveResult r;
-  auto e = femdev::solve_pcg(m->dev, preconditioner, int(max_iter), m->rel_tol, m->abs_tol, stream, r);
+  auto e = femdev::solve_pcg(m->dev, preconditioner, int(max_iter), m->rel_tol, m->abs_tol, stream, r, m->wide_loads);
   if (!e.empty()) {
     const bool numeric = e.rfind("PCG", 0) == 0;
     return fail(m, numeric ? FEM_ERR_VALIDATION : FEM_ERR_DEVICE, e);
@@ -2065,8 +2065,8 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
       return fail(m, FEM_ERR_ARGUMENT, "separate_lanes must be 0 (automatic), 4, 8, 16 or 32");
     m->sep_lanes = int(value);
     return FEM_OK;  // speed only; the pattern stays
-  } else if (k == "adjoint_wide_loads") {
-    m->adj_wide = value != 0;
+  } else if (k == "wide_loads") {
+    m->wide_loads = value != 0;
     return FEM_OK;  // speed only
   } else {
     return fail(m, FEM_ERR_ARGUMENT, "unknown option " + k);

@@ -65,7 +65,45 @@ __global__ void sv_spmv(const int64_t* __restrict__ rp, const int32_t* __restric
   }
 }
 
-// Ap = K p inside the update loop (returns at once after the stop flag)
+// Ap = K p inside the update loop (returns at once after the stop flag).
+// SPMV_QUAD_LANES lanes per row, each taking four neighbouring values per step
+// with two 16-byte value loads and one 16-byte column load, range-checked
+// buffer loads (32-bit offsets: below 4 GiB of values).  On the plate matrices
+// (rows of 9 to 27 values) this beats one 8-byte load per lane and value by
+// about a fifth: fewer, fuller memory requests.  Fixed summation order (each
+// lane's values in turn, then a butterfly over the row's lanes).
+constexpr int SPMV_QUAD_LANES = 4;
+__global__ void sv_spmv_it_quad(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                const double* __restrict__ v, const double* __restrict__ p, double* __restrict__ ap,
+                                int64_t n, const double* s, uint32_t v_bytes) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  if (s[S_DONE] != 0.0) return;
+  const int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SPMV_QUAD_LANES;
+  const int lig = threadIdx.x & (SPMV_QUAD_LANES - 1);
+  if (r >= n) return;
+  const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(v), 0, int(v_bytes), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rc =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(ci), 0, int(v_bytes / 2u), 0x00020000);
+  const int64_t end = rp[r + 1];
+  double acc = 0.0;
+  for (int64_t e = rp[r] + 4 * lig; e < end; e += 4 * SPMV_QUAD_LANES) {
+    const uint32_t ev = uint32_t(e);
+    // values and columns past the row end (or the array end: read as 0) are
+    // loaded but not used; their columns are valid indices or 0
+    const double2 w0 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rv, 8u * ev, 0, 0));
+    const double2 w1 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rv, 8u * ev + 16u, 0, 0));
+    const u32x4 c = __builtin_amdgcn_raw_buffer_load_b128(rc, 4u * ev, 0, 0);
+    acc = __builtin_fma(w0.x, p[int32_t(c.x)], acc);
+    if (e + 1 < end) acc = __builtin_fma(w0.y, p[int32_t(c.y)], acc);
+    if (e + 2 < end) acc = __builtin_fma(w1.x, p[int32_t(c.z)], acc);
+    if (e + 3 < end) acc = __builtin_fma(w1.y, p[int32_t(c.w)], acc);
+  }
+#pragma unroll
+  for (int o = SPMV_QUAD_LANES / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, SPMV_QUAD_LANES);
+  if (lig == 0) ap[r] = acc;
+}
+
+// the same with one 8-byte load per lane and value (values beyond 4 GiB)
 __global__ void sv_spmv_it(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                            const double* __restrict__ v, const double* __restrict__ p, double* __restrict__ ap,
                            int64_t n, const double* s) {
@@ -374,7 +412,7 @@ unsigned grid_for(int64_t items, int per_block) { return unsigned(std::max<int64
 }  // namespace
 
 std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
-                      SolveResult& res) {
+                      SolveResult& res, bool wide) {
   hipStream_t s = static_cast<hipStream_t>(stream);
   res = SolveResult();
   d->sol_done = false;
@@ -459,12 +497,15 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
     // update raises the stop flag on convergence or at max_iter, and every
     // kernel queued after that returns at once.  The count stays exact and the
     // host waits once per batch, not once per update.
-    const unsigned spmv_blocks = grid_for(na * SPMV_LANES, B);
+    const bool quad = wide && size_t(d->sep_nnz[0]) * 8 < (size_t(1) << 32);
+    const uint32_t v_bytes = quad ? uint32_t(size_t(d->sep_nnz[0]) * 8) : 0u;
+    const unsigned spmv_blocks = grid_for(na * (quad ? SPMV_QUAD_LANES : SPMV_LANES), B);
     int queued = 0, batch = 1;
     for (;;) {
       const int nq = std::min(batch, max_iter - queued);
       for (int q = 0; q < nq; ++q) {
-        sv_spmv_it<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, d->sol_s);
+        if (quad) sv_spmv_it_quad<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, d->sol_s, v_bytes);
+        else sv_spmv_it<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, d->sol_s);
         sv_dot_it<<<SV_PARTS, B, 0, s>>>(p, ap, na, d->sol_s, pa);
         sv_final<<<1, 1024, 0, s>>>(pa, nullptr, d->sol_s, 1, stop, double(max_iter));
         if (precond == 0)

@@ -294,10 +294,13 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
  *   "force_generic"      1 = always use the generic value path
  *   "separate_lanes"     lanes per CSR row in fem_separate: 4, 8, 16, 32
  *                        (0 = automatic, from the mean row length)
- *   "adjoint_wide_loads" 1 (default) = the backward reads each row run of a
- *                        dL/dK block with 16-byte range-checked loads when the
- *                        values span under 4 GiB; 0 = one 8-byte load per value
- *                        (the path taken above 4 GiB)
+ *   "wide_loads"         1 (default) = where the values span under 4 GiB,
+ *                        the backward reads each row run of a dL/dK block, and
+ *                        the PCG's SpMV four values per lane, with 16-byte
+ *                        range-checked loads; 0 = one 8-byte load per value
+ *                        (the paths taken above 4 GiB).  Gradients are bitwise
+ *                        equal either way; PCG iterates differ by rounding (the
+ *                        SpMV sums a row in another order)
  * Setting a plate or path option invalidates the pattern (rebuilt by the next
  * assembly). */
 int fem_set_option(fem_model* m, const char* key, int64_t value);

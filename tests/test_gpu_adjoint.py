@@ -59,7 +59,7 @@ def test_wide_and_narrow_gathers_agree(gpu, kind):
     m, rp, ci, g = _setup(mesh, 5)
     xyz = kind != "beam"
     wide = m.backward(_cuda(g), xyz=xyz)
-    m.set_option("adjoint_wide_loads", 0)
+    m.set_option("wide_loads", 0)
     narrow = m.backward(_cuda(g), xyz=xyz)
     for t in (TRUSS, BEAM, PLATE):
         a, b = wide[0].get(t), narrow[0].get(t)

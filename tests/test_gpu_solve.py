@@ -116,3 +116,34 @@ def test_solve_reproducible_and_equilibrium(gpu):
     K = sp.csr_matrix((mats["aa"][2], mats["aa"][1], mats["aa"][0]), shape=(len(aa), len(aa)))
     res = np.abs(b - K @ ua).sum()
     assert abs(rz + fz) <= res + 1e-9 * abs(fz)
+
+
+def test_narrow_spmv_matches_wide(gpu):
+    """The update loop's SpMV reads four values per lane with 16-byte loads
+    below 4 GiB of K_aa values and one value per load above; forced onto the
+    second path (option wide_loads = 0) the solve converges to the same
+    solution: both within the accuracy of the oracle's own PCG of the direct
+    solution (the two sum each row in another order, so the iterates differ by
+    rounding)."""
+    mesh = meshgen.plate_grid(48, seed=6)
+    m = FEM.from_mesh(mesh)
+    rp, ci, v = m.stiffness_csr()
+    diag = bc_cases.diagonal(rp, ci, v)
+    disp = bc_cases.clamp_face(mesh, diag, axis=1)
+    m.add_displacements(*disp)
+    rest = np.setdiff1d(np.arange(mesh.n_nodes), np.nonzero(np.isclose(mesh.xyz[:, 1], 0.0))[0])
+    m.add_concentrated_loads(mesh.node_numbers[rest], np.full(len(rest), 2), np.full(len(rest), -5.0))
+    mats, aa, bb, ra, ub = m.separate_sparse(assemble=False)
+    K = sp.csr_matrix((mats["aa"][2], mats["aa"][1], mats["aa"][0]), shape=(len(aa), len(aa)))
+    b = find_b_sparse(ra, mats["ab"], ub)
+    exact = spla.spsolve(K.tocsc(), b)
+    ref, it_ref, ok = pcg(mats["aa"], b, 20000, m.rel_tol, m.abs_tol, block_starts(aa))
+    assert ok
+    err_ref = np.linalg.norm(ref - exact)
+    wide = m.solve_pcg("block_jacobi", 20000)
+    m.set_option("wide_loads", 0)
+    narrow = m.solve_pcg("block_jacobi", 20000)
+    print("wide", wide[2], "narrow", narrow[2], "oracle", it_ref, flush=True)
+    for ua, rr, it, conv in (wide, narrow):
+        assert conv
+        assert np.linalg.norm(ua - exact) <= 2.0 * err_ref + 1e-12 * np.linalg.norm(exact)
