@@ -65,8 +65,9 @@ struct Device {
   int64_t *sep_rp = nullptr, *sep_sum = nullptr;
   int32_t* sep_ci = nullptr;
   int64_t* sep_dpos = nullptr;  // diagonal entry of each row (pattern-only)
+  uint32_t* sep_pack = nullptr;  // per dof: class << 30 | its index among the a (or b) dofs
   bool sep_dpos_valid = false;
-  size_t sep_cap[19] = {};
+  size_t sep_cap[20] = {};
   int64_t sep_bc_rows = -1;
   int64_t sep_rows[4] = {0, 0, 0, 0}, sep_nnz[4] = {0, 0, 0, 0}, sep_base[4] = {0, 0, 0, 0};
   bool sep_done = false;

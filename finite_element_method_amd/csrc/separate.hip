@@ -178,6 +178,145 @@ __global__ void sep_fill(const int64_t* __restrict__ rp, const int32_t* __restri
   }
 }
 
+// ---- four values per lane (values under 4 GiB, the default)
+// One 4-byte word per dof replaces the class byte and the 8-byte scan position
+// that the column gathers above read: class << 30 | the dof's index among the
+// a dofs (class 1) or the b dofs (class 2); 0 for an excluded dof.
+constexpr uint32_t SEP_IDX = (1u << 30) - 1u;
+
+__global__ void sep_pack(const uint8_t* __restrict__ cls, const uint64_t* __restrict__ pos, int64_t n,
+                         uint32_t* __restrict__ pack) {
+  const int64_t r = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const uint32_t k = cls[r];
+  pack[r] = k == 0 ? 0u : (k << 30) | (k == 1 ? lo32(pos[r]) : hi32(pos[r]));
+}
+
+// 4 lanes per row, each taking four neighbouring entries per step: two
+// 16-byte value loads and one 16-byte column load, range-checked buffer loads
+// (entries past the row end are loaded but not used; their columns are valid
+// indices or 0), and one 4-byte gather per non-zero entry.
+constexpr int SEP_QL = 4;
+typedef unsigned int sep_u32x4 __attribute__((ext_vector_type(4)));
+
+__device__ inline void sep_load4(__amdgpu_buffer_rsrc_t rv, __amdgpu_buffer_rsrc_t rc, int64_t e, double w[4],
+                                 uint32_t c[4]) {
+  const uint32_t ev = uint32_t(e);
+  const double2 w0 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rv, 8u * ev, 0, 0));
+  const double2 w1 = __builtin_bit_cast(double2, __builtin_amdgcn_raw_buffer_load_b128(rv, 8u * ev + 16u, 0, 0));
+  const sep_u32x4 cc = __builtin_amdgcn_raw_buffer_load_b128(rc, 4u * ev, 0, 0);
+  w[0] = w0.x; w[1] = w0.y; w[2] = w1.x; w[3] = w1.y;
+  c[0] = cc.x; c[1] = cc.y; c[2] = cc.z; c[3] = cc.w;
+}
+
+__global__ void sep_count_quad(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                               const double* __restrict__ val, const uint32_t* __restrict__ pack,
+                               const uint64_t* __restrict__ pos, const double* __restrict__ force,
+                               const double* __restrict__ disp, int64_t n, int64_t* __restrict__ cnt,
+                               int64_t* __restrict__ aa_idx, int64_t* __restrict__ bb_idx, double* __restrict__ r_a,
+                               double* __restrict__ u_b, uint32_t v_bytes) {
+  const int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SEP_QL;
+  const int lig = threadIdx.x & (SEP_QL - 1);
+  if (r >= n) return;
+  const uint32_t pr = pack[r];
+  const uint32_t k = pr >> 30;
+  if (k == 0) return;
+  const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(val), 0, int(v_bytes), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rc =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(ci), 0, int(v_bytes / 2u), 0x00020000);
+  const int64_t end = rp[r + 1];
+  int ca = 0, cb = 0;
+  for (int64_t e = rp[r] + 4 * lig; e < end; e += 4 * SEP_QL) {
+    double w[4];
+    uint32_t c[4];
+    sep_load4(rv, rc, e, w, c);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t kc = (e + j < end && w[j] != 0.0) ? pack[c[j]] >> 30 : 0u;
+      ca += kc == 1u;
+      cb += kc == 2u;
+    }
+  }
+  ca = group_sum<SEP_QL>(ca);
+  cb = group_sum<SEP_QL>(cb);
+  if (lig != 0) return;
+  const int64_t na = lo32(pos[n]), nb = hi32(pos[n]);
+  const int64_t pi = int64_t(pr & SEP_IDX);
+  if (k == 1) {
+    cnt[pi] = ca;
+    cnt[na + pi] = cb;
+    aa_idx[pi] = r;
+    r_a[pi] = force[r];
+  } else {
+    cnt[2 * na + pi] = ca;
+    cnt[2 * na + nb + pi] = cb;
+    bb_idx[pi] = r;
+    u_b[pi] = disp[r];
+  }
+}
+
+// Output slots in entry order: an exclusive scan of the per-lane counts over
+// the row's four lanes (lane l holds entries 4l .. 4l + 3 of the step), then
+// each lane's entries in turn.
+__global__ void sep_fill_quad(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                              const double* __restrict__ val, const uint32_t* __restrict__ pack,
+                              const uint64_t* __restrict__ pos, const int64_t* __restrict__ off, int64_t n,
+                              int32_t* __restrict__ out_ci, double* __restrict__ out_v, uint32_t v_bytes) {
+  const int64_t r = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) / SEP_QL;
+  const int lig = threadIdx.x & (SEP_QL - 1);
+  if (r >= n) return;
+  const uint32_t pr = pack[r];
+  const uint32_t k = pr >> 30;
+  if (k == 0) return;
+  const int64_t na = lo32(pos[n]), nb = hi32(pos[n]);
+  const int64_t pi = int64_t(pr & SEP_IDX);
+  int64_t o_a = k == 1 ? off[pi] : off[2 * na + pi];            // next slot of an entry in an a column
+  int64_t o_b = k == 1 ? off[na + pi] : off[2 * na + nb + pi];  // ... in a b column
+  const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(val), 0, int(v_bytes), 0x00020000);
+  const __amdgpu_buffer_rsrc_t rc =
+      __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(ci), 0, int(v_bytes / 2u), 0x00020000);
+  const int64_t end = rp[r + 1];
+  for (int64_t base = rp[r]; base < end; base += 4 * SEP_QL) {
+    const int64_t e = base + 4 * lig;
+    double w[4];
+    uint32_t c[4], q[4];
+    sep_load4(rv, rc, e, w, c);
+    int la = 0, lb = 0;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      q[j] = (e + j < end && w[j] != 0.0) ? pack[c[j]] : 0u;
+      la += (q[j] >> 30) == 1u;
+      lb += (q[j] >> 30) == 2u;
+    }
+    int sa = la, sb = lb;  // inclusive scan over the row's lanes
+#pragma unroll
+    for (int o = 1; o < SEP_QL; o <<= 1) {
+      const int ta = __shfl_up(sa, o, SEP_QL), tb = __shfl_up(sb, o, SEP_QL);
+      if (lig >= o) {
+        sa += ta;
+        sb += tb;
+      }
+    }
+    const int tot_a = __shfl(sa, SEP_QL - 1, SEP_QL), tot_b = __shfl(sb, SEP_QL - 1, SEP_QL);
+    int64_t ia = o_a + (sa - la), ib = o_b + (sb - lb);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t kc = q[j] >> 30;
+      if (kc == 1u) {
+        out_ci[ia] = int32_t(q[j] & SEP_IDX);
+        out_v[ia] = w[j];
+        ++ia;
+      } else if (kc == 2u) {
+        out_ci[ib] = int32_t(q[j] & SEP_IDX);
+        out_v[ib] = w[j];
+        ++ib;
+      }
+    }
+    o_a += tot_a;
+    o_b += tot_b;
+  }
+}
+
 // Matrix w's row pointers start at out_rp[start_w + w] (rows_w + 1 entries, first 0).
 __global__ void sep_finish(const uint64_t* __restrict__ pos, const int64_t* __restrict__ off,
                            const int64_t* __restrict__ err, int64_t n, int64_t* __restrict__ out_rp,
@@ -262,6 +401,13 @@ std::string separate_sparse(Device* d, SepResult& res, void* stream, bool timed,
   if (timed) HIP_TRY(hipEventRecord(d->ev4, s));
   // lanes per row: 8 by default (4 for rows shorter than 8 entries on average);
   // measured faster than 4, 16 or 32 on plate, frame and lattice meshes
+  // default: four entries per lane (sep_*_quad) while the values span under
+  // 4 GiB; an explicit lanes option (or larger values) takes the one-entry
+  // kernels
+  const bool quad = lanes_opt == 0 && size_t(d->nnz) * 8 < (size_t(1) << 32);
+  const uint32_t v_bytes = quad ? uint32_t(size_t(d->nnz) * 8) : 0u;
+  if (quad && !(e = dgrow(d->sep_pack, d->sep_cap[19], un)).empty()) return e;
+  const unsigned GQ = unsigned((n * SEP_QL + B - 1) / B);
   const int lanes = lanes_opt ? lanes_opt : (d->nnz / n >= 8 ? 8 : 4);
   const unsigned G = unsigned((n * lanes + B - 1) / B);
   auto count = lanes == 32 ? sep_count<32> : lanes == 16 ? sep_count<16> : lanes == 8 ? sep_count<8> : sep_count<4>;
@@ -271,11 +417,21 @@ std::string separate_sparse(Device* d, SepResult& res, void* stream, bool timed,
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipcub::DeviceReduce::Min(d->sep_tmp, b0, d->sep_key, d->sep_sum + SEP_SUMMARY, int(n), s));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d->sep_tmp, b1, d->sep_flag, d->sep_pos, int(n + 1), s));
-  count<<<G, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_cls, d->sep_pos, d->sep_force, d->sep_disp, n,
-                        d->sep_cnt, d->sep_aa, d->sep_bb, d->sep_ra, d->sep_ub);
+  if (quad) {
+    sep_pack<<<unsigned((n + B - 1) / B), B, 0, s>>>(d->sep_cls, d->sep_pos, n, d->sep_pack);
+    sep_count_quad<<<GQ, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_pack, d->sep_pos, d->sep_force,
+                                    d->sep_disp, n, d->sep_cnt, d->sep_aa, d->sep_bb, d->sep_ra, d->sep_ub, v_bytes);
+  } else {
+    count<<<G, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_cls, d->sep_pos, d->sep_force, d->sep_disp, n,
+                          d->sep_cnt, d->sep_aa, d->sep_bb, d->sep_ra, d->sep_ub);
+  }
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d->sep_tmp, b2, d->sep_cnt, d->sep_off, int(2 * n + 1), s));
-  fill<<<G, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_cls, d->sep_pos, d->sep_off, n, d->sep_ci, d->sep_v);
+  if (quad)
+    sep_fill_quad<<<GQ, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_pack, d->sep_pos, d->sep_off, n, d->sep_ci,
+                                   d->sep_v, v_bytes);
+  else
+    fill<<<G, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_cls, d->sep_pos, d->sep_off, n, d->sep_ci, d->sep_v);
   HIP_TRY(hipGetLastError());
   const unsigned GF = unsigned((2 * n + 4 + B - 1) / B);
   sep_finish<<<GF, B, 0, s>>>(d->sep_pos, d->sep_off, d->sep_sum + SEP_SUMMARY, n, d->sep_rp, d->sep_sum);

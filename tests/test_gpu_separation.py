@@ -68,10 +68,14 @@ def _global_row_max(rp, v):
     return rm
 
 
+@pytest.mark.parametrize("lanes", [0, 8])
 @pytest.mark.parametrize("name", sorted(CASES))
-def test_separation_matches_oracle(gpu, name):
+def test_separation_matches_oracle(gpu, name, lanes):
+    """lanes 0: the default kernels (four entries per lane, one packed word per
+    dof); 8: the one-entry-per-lane kernels, taken with an explicit lanes
+    option or above 4 GiB of values."""
     mesh = CASES[name]()
-    print("case", name, mesh.n_nodes, mesh.n_elements, flush=True)
+    print("case", name, mesh.n_nodes, mesh.n_elements, "lanes", lanes, flush=True)
     om = meshgen.load_oracle(mesh, OracleModel)
     k_ref = om.csr()
     diag = bc_cases.diagonal(*k_ref)
@@ -79,6 +83,8 @@ def test_separation_matches_oracle(gpu, name):
     disp, loads = bc_cases.random_bc(mesh, diag, seed=len(name))
     bc_cases.apply(om, disp, loads)
     m = FEM.from_mesh(mesh)
+    if lanes:
+        m.set_option("separate_lanes", lanes)
     bc_cases.apply(m, disp, loads)
     got = m.separate_sparse()
     print(name, m.assembly_path(), "n_aa", len(got[1]), "n_bb", len(got[2]), "ms", m.last_separate_ms(), flush=True)
