@@ -1,7 +1,11 @@
 // bar_owner.hip -- one-pass assembly of truss and beam meshes.
 //
+// The host cuts the nodes into ranges of consecutive nodes.  Ranges whose
+// nodes touch trusses only go to truss_owner_kernel (below: a compact entry
+// record, rows 0..2 only); the others to bar_owner_kernel:
+//
 // One workgroup per range of owned nodes (<= 42 nodes, <= 256 incident
-// (node, element) entries, cut on the host).  Phase 1: one lane per entry
+// (node, element) entries, <= 4096 CSR values).  Phase 1: one lane per entry
 // stages the element's constants and its blocks' masks and row offsets in LDS,
 // all global loads in parallel.  Phase 2: one lane per (owned node a, CSR row
 // i) walks a's entries in element-insertion order and computes, for each, row
