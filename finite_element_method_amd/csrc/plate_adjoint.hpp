@@ -19,102 +19,15 @@
 
 namespace fem {
 
-// Reverse sweep of one integration point.  Adds to gxl the derivative with
-// respect to xl of
+// Reverse sweep of one integration point: the derivative with respect to xl of
 //   l = sum_q w [dx_p (cxx_q dx_q + cxy_q dy_q) + dy_p (cyx_q dx_q + cyy_q dy_q)
 //               + al_p cA_q al_q + be_p cB_q be_q]
 // with c = {cxx[4], cxy[4], cyx[4], cyy[4], cA[4], cB[4]} held fixed, where
-// w, dx, dy, al, be are the per-point data of plate_point (w = det J).
-__host__ __device__ inline void plate_point_adjoint(const double xl[6], int ip, int p, const double c[24],
-                                                    double gxl[6]) {
-  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
-  const double g = sqrt(double(ONE_THIRD_F32));
-  const double r = (ip == 0 || ip == 3) ? g : -g;
-  const double s = (ip < 2) ? g : -g;
-  const double q4 = 0.25;
-  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
-  // Jacobian rows: (j00, j01) = us/4, (j10, j11) = ur/4 with
-  //   us = x1 (1+s) - x2 (1+s) + x4 (1-s),  ur = x1 (1+r) + x2 (1-r) - x4 (1+r)
-  const double usx = (x1 * ps - x2 * ps) + x4 * ms_, usy = (y1 * ps - y2 * ps) + y4 * ms_;
-  const double urx = (x1 * pr + x2 * mr_) - x4 * pr, ury = (y1 * pr + y2 * mr_) - y4 * pr;
-  const double j00 = usx * q4, j01 = usy * q4, j10 = urx * q4, j11 = ury * q4;
-  const double det = j00 * j11 - j01 * j10;
-  const double rdet = 1.0 / det;
-  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
-  const double dr[4] = {q4 + q4 * s, -(q4 + q4 * s), -(q4 - q4 * s), q4 - q4 * s};
-  const double ds[4] = {q4 + q4 * r, q4 - q4 * r, -(q4 - q4 * r), -(q4 + q4 * r)};
-  double dx[4], dy[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    dx[a] = i00 * dr[a] + i01 * ds[a];
-    dy[a] = i10 * dr[a] + i11 * ds[a];
-  }
-  // MITC4 multipliers: mr = |ur| / (8 det), ms = |us| / (8 det) (plate.rs:424-447)
-  const double sqr = sqrt(urx * urx + ury * ury), sqs = sqrt(usx * usx + usy * usy);
-  const double m_r = sqr * 0.125 * rdet, m_s = sqs * 0.125 * rdet;
-  const double fa[4] = {ps, ps, ms_, ms_}, fb[4] = {pr, mr_, mr_, pr};
-  double al[4], be[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    al[a] = fa[a] * m_r;
-    be[a] = fb[a] * m_s;
-  }
-  const double dxp = sel4(dx[0], dx[1], dx[2], dx[3], p), dyp = sel4(dy[0], dy[1], dy[2], dy[3], p);
-  const double alp = sel4(al[0], al[1], al[2], al[3], p), bep = sel4(be[0], be[1], be[2], be[3], p);
-  double SX = 0.0, SY = 0.0, SA = 0.0, SB = 0.0;
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    SX += c[q] * dx[q] + c[4 + q] * dy[q];
-    SY += c[8 + q] * dx[q] + c[12 + q] * dy[q];
-    SA += c[16 + q] * al[q];
-    SB += c[20 + q] * be[q];
-  }
-  // adjoints of the point data
-  double g_det = ((dxp * SX + dyp * SY) + alp * SA) + bep * SB;
-  double g_i00 = 0.0, g_i01 = 0.0, g_i10 = 0.0, g_i11 = 0.0, g_mr = 0.0, g_ms = 0.0;
-#pragma unroll
-  for (int a = 0; a < 4; ++a) {
-    const bool self = a == p;
-    const double gdx = det * ((dxp * c[a] + dyp * c[8 + a]) + (self ? SX : 0.0));
-    const double gdy = det * ((dxp * c[4 + a] + dyp * c[12 + a]) + (self ? SY : 0.0));
-    const double gal = det * (alp * c[16 + a] + (self ? SA : 0.0));
-    const double gbe = det * (bep * c[20 + a] + (self ? SB : 0.0));
-    g_i00 += gdx * dr[a];
-    g_i01 += gdx * ds[a];
-    g_i10 += gdy * dr[a];
-    g_i11 += gdy * ds[a];
-    g_mr += gal * fa[a];
-    g_ms += gbe * fb[a];
-  }
-  // m_r = sqr / (8 det), m_s = sqs / (8 det)
-  g_det -= (g_mr * m_r + g_ms * m_s) * rdet;
-  const double g_sqr = g_mr * 0.125 * rdet, g_sqs = g_ms * 0.125 * rdet;
-  // inverse Jacobian
-  double g_j11 = g_i00 * rdet, g_j01 = -g_i01 * rdet, g_j10 = -g_i10 * rdet, g_j00 = g_i11 * rdet;
-  g_det -= (((g_i00 * i00 + g_i01 * i01) + g_i10 * i10) + g_i11 * i11) * rdet;
-  // det = j00 j11 - j01 j10
-  g_j00 += g_det * j11;
-  g_j11 += g_det * j00;
-  g_j01 -= g_det * j10;
-  g_j10 -= g_det * j01;
-  // back to the linear forms us, ur (sqrt terms: d|v| = v/|v|)
-  const double kux = g_j00 * q4 + (sqs > 0.0 ? g_sqs * usx / sqs : 0.0);
-  const double kuy = g_j01 * q4 + (sqs > 0.0 ? g_sqs * usy / sqs : 0.0);
-  const double krx = g_j10 * q4 + (sqr > 0.0 ? g_sqr * urx / sqr : 0.0);
-  const double kry = g_j11 * q4 + (sqr > 0.0 ? g_sqr * ury / sqr : 0.0);
-  gxl[0] += kux * ps + krx * pr;
-  gxl[1] += kuy * ps + kry * pr;
-  gxl[2] += -kux * ps + krx * mr_;
-  gxl[3] += -kuy * ps + kry * mr_;
-  gxl[4] += kux * ms_ - krx * pr;
-  gxl[5] += kuy * ms_ - kry * pr;
-}
-
-// The same sweep in two parts.  The second part is linear in the seven
-// adjoints of the point's forward quantities (det, the inverse Jacobian, the
-// two MITC4 multipliers), so the four row lanes of an element can add their
-// seven values first and run the second part once per point.
-//
+// w, dx, dy, al, be are the per-point data of plate_point (w = det J).  It
+// runs in two parts.  The second is linear in the seven adjoints of the
+// point's forward quantities (det, the inverse Jacobian, the two MITC4
+// multipliers), so the four row lanes of an element add their seven values
+// first and run the second part once per point.
 // Part 1: the seven adjoints s7 = (g_det, g_i00, g_i01, g_i10, g_i11, g_mr,
 // g_ms) of row p at point ip, from that point's data g (plate_point).
 __host__ __device__ inline void plate_point_scalars(const PlatePoint& g, int ip, int p, const double c[24],
@@ -236,7 +149,7 @@ __host__ __device__ inline void plate_frame_adjoint_sub(const double* k, const d
 // integrals of column q (Mxx, Mxy, Myx, Myy, A, B; plate_row_accumulate),
 // vp/up and vq/uq: the unscaled shear vectors of nodes p and q, gl: the
 // local-frame gradient block (p, q), 6x6 row-major.  cq receives the integral
-// coefficients of column q (layout of plate_point_adjoint: cxx, cxy, cyx, cyy,
+// coefficients of column q (layout of plate_point_scalars: cxx, cxy, cyx, cyy,
 // cA, cB).
 __host__ __device__ inline void plate_block_adjoint_q(const double Mq[6], const PlateMat& m, const double vp[3],
                                                       const double up[3], const double vq[3], const double uq[3],
