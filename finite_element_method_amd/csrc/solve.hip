@@ -213,9 +213,16 @@ __global__ void sv_block_starts(const uint32_t* __restrict__ flag, const uint32_
 }
 
 // inverse of each node block (Gauss-Jordan, partial pivoting); status bit 1 on a singular block
-__global__ void sv_block_inverse(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
-                                 const double* __restrict__ v, const int32_t* __restrict__ starts, int64_t nblk,
-                                 double* __restrict__ inv, double* s) {
+__global__ __launch_bounds__(256) void sv_block_inverse(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
+                                                        const double* __restrict__ v,
+                                                        const int32_t* __restrict__ starts, int64_t nblk,
+                                                        double* __restrict__ inv, double* s) {
+  // The augmented block [A | I] lives in registers: every loop below runs over
+  // compile-time indices, so row swaps and column placement are selects (a
+  // runtime-indexed array would sit in scratch memory).  A block of m < 6 dofs
+  // is padded with the identity: its rows and columns past m never become
+  // pivots or elimination targets of the first m columns, so the m x m inverse
+  // is the one the unpadded elimination gives.
   const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= nblk) return;
   const int s0 = starts[b], m = starts[b + 1] - s0;
@@ -226,33 +233,65 @@ __global__ void sv_block_inverse(const int64_t* __restrict__ rp, const int32_t* 
     return;
   }
   double a[BJ_MAX][2 * BJ_MAX];
-  for (int i = 0; i < BJ_MAX; ++i)
-    for (int j = 0; j < 2 * BJ_MAX; ++j) a[i][j] = 0.0;
-  for (int i = 0; i < m; ++i) {
-    for (int64_t e = rp[s0 + i]; e < rp[s0 + i + 1]; ++e) {
-      const int c = ci[e] - s0;
-      if (c >= 0 && c < m) a[i][c] = v[e];
+#pragma unroll
+  for (int i = 0; i < BJ_MAX; ++i) {
+#pragma unroll
+    for (int j = 0; j < 2 * BJ_MAX; ++j) a[i][j] = (j == BJ_MAX + i || (i >= m && j == i)) ? 1.0 : 0.0;
+    if (i < m) {
+      // the block's columns s0 .. s0 + m - 1 are one run of the sorted row:
+      // find its start by bisection, then place at most m entries
+      int64_t lo = rp[s0 + i], hi = rp[s0 + i + 1];
+      const int64_t end = hi;
+      while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (ci[mid] < s0) lo = mid + 1; else hi = mid;
+      }
+      for (int64_t e = lo; e < end; ++e) {
+        const int c = ci[e] - s0;
+        if (c >= m) break;
+        const double x = v[e];
+#pragma unroll
+        for (int j = 0; j < BJ_MAX; ++j) a[i][j] = c == j ? x : a[i][j];
+      }
     }
-    a[i][BJ_MAX + i] = 1.0;
   }
   bool ok = true;
-  for (int c = 0; c < m && ok; ++c) {
+#pragma unroll
+  for (int c = 0; c < BJ_MAX; ++c) {
+    // partial pivoting: the first row of largest |a[i][c]|, i >= c
     int piv = c;
-    for (int i = c + 1; i < m; ++i)
-      if (fabs(a[i][c]) > fabs(a[piv][c])) piv = i;
-    if (a[piv][c] == 0.0) { ok = false; break; }
-    if (piv != c)
-      for (int j = 0; j < 2 * BJ_MAX; ++j) { const double t = a[c][j]; a[c][j] = a[piv][j]; a[piv][j] = t; }
+    double best = fabs(a[c][c]);
+#pragma unroll
+    for (int i = c + 1; i < BJ_MAX; ++i)
+      if (fabs(a[i][c]) > best) {
+        piv = i;
+        best = fabs(a[i][c]);
+      }
+    ok = ok && best != 0.0;
+#pragma unroll
+    for (int i = c + 1; i < BJ_MAX; ++i) {
+      const bool sw = piv == i;
+#pragma unroll
+      for (int j = 0; j < 2 * BJ_MAX; ++j) {
+        const double t = a[c][j];
+        a[c][j] = sw ? a[i][j] : t;
+        a[i][j] = sw ? t : a[i][j];
+      }
+    }
     const double d = a[c][c];
+#pragma unroll
     for (int j = 0; j < 2 * BJ_MAX; ++j) a[c][j] /= d;
-    for (int i = 0; i < m; ++i)
+#pragma unroll
+    for (int i = 0; i < BJ_MAX; ++i)
       if (i != c) {
         const double f = a[i][c];
-        if (f != 0.0)
-          for (int j = 0; j < 2 * BJ_MAX; ++j) a[i][j] -= f * a[c][j];
+#pragma unroll
+        for (int j = 0; j < 2 * BJ_MAX; ++j) a[i][j] = f != 0.0 ? a[i][j] - f * a[c][j] : a[i][j];
       }
   }
+#pragma unroll
   for (int i = 0; i < BJ_MAX; ++i)
+#pragma unroll
     for (int j = 0; j < BJ_MAX; ++j) o[6 * i + j] = (i < m && j < m) ? a[i][BJ_MAX + j] : 0.0;
   if (!ok) s[S_STATUS] = 1.0;  // any writer stores the same value
 }
