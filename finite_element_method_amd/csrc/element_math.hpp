@@ -224,10 +224,24 @@ __host__ __device__ inline PlateMat plate_material(double E, double nu, double t
 // Local-frame 6x6 block (p, q) of the plate stiffness accumulated over one
 // integration point: membrane (u, v), bending (theta_x, theta_y) and MITC4
 // shear (w, theta_x, theta_y) terms of plate.rs:513-672.
+__host__ __device__ inline double sel4(double a, double b, double c, double d, int p) {
+  return p == 0 ? a : (p == 1 ? b : (p == 2 ? c : d));
+}
+
+// Node data are picked with selects, not by runtime index: p and q vary per
+// thread, and an array indexed at run time would sit in scratch memory.
 __host__ __device__ inline void plate_block_add_ip(const PlateIP& g, const PlateMat& m, int p, int q,
                                                    double kb[36]) {
   const double w = g.w;
-  const double dxp = g.dx[p], dyp = g.dy[p], dxq = g.dx[q], dyq = g.dy[q];
+  const double dxp = sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], p), dyp = sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], p);
+  const double dxq = sel4(g.dx[0], g.dx[1], g.dx[2], g.dx[3], q), dyq = sel4(g.dy[0], g.dy[1], g.dy[2], g.dy[3], q);
+  double srp[3], srq[3], ssp[3], ssq[3];
+  for (int i = 0; i < 3; ++i) {
+    srp[i] = sel4(g.sr[0][i], g.sr[1][i], g.sr[2][i], g.sr[3][i], p);
+    srq[i] = sel4(g.sr[0][i], g.sr[1][i], g.sr[2][i], g.sr[3][i], q);
+    ssp[i] = sel4(g.ss[0][i], g.ss[1][i], g.ss[2][i], g.ss[3][i], p);
+    ssq[i] = sel4(g.ss[0][i], g.ss[1][i], g.ss[2][i], g.ss[3][i], q);
+  }
   // membrane: B_mem columns u_a = (dx_a, 0, dy_a), v_a = (0, dy_a, dx_a)
   double muu = m.cm00 * dxp * dxq + m.cm22 * dyp * dyq;
   double muv = m.cm01 * dxp * dyq + m.cm22 * dyp * dxq;
@@ -244,7 +258,7 @@ __host__ __device__ inline void plate_block_add_ip(const PlateIP& g, const Plate
   const double cs = m.cs;
   for (int i = 0; i < 3; ++i)
     for (int j = 0; j < 3; ++j) {
-      double sh = cs * (g.sr[p][i] * g.sr[q][j] + g.ss[p][i] * g.ss[q][j]);
+      double sh = cs * (srp[i] * srq[j] + ssp[i] * ssq[j]);
       double be = 0.0;
       if (i == 1 && j == 1) be = bxx;
       else if (i == 1 && j == 2) be = bxy;
@@ -252,10 +266,6 @@ __host__ __device__ inline void plate_block_add_ip(const PlateIP& g, const Plate
       else if (i == 2 && j == 2) be = byy;
       kb[(2 + i) * 6 + 2 + j] += be * w + sh * w;
     }
-}
-
-__host__ __device__ inline double sel4(double a, double b, double c, double d, int p) {
-  return p == 0 ? a : (p == 1 ? b : (p == 2 ? c : d));
 }
 
 // Local-frame row block of node p (columns of all four nodes q) of a flat
