@@ -22,6 +22,7 @@
 #include "device_state.hpp"
 #include "element_math.hpp"
 #include "fem_device.hpp"
+#include "lane_ops.hpp"
 #include "plate_adjoint.hpp"
 
 namespace femdev {
@@ -443,16 +444,6 @@ __global__ __launch_bounds__(256) void truss_adjoint_kernel(AdjArgs A) {
 }
 
 // ---------------------------------------------------------------- beams
-// every lane of a quad receives lane L's value
-template <int L>
-__device__ inline double quad_bcast(double v) {
-  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-  const u32x2 x = __builtin_bit_cast(u32x2, v);
-  u32x2 r;
-  r.x = uint32_t(__builtin_amdgcn_mov_dpp(int(x.x), L * 0x55, 0xF, 0xF, false));
-  r.y = uint32_t(__builtin_amdgcn_mov_dpp(int(x.y), L * 0x55, 0xF, 0xF, false));
-  return __builtin_bit_cast(double, r);
-}
 
 // dL/dK: four lanes per beam, lane pq = block (p, q), so the four blocks'
 // scattered gathers run in parallel; lane 0 adds the term sums in block order

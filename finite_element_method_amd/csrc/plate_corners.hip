@@ -26,6 +26,7 @@
 #include "device_state.hpp"
 #include "element_math.hpp"
 #include "fem_device.hpp"
+#include "lane_ops.hpp"
 #include "plate_layout.hpp"
 
 namespace femdev {
@@ -57,16 +58,6 @@ struct CornerArgs {
   double* __restrict__ val;
   uint32_t val_bytes;
 };
-
-template <int L>
-__device__ inline double quad_bcast(double v) {
-  // every lane of a quad receives lane L's value
-  const u32x2 x = __builtin_bit_cast(u32x2, v);
-  u32x2 r;
-  r.x = uint32_t(__builtin_amdgcn_mov_dpp(int(x.x), L * 0x55, 0xF, 0xF, false));
-  r.y = uint32_t(__builtin_amdgcn_mov_dpp(int(x.y), L * 0x55, 0xF, 0xF, false));
-  return __builtin_bit_cast(double, r);
-}
 
 __device__ inline double lane_pull(double v, int byte_addr) {
   const u32x2 x = __builtin_bit_cast(u32x2, v);
