@@ -24,7 +24,7 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the forward value kernel, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE counter runs of this script (scaled by a 1 GiB copy kernel that reads
 # FETCH_SIZE = 0.5 x its bytes and WRITE_SIZE = 1.0 x)
-MEASURED_TRAFFIC = {("plate_512", "plate_blocks"): 348.3e6}
+MEASURED_TRAFFIC = {("plate_512", "plate_blocks"): 348.4e6}
 
 WORKLOADS = {
     "plate_512": ("plate_grid", dict(nx=512, seed=0)),
