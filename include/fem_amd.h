@@ -250,7 +250,7 @@ int fem_global_result(fem_model* m, uint32_t* node_numbers, int32_t* dofs, doubl
  * Many independent meshes assembled by one hipGraph launch.  The meshes'
  * nodes and elements are copied (at create) into per-group disjoint-union
  * models: flat plate-only meshes (fused plate kernel), truss and beam meshes
- * (node-owner kernel) and all others (generic path); the input handles are
+ * (node-owner kernels) and all others (generic path); the input handles are
  * not modified and may be destroyed afterwards.  Each mesh keeps its own CSR:
  * its rows and values are contiguous slices of its group's CSR, and every
  * value is summed in the mesh's own element order by the same value path as
