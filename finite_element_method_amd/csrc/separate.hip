@@ -275,6 +275,9 @@ __global__ void sep_fill_quad(const int64_t* __restrict__ rp, const int32_t* __r
   const __amdgpu_buffer_rsrc_t rv = __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(val), 0, int(v_bytes), 0x00020000);
   const __amdgpu_buffer_rsrc_t rc =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(ci), 0, int(v_bytes / 2u), 0x00020000);
+  // outputs: the four matrices' values and columns together hold at most nnz entries
+  const __amdgpu_buffer_rsrc_t ov = __builtin_amdgcn_make_buffer_rsrc(out_v, 0, int(v_bytes), 0x00020000);
+  const __amdgpu_buffer_rsrc_t oc = __builtin_amdgcn_make_buffer_rsrc(out_ci, 0, int(v_bytes / 2u), 0x00020000);
   const int64_t end = rp[r + 1];
   for (int64_t base = rp[r]; base < end; base += 4 * SEP_QL) {
     const int64_t e = base + 4 * lig;
@@ -299,6 +302,24 @@ __global__ void sep_fill_quad(const int64_t* __restrict__ rp, const int32_t* __r
     }
     const int tot_a = __shfl(sa, SEP_QL - 1, SEP_QL), tot_b = __shfl(sb, SEP_QL - 1, SEP_QL);
     int64_t ia = o_a + (sa - la), ib = o_b + (sb - lb);
+    if (la == 4) {
+      // all four entries kept in an a column (nearly every lane: the b dofs
+      // are the restrained ones): four consecutive slots, written as two
+      // 16-byte value stores and one 16-byte column store, so that a wave's
+      // stores cover their span densely
+      const uint32_t sa8 = 8u * uint32_t(ia), sa4 = 4u * uint32_t(ia);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(sep_u32x4, make_double2(w[0], w[1])), ov, sa8, 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(sep_u32x4, make_double2(w[2], w[3])), ov, sa8 + 16u, 0, 0);
+      sep_u32x4 cc;
+      cc.x = q[0] & SEP_IDX;
+      cc.y = q[1] & SEP_IDX;
+      cc.z = q[2] & SEP_IDX;
+      cc.w = q[3] & SEP_IDX;
+      __builtin_amdgcn_raw_buffer_store_b128(cc, oc, sa4, 0, 0);
+      o_a += tot_a;
+      o_b += tot_b;
+      continue;
+    }
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t kc = q[j] >> 30;
