@@ -46,6 +46,7 @@ struct BarArgs {
   const uint32_t* __restrict__ conn[2];
   const double* __restrict__ params[2];
   const double* __restrict__ R[2];
+  const double* __restrict__ tdir;  // trusses: first rows of R, 3 doubles each
   const double* __restrict__ derived;
   const uint32_t* __restrict__ bid[2];
   const uint64_t* __restrict__ bmask;
@@ -117,7 +118,7 @@ __device__ inline double bar_shape(const BarArgs& A, int type, size_t e, int p, 
 __device__ inline void truss_constants(const BarArgs& A, size_t e, int p, double o[5]) {
   double bp, bq;
   const double J = bar_shape(A, 0, e, p, bp, bq);
-  const double* Rp = A.R[0] + 9 * e;
+  const double* Rp = A.tdir + 3 * e;  // R's first row, stored compactly by the setup
   const double* prm = A.params[0] + 3 * e;
   const double E = prm[0], Ar = prm[1], A2 = prm[2];
   const double area = (A2 == A2) ? (A2 - Ar) / 2.0 * 0.0 + Ar - (A2 - Ar) / 2.0 * -1.0 : Ar;
@@ -367,6 +368,7 @@ std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed)
       A.bid[t] = d->bo_bid[t];
     }
     A.derived = d->derived;
+    A.tdir = d->truss_dir;
     A.bmask = d->bo_bmask;
     A.boff = d->bo_boff;
     A.inc_ptr = d->bo_inc_ptr;

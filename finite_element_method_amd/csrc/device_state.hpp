@@ -15,6 +15,7 @@ struct Device {
   uint32_t* conn[3] = {nullptr, nullptr, nullptr};
   double* params[3] = {nullptr, nullptr, nullptr};
   double* R[3] = {nullptr, nullptr, nullptr};
+  double* truss_dir = nullptr;  // trusses: R's first row, 3 doubles each (the only row the stiffness reads)
   double* derived = nullptr;  // beams: i11p, i22p
   uint16_t* rzero[3] = {nullptr, nullptr, nullptr};
   size_t n_elem[3] = {0, 0, 0};

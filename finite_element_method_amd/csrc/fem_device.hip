@@ -52,7 +52,7 @@ void destroy_device(Device* d) {
   auto f = [](auto*& p) { if (p) { hipFree(p); p = nullptr; } };
   f(d->xyz);
   for (int t = 0; t < 3; ++t) { f(d->conn[t]); f(d->params[t]); f(d->R[t]); f(d->rzero[t]); f(d->eclass[t]); f(d->ebase[t]); }
-  f(d->derived); f(d->row_ptr); f(d->col_idx); f(d->val); f(d->class_mask); f(d->class_off); f(d->contrib);
+  f(d->derived); f(d->truss_dir); f(d->row_ptr); f(d->col_idx); f(d->val); f(d->class_mask); f(d->class_off); f(d->contrib);
   f(d->slot_ptr); f(d->slot_src); f(d->pf_hdr); f(d->pf_conn); f(d->pf_eid); f(d->pf_rows); f(d->pf_oinfo);
   f(d->pf_oboff); f(d->pf_oooff); f(d->pf_blk); f(d->pf_obase);
   for (int t = 0; t < 3; ++t) f(d->adj_bid[t]);
@@ -93,7 +93,8 @@ __device__ inline uint16_t zero_bits(const double R[9]) {
 
 // Truss::create (truss.rs:230-255): direction frame
 __global__ void truss_setup_kernel(const uint32_t* __restrict__ conn, const double* __restrict__ xyz, size_t n,
-                                   double abs_tol, double* __restrict__ Rout, uint16_t* __restrict__ rz) {
+                                   double abs_tol, double* __restrict__ Rout, double* __restrict__ dir,
+                                   uint16_t* __restrict__ rz) {
   size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (e >= n) return;
   uint32_t a = conn[2 * e], b = conn[2 * e + 1];
@@ -102,6 +103,7 @@ __global__ void truss_setup_kernel(const uint32_t* __restrict__ conn, const doub
   bool ok = fem::truss_rotation(d, abs_tol, R);
   if (!ok) for (int k = 0; k < 9; ++k) R[k] = 0.0;
   for (int k = 0; k < 9; ++k) Rout[9 * e + k] = R[k];
+  for (int k = 0; k < 3; ++k) dir[3 * e + k] = R[k];  // compact copy of the row the assembly reads
   rz[e] = ok ? zero_bits(R) : uint16_t(0x8000);
 }
 
@@ -156,7 +158,9 @@ std::string run_setup(Device* d, int type, size_t n, double rel_tol, double abs_
   if (!e.empty()) return e;
   unsigned blocks = unsigned((n + 255) / 256);
   if (type == TRUSS) {
-    truss_setup_kernel<<<blocks, 256>>>(d->conn[0], d->xyz, n, abs_tol, d->R[0], d->rzero[0]);
+    e = dalloc(d->truss_dir, 3 * n);
+    if (!e.empty()) return e;
+    truss_setup_kernel<<<blocks, 256>>>(d->conn[0], d->xyz, n, abs_tol, d->R[0], d->truss_dir, d->rzero[0]);
   } else if (type == BEAM) {
     e = dalloc(d->derived, 2 * n);
     if (!e.empty()) return e;
