@@ -20,7 +20,9 @@ import os
 
 import numpy as np
 
-__all__ = ["FEM", "FEMBatch", "FemError", "DeviceError", "DOFParameter", "TRUSS", "BEAM", "PLATE", "lib"]
+__all__ = ["FEM", "FEMBatch", "FemError", "DeviceError", "DOFParameter", "TRUSS", "BEAM", "PLATE", "lib",
+           "is_points_of_quadrilateral_on_the_same_line", "is_points_of_quadrilateral_on_the_same_plane",
+           "convex_hull_on_four_points_on_plane"]
 
 TRUSS, BEAM, PLATE = 0, 1, 2
 SEPARATED = ("aa", "ab", "ba", "bb")
@@ -121,6 +123,9 @@ def lib():
                                     ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(i64),
                                     ctypes.POINTER(i64)]),
         "fem_batch_mesh_csr_to_host": (ci, [vp, sz, vp, vp, vp]),
+        "fem_quad_points_on_same_line": (ci, [vp, ctypes.POINTER(ci)]),
+        "fem_quad_points_on_same_plane": (ci, [vp, dbl, ctypes.POINTER(ci)]),
+        "fem_convex_hull_on_four_points_on_plane": (ci, [vp, vp, dbl, dbl, vp, psz, ctypes.c_char_p, sz]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -627,3 +632,45 @@ class FEMBatch:
         v = np.zeros(nnz, dtype=np.float64)
         self._call(lib().fem_batch_mesh_csr_to_host(self._h, int(i), _ptr(rp), _ptr(ci), _ptr(v)))
         return rp, ci, v
+
+
+# ------------------------------------------------------------------ quadrilateral predicates
+def _quad_points(points):
+    pts = np.ascontiguousarray(points, dtype=np.float64).reshape(4, 3)
+    return pts
+
+
+def is_points_of_quadrilateral_on_the_same_line(point_1, point_2, point_3, point_4):
+    """quadrilateral_4n_element_functions.rs:14-86 (re-exported, src/lib.rs:19-23)."""
+    pts = _quad_points([point_1, point_2, point_3, point_4])
+    out = ctypes.c_int()
+    if lib().fem_quad_points_on_same_line(_ptr(pts), ctypes.byref(out)) != FEM_OK:
+        raise ValueError("fem_quad_points_on_same_line failed")
+    return bool(out.value)
+
+
+def is_points_of_quadrilateral_on_the_same_plane(point_1, point_2, point_3, point_4, abs_tol):
+    """quadrilateral_4n_element_functions.rs:88-129 (re-exported, src/lib.rs:19-23)."""
+    pts = _quad_points([point_1, point_2, point_3, point_4])
+    out = ctypes.c_int()
+    if lib().fem_quad_points_on_same_plane(_ptr(pts), float(abs_tol), ctypes.byref(out)) != FEM_OK:
+        raise ValueError("fem_quad_points_on_same_plane failed")
+    return bool(out.value)
+
+
+def convex_hull_on_four_points_on_plane(point_numbers, points, rel_tol, abs_tol):
+    """quadrilateral_4n_element_functions.rs:173-250 (re-exported, src/lib.rs:19-23):
+    the point numbers of the convex hull of four points, in the crate's order.
+    Raises FemError with the crate's message when points 2-4 have no frame."""
+    nums = np.ascontiguousarray(point_numbers, dtype=np.uint32).reshape(4)
+    pts = _quad_points(points)
+    hull = np.zeros(4, dtype=np.uint32)
+    n = ctypes.c_size_t()
+    err = ctypes.create_string_buffer(256)
+    rc = lib().fem_convex_hull_on_four_points_on_plane(_ptr(nums), _ptr(pts), float(rel_tol), float(abs_tol),
+                                                        _ptr(hull), ctypes.byref(n), err, len(err))
+    if rc == FEM_ERR_VALIDATION:
+        raise FemError(err.value.decode())
+    if rc != FEM_OK:
+        raise ValueError("fem_convex_hull_on_four_points_on_plane failed (%d)" % rc)
+    return [int(x) for x in hull[: n.value]]

@@ -1,17 +1,20 @@
 """Builds libfem_amd.so (host C++ + gfx950 HIP kernels) in-tree with hipcc.
 
-Usage: python -m finite_element_method_amd.build   (or build() from Python)
+Each source compiles to its own object (in parallel, under _build/), then one
+link.  Usage: python -m finite_element_method_amd.build   (or build() from Python)
 """
 
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
+OBJ = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libfem_amd.so")
 SOURCES = ["fem_host.cpp", "fem_device.hip", "plate_fused.hip", "adjoint.hip", "batch.hip", "bar_owner.hip",
-           "separate.hip", "plate_blocks.hip", "plate_corners.hip", "solve.hip",
+           "separate.hip", "plate_blocks.hip", "plate_corners.hip", "plate_lanes.hip", "solve.hip",
            "forces.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("FEM_AMD_ARCH", "gfx950")
@@ -21,7 +24,6 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     "-Wall",
     "-Wno-unused-command-line-argument",
     # packed FP32 instructions disabled in device code
@@ -29,19 +31,36 @@ FLAGS = [
 ]
 
 
+def _compile(src, verbose):
+    obj = os.path.join(OBJ, os.path.splitext(src)[0] + ".o")
+    cmd = [HIPCC] + FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    return obj, res
+
+
 def build(verbose=False):
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    os.makedirs(OBJ, exist_ok=True)
+    jobs = max(1, min(len(SOURCES), int(os.environ.get("MAX_JOBS", "0") or 0) or (os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        results = list(ex.map(lambda s: _compile(s, verbose), SOURCES))
+    failed = [r for _, r in results if r.returncode != 0]
+    for r in failed:
+        sys.stderr.write(r.stdout + r.stderr)
+    if failed:
+        raise RuntimeError("hipcc failed building libfem_amd.so")
     tmp = LIB + ".tmp"
-    cmd = [HIPCC] + FLAGS + srcs + ["-o", tmp]
+    cmd = [HIPCC, "--offload-arch=" + ARCH, "-shared", "-fPIC"] + [o for o, _ in results] + ["-o", tmp]
     if verbose:
         print(" ".join(cmd), flush=True)
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError("hipcc failed building libfem_amd.so")
+        raise RuntimeError("hipcc failed linking libfem_amd.so")
     os.replace(tmp, LIB)
     return LIB
 
 
 if __name__ == "__main__":
-    print(build(verbose=True))
+    print(build(verbose="-v" in sys.argv))

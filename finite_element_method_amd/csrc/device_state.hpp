@@ -41,6 +41,11 @@ struct Device {
   // plate block-lane maps
   int pb_patches = 0, pb_grid = 0;
   uint32_t *pb_hdr = nullptr, *pb_conn = nullptr, *pb_eid = nullptr, *pb_blk = nullptr;
+  // plate balanced-lane maps (the element lists share pb_conn / pb_eid)
+  int pl_waves = 0;
+  uint32_t *pl_hdr = nullptr, *pl_nodes = nullptr, *pl_lanes = nullptr;
+  double* pl_geo = nullptr;   // per wave element slot: local coordinates and parameters
+  bool pl_geo_dirty = false;  // node coordinates or plate parameters changed since pl_geo was filled
   // bar owner maps
   uint32_t* bo_bid[2] = {nullptr, nullptr};
   uint64_t* bo_bmask = nullptr;

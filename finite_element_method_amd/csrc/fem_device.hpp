@@ -99,6 +99,31 @@ std::string upload_plate_block_maps(Device* d, const PlateBlockMaps& pm, int wg_
 std::string set_plate_block_grid(Device* d, int wg_per_cu);
 std::string assemble_plate_blocks(Device* d, double* val, void* stream, bool timed = true);
 
+// Balanced-lane fused plate path (plate_lanes.hip): same preconditions as the
+// maps above, and every node in at most four elements with at most nine
+// node-pair blocks.  One wave per patch of owned nodes.
+constexpr int PL_NODES = 16;         // owned nodes of a wave patch
+constexpr int PL_ELEMS = 32;         // elements touching a wave patch
+constexpr int PL_TRIPS = 4;          // contributions per lane
+constexpr int PL_MAX_NB = 9;         // node-pair blocks per owned node
+constexpr int PL_NODE_STRIDE = 120;  // staging doubles per owned node: 13 nb + 1 values and a parity pad
+struct PlateLaneMaps {
+  int n_waves = 0;
+  std::vector<uint32_t> hdr;    // per wave: n_elems | n_nodes << 8 | trips << 16
+  // per wave PL_ELEMS element slots (insertion order within the wave; unused
+  // slots have element id 0xFFFFFFFF): 4 node indices, element id
+  std::vector<uint32_t> sconn;
+  std::vector<uint32_t> seid;
+  std::vector<uint32_t> nodes;  // per wave PL_NODES x 2: CSR position of row 6a of the owned node, nb
+  // per wave 64 lanes x 2 words (64 bits): up to four items of 13 bits
+  // (wave element | p << 5 | q << 7 | block k << 9) in bits 0..51, item count
+  // in 52..54, owned-node slot in 55..58, nb in 59..62, parity of the node's
+  // CSR position in 63.  Items are grouped by block, insertion order within.
+  std::vector<uint32_t> lanes;
+};
+std::string upload_plate_lane_maps(Device* d, const PlateLaneMaps& pm);
+std::string assemble_plate_lanes(Device* d, double* val, void* stream, bool timed = true);
+
 // Corner-lane fused plate path (plate_corners.hip): same preconditions as the
 // maps above, every node in at most four elements, every off-diagonal node
 // pair in at most two, and each element corner sharing at most two of its

@@ -98,8 +98,9 @@ struct fem_model {
   int pf_nodes = 64;   // fused plate: owned nodes per patch (tuning option)
   int pf_wg_per_cu = -1;  // fused plate: resident workgroups per CU looping over patches (0 = one per
                           // patch, -1 = the kernel's default: 2 for plate_fused, 0 for plate_blocks)
-  int pf_kernel = 1;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block
-                          // (plate_blocks), 2 = one lane per element corner (plate_corners)
+  int pf_kernel = 3;      // fused plate kernel: 0 = row staging (plate_fused), 1 = one lane per block
+                          // (plate_blocks), 2 = one lane per element corner (plate_corners),
+                          // 3 = balanced lanes (plate_lanes; falls back to plate_blocks)
   bool force_generic = false;
   bool wide_loads = true;  // backward block gathers and PCG SpMV: 16-byte loads when the values fit 32-bit offsets
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
@@ -250,7 +251,7 @@ void hp_sort(HP* a, long low, long high) {
   hp_sort(a, low, store - 1);
   hp_sort(a, store + 1, high);
 }
-size_t hull_size(std::vector<HP> u) {
+std::vector<HP> hull_points(std::vector<HP> u) {
   double sx = u[0].x, my = u[0].y;
   size_t mp = 0;
   for (size_t i = 0; i < u.size(); ++i)
@@ -264,12 +265,13 @@ size_t hull_size(std::vector<HP> u) {
     if (area <= 0.0) u.erase(u.begin() + long(i) + 1);
     else i += 1;
   }
-  return u.size();
+  return u;
 }
-// quadrilateral_4n_element_functions.rs:173-250; returns -1 when no frame exists
-int quad_hull_points(const uint32_t num[4], const double* p[4], double abs_tol) {
+// quadrilateral_4n_element_functions.rs:173-250: the hull's point numbers in
+// the crate's order; false when the frame of nodes 2-4 does not exist
+bool quad_hull(const uint32_t num[4], const double* p[4], double abs_tol, std::vector<uint32_t>& out) {
   double R[9];
-  if (!fem::quad_rotation(p[1], p[2], p[3], abs_tol, R)) return -1;
+  if (!fem::quad_rotation(p[1], p[2], p[3], abs_tol, R)) return false;
   auto tr = [&](const double* q, double& x, double& y) {
     double d[3] = {q[0] - p[2][0], q[1] - p[2][1], q[2] - p[2][2]};
     x = (R[0] * d[0] + R[1] * d[1]) + R[2] * d[2];
@@ -280,7 +282,15 @@ int quad_hull_points(const uint32_t num[4], const double* p[4], double abs_tol) 
   hp[1].n = num[1]; tr(p[1], hp[1].x, hp[1].y);
   hp[2] = {num[2], 0.0, 0.0};
   hp[3].n = num[3]; tr(p[3], hp[3].x, hp[3].y);
-  return int(hull_size(hp));
+  out.clear();
+  for (const HP& q : hull_points(hp)) out.push_back(q.n);
+  return true;
+}
+// number of hull points, -1 when no frame exists
+int quad_hull_points(const uint32_t num[4], const double* p[4], double abs_tol) {
+  std::vector<uint32_t> h;
+  if (!quad_hull(num, p, abs_tol, h)) return -1;
+  return int(h.size());
 }
 
 // -------------------------------------------------------- structural patterns
@@ -652,6 +662,122 @@ bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, con
   return flush();
 }
 
+// Maps of the balanced-lane plate kernel (plate_lanes.hip).  Called only after
+// build_plate_maps accepted the mesh.  Returns false when a node has more than
+// PL_TRIPS elements or more than PL_MAX_NB node-pair blocks (the caller then
+// uses the block-lane maps).
+bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
+                           const std::vector<int64_t>& row_ptr, femdev::PlateLaneMaps& pm) {
+  const auto& S = m->set[PLATE];
+  const size_t E = S.size();
+  if (row_ptr.back() >= (int64_t(1) << 31)) return false;
+  std::vector<uint32_t> deg, nptr, nel;
+  std::vector<std::pair<uint64_t, uint32_t>> order;
+  plate_node_incidence(m, deg, nptr, nel, order);
+  struct Item { uint32_t e, p, q, k; };
+  // lanes of one node: up to PL_TRIPS items each, blocks never split
+  auto node_lanes = [&](uint32_t a, std::vector<std::vector<Item>>& lanes) -> bool {
+    lanes.clear();
+    const uint32_t d = deg[a];
+    const uint64_t k0 = bptr[a], k1 = bptr[a + 1];
+    if (d > uint32_t(femdev::PL_TRIPS) || k1 - k0 > uint64_t(femdev::PL_MAX_NB)) return false;
+    std::vector<std::vector<Item>> blk(size_t(k1 - k0));
+    const uint32_t* el = &nel[nptr[a]];
+    for (uint64_t k = k0; k < k1; ++k) {
+      const uint32_t b = blocks[k].b;
+      for (uint32_t r = 0; r < d; ++r) {  // element insertion order
+        const uint32_t e = el[r] & 0x3FFFFFFFu, p = el[r] >> 30;
+        for (uint32_t q = 0; q < 4; ++q)
+          if (S.conn[4 * size_t(e) + q] == b) blk[size_t(k - k0)].push_back({e, p, q, uint32_t(k - k0)});
+      }
+      if (blk[size_t(k - k0)].empty() || blk[size_t(k - k0)].size() > size_t(femdev::PL_TRIPS)) return false;
+    }
+    // first fit, blocks by decreasing contribution count (stable in k)
+    std::vector<size_t> idx(blk.size());
+    std::iota(idx.begin(), idx.end(), size_t(0));
+    std::stable_sort(idx.begin(), idx.end(), [&](size_t x, size_t y) { return blk[x].size() > blk[y].size(); });
+    for (size_t i : idx) {
+      size_t ln = 0;
+      while (ln < lanes.size() && lanes[ln].size() + blk[i].size() > size_t(femdev::PL_TRIPS)) ++ln;
+      if (ln == lanes.size()) lanes.emplace_back();
+      lanes[ln].insert(lanes[ln].end(), blk[i].begin(), blk[i].end());
+    }
+    return true;
+  };
+  std::vector<int32_t> elem_local(E, -1);
+  std::vector<uint8_t> in_wave(E, 0);
+  std::vector<uint32_t> cur, cur_elems;
+  std::vector<std::vector<std::vector<Item>>> cur_lanes;
+  size_t cur_nlanes = 0;
+  auto flush = [&]() {
+    if (cur.empty()) return;
+    std::sort(cur_elems.begin(), cur_elems.end());  // insertion order
+    for (size_t i = 0; i < cur_elems.size(); ++i) elem_local[cur_elems[i]] = int32_t(i);
+    size_t lane_base = pm.lanes.size();
+    pm.lanes.resize(lane_base + 128, 0u);
+    pm.nodes.resize(pm.nodes.size() + 2 * femdev::PL_NODES, 0u);
+    uint32_t* nd = pm.nodes.data() + pm.nodes.size() - 2 * femdev::PL_NODES;
+    size_t ln = 0;
+    uint32_t trips = 0;
+    for (size_t s = 0; s < cur.size(); ++s) {
+      const uint32_t a = cur[s];
+      const uint32_t base = uint32_t(row_ptr[6 * size_t(a)]);
+      const uint32_t nb = uint32_t(bptr[a + 1] - bptr[a]);
+      nd[2 * s] = base;
+      nd[2 * s + 1] = nb;
+      for (const auto& lane : cur_lanes[s]) {
+        uint64_t bits = 0;
+        for (size_t t = 0; t < lane.size(); ++t) {
+          const Item& it = lane[t];
+          const uint64_t code = uint64_t(elem_local[it.e]) | uint64_t(it.p) << 5 | uint64_t(it.q) << 7 |
+                                uint64_t(it.k) << 9;
+          bits |= code << (13 * t);
+        }
+        bits |= uint64_t(lane.size()) << 52 | uint64_t(s) << 55 | uint64_t(nb) << 59 | uint64_t(base & 1u) << 63;
+        pm.lanes[lane_base + 2 * ln] = uint32_t(bits);
+        pm.lanes[lane_base + 2 * ln + 1] = uint32_t(bits >> 32);
+        trips = std::max(trips, uint32_t(lane.size()));
+        ++ln;
+      }
+    }
+    for (size_t i = 0; i < size_t(femdev::PL_ELEMS); ++i) {
+      const bool used = i < cur_elems.size();
+      const uint32_t e = used ? cur_elems[i] : 0u;
+      pm.seid.push_back(used ? e : 0xFFFFFFFFu);
+      for (int q = 0; q < 4; ++q) pm.sconn.push_back(used ? S.conn[4 * size_t(e) + q] : 0u);
+      if (used) {
+        elem_local[e] = -1;
+        in_wave[e] = 0;
+      }
+    }
+    pm.hdr.push_back(uint32_t(cur_elems.size()) | uint32_t(cur.size()) << 8 | trips << 16);
+    pm.n_waves++;
+    cur.clear();
+    cur_elems.clear();
+    cur_lanes.clear();
+    cur_nlanes = 0;
+  };
+  std::vector<std::vector<Item>> lanes;
+  for (auto& o : order) {
+    const uint32_t a = o.second;
+    if (!node_lanes(a, lanes)) return false;
+    uint32_t new_elems = 0;
+    for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_wave[nel[nptr[a] + r] & 0x3FFFFFFFu];
+    if (cur.size() == size_t(femdev::PL_NODES) || cur_nlanes + lanes.size() > 64 ||
+        cur_elems.size() + new_elems > size_t(femdev::PL_ELEMS))
+      flush();
+    cur.push_back(a);
+    cur_nlanes += lanes.size();
+    cur_lanes.push_back(lanes);
+    for (uint32_t r = 0; r < deg[a]; ++r) {
+      const uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
+      if (!in_wave[e]) { in_wave[e] = 1; cur_elems.push_back(e); }
+    }
+  }
+  flush();
+  return true;
+}
+
 // Corner-lane maps (plate_corners.hip).  Returns false when the mesh does not
 // fit the kernel's exchange scheme (the caller then uses the block-lane maps).
 bool build_plate_corner_maps(fem_model* m, const std::vector<uint64_t>& bptr, const std::vector<PairRef>& blocks,
@@ -927,7 +1053,12 @@ std::string build_pattern(fem_model* m) {
     if (!m->force_generic && build_plate_maps(m, bptr, blocks, row_ptr, pm)) {
       femdev::PlateCornerMaps pcm;
       femdev::PlateBlockMaps pbm;
-      if (m->pf_kernel == 2 && build_plate_corner_maps(m, bptr, blocks, row_ptr, pcm)) {
+      femdev::PlateLaneMaps plm;
+      if (m->pf_kernel == 3 && build_plate_lane_maps(m, bptr, blocks, row_ptr, plm)) {
+        e = femdev::upload_plate_lane_maps(m->dev, plm);
+        if (!e.empty()) return e;
+        m->path = "plate_lanes";
+      } else if (m->pf_kernel == 2 && build_plate_corner_maps(m, bptr, blocks, row_ptr, pcm)) {
         e = femdev::upload_plate_corner_maps(m->dev, pcm, m->pf_wg_per_cu < 0 ? 0 : m->pf_wg_per_cu);
         if (!e.empty()) return e;
         m->path = "plate_corners";
@@ -1076,6 +1207,8 @@ int ensure_ready(fem_model* m) {
     m->own_vals = false;
   }
   if (!m->pattern_valid) {
+    // a new pattern comes with a new (unwritten) values buffer
+    m->own_vals = false;
     auto e = build_pattern(m);
     if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
   }
@@ -1277,6 +1410,7 @@ int batch_ready(fem_batch* b) {
 
 std::string launch_values(fem_model* g, double* val, void* stream, bool timed) {
   return g->path == "plate_fused" ? femdev::assemble_plate_fused(g->dev, val, stream, timed)
+         : g->path == "plate_lanes" ? femdev::assemble_plate_lanes(g->dev, val, stream, timed)
          : g->path == "plate_blocks" ? femdev::assemble_plate_blocks(g->dev, val, stream, timed)
          : g->path == "plate_corners" ? femdev::assemble_plate_corners(g->dev, val, stream, timed)
          : g->path == "bar_owner" ? femdev::assemble_bar_owner(g->dev, val, stream, timed)
@@ -1725,11 +1859,47 @@ int fem_build_pattern(fem_model* m) {
   return ensure_ready(m);
 }
 
+int fem_quad_points_on_same_line(const double* points, int* out) {
+  if (!points || !out) return FEM_ERR_ARGUMENT;
+  const double* p[4] = {points, points + 3, points + 6, points + 9};
+  *out = quad_on_line(p) ? 1 : 0;
+  return FEM_OK;
+}
+
+int fem_quad_points_on_same_plane(const double* points, double abs_tol, int* out) {
+  if (!points || !out) return FEM_ERR_ARGUMENT;
+  const double* p[4] = {points, points + 3, points + 6, points + 9};
+  *out = quad_on_plane(p, abs_tol) ? 1 : 0;
+  return FEM_OK;
+}
+
+int fem_convex_hull_on_four_points_on_plane(const uint32_t* numbers, const double* points, double rel_tol,
+                                            double abs_tol, uint32_t* hull, size_t* n_hull, char* err,
+                                            size_t err_cap) {
+  (void)rel_tol;  // the restated alignment snaps with abs_tol only
+  if (!numbers || !points || !hull || !n_hull) return FEM_ERR_ARGUMENT;
+  const double* p[4] = {points, points + 3, points + 6, points + 9};
+  std::vector<uint32_t> h;
+  if (!quad_hull(numbers, p, abs_tol, h)) {
+    static const char msg[] = "Zero length vector could not be aligned!";
+    if (err && err_cap) {
+      std::strncpy(err, msg, err_cap - 1);
+      err[err_cap - 1] = '\0';
+    }
+    *n_hull = 0;
+    return FEM_ERR_VALIDATION;
+  }
+  for (size_t i = 0; i < h.size(); ++i) hull[i] = h[i];
+  *n_hull = h.size();
+  return FEM_OK;
+}
+
 int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
   if (!m || (!val_device && m->nnz)) return FEM_ERR_ARGUMENT;
   int rc = ensure_ready(m);
   if (rc) return rc;
   std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
+                  : m->path == "plate_lanes" ? femdev::assemble_plate_lanes(m->dev, val_device, stream)
                   : m->path == "plate_blocks" ? femdev::assemble_plate_blocks(m->dev, val_device, stream)
                   : m->path == "plate_corners" ? femdev::assemble_plate_corners(m->dev, val_device, stream)
                   : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream)
@@ -2058,7 +2228,7 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
   } else if (k == "force_generic") {
     m->force_generic = value != 0;
   } else if (k == "plate_kernel") {
-    if (value < 0 || value > 2) return fail(m, FEM_ERR_ARGUMENT, "plate_kernel must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(m, FEM_ERR_ARGUMENT, "plate_kernel must be 0, 1, 2 or 3");
     m->pf_kernel = int(value);
   } else if (k == "separate_lanes") {
     if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
@@ -2072,6 +2242,7 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
     return fail(m, FEM_ERR_ARGUMENT, "unknown option " + k);
   }
   m->pattern_valid = false;
+  m->own_vals = false;
   return FEM_OK;
 }
 
@@ -2211,7 +2382,7 @@ int fem_batch_autotune(fem_batch* b) {
             if (rc) return rc;
             if (ms < best_ms) { best_ms = ms; cfg = {pn, wg, 256}; }
           }
-        } else if (g->path == "bar_owner") {
+        } else if (g->path == "bar_owner" || g->path == "plate_lanes") {
           cfg = {pn, g->pf_wg_per_cu, 256};  // fixed workgroup shape
         } else {
           for (int bs : {128, 256, 512}) {

@@ -10,7 +10,7 @@
  *     first failing one are added, `*n_ok` receives how many.  This is the
  *     crate's per-call Result semantics applied element by element.
  *   - Assembly is deferred: the crate assembles eagerly inside add_*, but its
- *     global matrix is crate-private (src/fem/fem.rs:170-176) and is only seen
+ *     global matrix is crate-private (src/fem/fem.rs:70-72) and is only seen
  *     through separate_stiffness_matrix_*, so deferring is unobservable.
  *   - The global matrix is 6N x 6N (N = nodes_number, dof = 6 * node_index + k,
  *     k in X, Y, Z, ThX, ThY, ThZ), stored as CSR: int64 row_ptr, int32
@@ -81,6 +81,29 @@ int fem_model_size(const fem_model* m, size_t* n_nodes, size_t* n_truss, size_t*
  *                                                        src/fem/fem.rs:171-202
  * Computed on the device; row-major. */
 int fem_get_rotation(fem_model* m, int type, uint32_t number, double out[9]);
+
+/* ---- quadrilateral predicates (re-exported by the crate, src/lib.rs:19-23) --
+ *
+ * points: 4 x 3 row-major (points 1..4 of the quadrilateral).  No handle. */
+
+/* is_points_of_quadrilateral_on_the_same_line(p1, p2, p3, p4)
+ *                                   quadrilateral_4n_element_functions.rs:14-86
+ * *out = 1 when the cross product of some corner's two edges has zero norm. */
+int fem_quad_points_on_same_line(const double* points, int* out);
+/* is_points_of_quadrilateral_on_the_same_plane(p1, p2, p3, p4, abs_tol)
+ *                                   quadrilateral_4n_element_functions.rs:88-129
+ * *out = 1 when point 1 lies in the plane of points 2, 3, 4 (abs_tol snapping). */
+int fem_quad_points_on_same_plane(const double* points, double abs_tol, int* out);
+/* convex_hull_on_four_points_on_plane(point_numbers, points, rel_tol, abs_tol)
+ *                                   quadrilateral_4n_element_functions.rs:173-250,
+ *                                   convex_hull_on_plane.rs:14-214
+ * Ok: hull[0 .. *n_hull) = the hull's point numbers in the crate's order
+ * (lowest point first, then by polar angle; *n_hull <= 4).  Err (no frame for
+ * points 2-4): FEM_ERR_VALIDATION, the message copied into err (err_cap bytes,
+ * NUL-terminated; err may be NULL). */
+int fem_convex_hull_on_four_points_on_plane(const uint32_t* numbers, const double* points, double rel_tol,
+                                            double abs_tol, uint32_t* hull, size_t* n_hull, char* err,
+                                            size_t err_cap);
 
 /* ---- assembly ---------------------------------------------------------- */
 

@@ -662,7 +662,8 @@ std::vector<HullPoint<V>> convex_hull_on_plane(std::vector<HullPoint<V>> u) {
 }
 // quad:173-250
 template <class V>
-bool quad_convex(const uint32_t num[4], const Vec3<V> pts[4], V abs_tol, size_t& hull_len, std::string& err) {
+bool quad_hull(const uint32_t num[4], const Vec3<V> pts[4], V abs_tol, std::vector<HullPoint<V>>& hull_out,
+               std::string& err) {
   std::array<V, 9> R;
   if (!quad_rotation(pts[1], pts[2], pts[3], abs_tol, R, err)) return false;
   auto tr = [&](const Vec3<V>& p) {
@@ -674,7 +675,14 @@ bool quad_convex(const uint32_t num[4], const Vec3<V> pts[4], V abs_tol, size_t&
   auto t1 = tr(pts[0]), t2 = tr(pts[1]), t4 = tr(pts[3]);
   std::vector<HullPoint<V>> hp = {{num[0], t1[0], t1[1]}, {num[1], t2[0], t2[1]}, {num[2], V(0), V(0)},
                                   {num[3], t4[0], t4[1]}};
-  hull_len = convex_hull_on_plane(hp).size();
+  hull_out = convex_hull_on_plane(hp);
+  return true;
+}
+template <class V>
+bool quad_convex(const uint32_t num[4], const Vec3<V> pts[4], V abs_tol, size_t& hull_len, std::string& err) {
+  std::vector<HullPoint<V>> h;
+  if (!quad_hull(num, pts, abs_tol, h, err)) return false;
+  hull_len = h.size();
   return true;
 }
 
@@ -1560,6 +1568,28 @@ int om_eval_element(int type, const double* xyz, const double* params, double re
 }
 
 // principal moments of inertia (beam.rs:119-160): out = [i11_p, i22_p, angle]
+// quadrilateral predicates re-exported by the crate (src/lib.rs:19-23); points: 4 x 3
+int om_quad_on_line(const double* p) {
+  Vec3<double> v[4];
+  for (int k = 0; k < 4; ++k) v[k] = Vec3<double>{p[3 * k], p[3 * k + 1], p[3 * k + 2]};
+  return quad_on_line(v[0], v[1], v[2], v[3]) ? 1 : 0;
+}
+int om_quad_on_plane(const double* p, double abs_tol) {
+  Vec3<double> v[4];
+  for (int k = 0; k < 4; ++k) v[k] = Vec3<double>{p[3 * k], p[3 * k + 1], p[3 * k + 2]};
+  return quad_on_plane(v[0], v[1], v[2], v[3], abs_tol) ? 1 : 0;
+}
+// hull point numbers into out (up to 4); returns their count, -1 without a frame
+int om_convex_hull(const uint32_t* num, const double* p, double abs_tol, uint32_t* out) {
+  Vec3<double> v[4];
+  for (int k = 0; k < 4; ++k) v[k] = Vec3<double>{p[3 * k], p[3 * k + 1], p[3 * k + 2]};
+  std::vector<HullPoint<double>> h;
+  std::string err;
+  if (!quad_hull(num, v, abs_tol, h, err)) return -1;
+  for (size_t i = 0; i < h.size(); ++i) out[i] = h[i].n;
+  return int(h.size());
+}
+
 void om_principal_moments(double i11, double i22, double i12, double rel_tol, double out[3]) {
   principal_moments<double>(i11, i22, i12, rel_tol, out[0], out[1], out[2]);
 }

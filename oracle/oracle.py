@@ -46,6 +46,12 @@ def lib():
         L.om_element_matrix.restype = ctypes.c_int
         L.om_element_matrix.argtypes = [vp, ctypes.c_int, u32, ctypes.c_int, vp]
         L.om_principal_moments.argtypes = [dbl, dbl, dbl, dbl, vp]
+        L.om_quad_on_line.restype = ctypes.c_int
+        L.om_quad_on_line.argtypes = [vp]
+        L.om_quad_on_plane.restype = ctypes.c_int
+        L.om_quad_on_plane.argtypes = [vp, dbl]
+        L.om_convex_hull.restype = ctypes.c_int
+        L.om_convex_hull.argtypes = [vp, vp, dbl, vp]
         L.om_rust_debug.restype = ctypes.c_int
         L.om_rust_debug.argtypes = [dbl, ctypes.c_int, ctypes.c_char_p, ctypes.c_int]
         L.om_export.restype = i64
@@ -112,6 +118,28 @@ def eval_element(etype, xyz, params, rel_tol=1e-4, abs_tol=1e-12, parts=None):
     if n < 0:
         raise OracleError("no element frame for these coordinates")
     return out[: n * n].reshape(n, n)
+
+
+def quad_on_line(points):
+    """is_points_of_quadrilateral_on_the_same_line (quadrilateral_4n_element_functions.rs:14-86)."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(12)
+    return bool(lib().om_quad_on_line(_ptr(p)))
+
+
+def quad_on_plane(points, abs_tol):
+    """is_points_of_quadrilateral_on_the_same_plane (quadrilateral_4n_element_functions.rs:88-129)."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(12)
+    return bool(lib().om_quad_on_plane(_ptr(p), float(abs_tol)))
+
+
+def convex_hull(numbers, points, abs_tol):
+    """convex_hull_on_four_points_on_plane (quadrilateral_4n_element_functions.rs:173-250):
+    the hull's point numbers, or None when points 2-4 have no frame."""
+    p = np.ascontiguousarray(points, dtype=np.float64).reshape(12)
+    nums = np.ascontiguousarray(numbers, dtype=np.uint32).reshape(4)
+    out = np.zeros(4, dtype=np.uint32)
+    n = lib().om_convex_hull(_ptr(nums), _ptr(p), float(abs_tol), _ptr(out))
+    return None if n < 0 else [int(x) for x in out[:n]]
 
 
 def principal_moments(i11, i22, i12, rel_tol):

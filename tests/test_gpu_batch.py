@@ -49,7 +49,7 @@ def test_batch_autotune_and_replay(gpu):
     batch = FEMBatch(models)
     cfg = batch.autotune()
     print("config", cfg, flush=True)
-    assert "plate_blocks" in cfg and "bar_owner" in cfg
+    assert "plate_lanes" in cfg and "bar_owner" in cfg
     batch.assemble()
     first = [batch.mesh_csr(i) for i in range(len(meshes))]
     for _ in range(3):

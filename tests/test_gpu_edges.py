@@ -49,7 +49,8 @@ def _fan(k, seed):
                 conn.astype(np.uint32), props)
 
 
-@pytest.mark.parametrize("k,path", [(7, "plate_blocks"), (8, "generic")])
+@pytest.mark.parametrize("k,path", [(3, "plate_lanes"), (4, "plate_lanes"), (5, "plate_blocks"), (7, "plate_blocks"),
+                                    (8, "generic")])
 def test_plate_fan_matches_oracle(gpu, k, path):
     mesh = _fan(k, seed=k)
     ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)

@@ -23,7 +23,7 @@ CASES = {
     "plate_12_tilted": lambda: meshgen.plate_grid(12, seed=8, rotation=_tilt()),
 }
 
-EXPECTED_PATH = {"plate_8": "plate_blocks", "plate_17x9": "plate_blocks", "plate_64_shuffled": "plate_blocks",
+EXPECTED_PATH = {"plate_8": "plate_lanes", "plate_17x9": "plate_lanes", "plate_64_shuffled": "plate_lanes",
                  "plate_12_tilted": "generic", "beam_frame_6": "bar_owner", "truss_lattice_5": "bar_owner",
                  "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner", "beam_frame_5": "bar_owner",
                  "beam_frame_3_seed1": "bar_owner"}
@@ -88,22 +88,27 @@ def test_rotation_getters(gpu):
 
 @pytest.mark.parametrize("name", ["plate_8", "plate_17x9", "plate_64_shuffled"])
 def test_plate_kernels_agree(gpu, name):
-    """The three fused plate kernels (row staging, block lanes, corner lanes)
-    give bitwise identical values, and all match the oracle."""
+    """The four fused plate kernels all match the oracle.  Row staging, block
+    lanes and corner lanes fold the Gauss rule per node pair in the same
+    operation order and give bitwise identical values; the balanced-lane
+    kernel forms each element's point integrals once (symmetric pairs
+    shared), so it agrees with them to rounding."""
     mesh = CASES[name]()
     ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)
     out = {}
-    for kern, path in ((0, "plate_fused"), (1, "plate_blocks"), (2, "plate_corners")):
+    for kern, path in ((0, "plate_fused"), (1, "plate_blocks"), (2, "plate_corners"), (3, "plate_lanes")):
         m = FEM.from_mesh(mesh)
         m.set_option("plate_kernel", kern)
         out[kern] = m.stiffness_csr()
-        print(name, "kernel", kern, m.assembly_path(), flush=True)
-        assert m.assembly_path() == path
         r = compare(out[kern], ref)
+        print(name, "kernel", kern, m.assembly_path(), r, flush=True)
+        assert m.assembly_path() == path
         assert r["pattern_equal"] and r["max_ratio"] <= 1.0, (kern, r)
     for k in (1, 2):
         for a, b in zip(out[0], out[k]):
             assert np.array_equal(a, b), k
+    assert np.array_equal(out[3][0], out[0][0]) and np.array_equal(out[3][1], out[0][1])
+    assert np.abs(out[3][2] - out[0][2]).max() <= 1e-15 * np.abs(out[0][2]).max()
 
 
 def _mixed_bars(model_cls):
