@@ -43,7 +43,7 @@ struct Device {
   uint32_t *pb_hdr = nullptr, *pb_conn = nullptr, *pb_eid = nullptr, *pb_blk = nullptr;
   // plate balanced-lane maps (the element lists share pb_conn / pb_eid)
   int pl_waves = 0;
-  uint32_t *pl_hdr = nullptr, *pl_nodes = nullptr, *pl_lanes = nullptr;
+  uint32_t *pl_nodes = nullptr, *pl_lanes = nullptr;
   double* pl_geo = nullptr;   // per wave element slot: local coordinates and parameters
   bool pl_geo_dirty = false;  // node coordinates or plate parameters changed since pl_geo was filled
   // bar owner maps
@@ -104,7 +104,7 @@ struct Device {
 
 template <class T>
 inline std::string dalloc(T*& p, size_t n) {
-  if (p) { hipFree(p); p = nullptr; }
+  if (p) { (void)hipFree(p); p = nullptr; }
   if (n == 0) return "";
   HIP_TRY(hipMalloc(&p, n * sizeof(T)));
   return "";

@@ -109,12 +109,13 @@ constexpr int PL_MAX_NB = 9;         // node-pair blocks per owned node
 constexpr int PL_NODE_STRIDE = 120;  // staging doubles per owned node: 13 nb + 1 values and a parity pad
 struct PlateLaneMaps {
   int n_waves = 0;
-  std::vector<uint32_t> hdr;    // per wave: n_elems | n_nodes << 8 | trips << 16
   // per wave PL_ELEMS element slots (insertion order within the wave; unused
   // slots have element id 0xFFFFFFFF): 4 node indices, element id
   std::vector<uint32_t> sconn;
   std::vector<uint32_t> seid;
-  std::vector<uint32_t> nodes;  // per wave PL_NODES x 2: CSR position of row 6a of the owned node, nb
+  // per wave PL_NODES x 2: CSR position of row 6a of owned node s (0 past the
+  // last owned node), nb | (n_elems | n_nodes << 8 | trips << 16) << 4
+  std::vector<uint32_t> nodes;
   // per wave 64 lanes x 2 words (64 bits): up to four items of 13 bits
   // (wave element | p << 5 | q << 7 | block k << 9) in bits 0..51, item count
   // in 52..54, owned-node slot in 55..58, nb in 59..62, parity of the node's

@@ -750,7 +750,8 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
         in_wave[e] = 0;
       }
     }
-    pm.hdr.push_back(uint32_t(cur_elems.size()) | uint32_t(cur.size()) << 8 | trips << 16);
+    const uint32_t hdr = uint32_t(cur_elems.size()) | uint32_t(cur.size()) << 8 | trips << 16;
+    for (int s = 0; s < femdev::PL_NODES; ++s) nd[2 * s + 1] |= hdr << 4;
     pm.n_waves++;
     cur.clear();
     cur_elems.clear();

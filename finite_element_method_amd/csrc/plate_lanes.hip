@@ -45,7 +45,8 @@ typedef unsigned int pl_u32x2 __attribute__((ext_vector_type(2)));
 typedef unsigned int pl_u32x4 __attribute__((ext_vector_type(4)));
 
 // element record (doubles): Mxx (10, p <= q), Myy (10), Mxy (16, 4 p + q),
-// A (3, alpha classes), B (3, beta classes), shear edge vectors (8), material (7), pad
+// A (3, alpha classes), B (3, beta classes), shear edge vectors (8), material (cm00, cm01,
+// cm22, t^2/12, cs), pad
 constexpr int PL_REC = 58;
 constexpr int R_MYY = 10, R_MXY = 20, R_A = 36, R_B = 39, R_SH = 42, R_MAT = 50;
 constexpr int PL_NINT = 42;
@@ -57,13 +58,28 @@ constexpr int PL_LDS = PL_ELEMS * PL_REC > PL_NODES * PL_NODE_STRIDE ? PL_ELEMS 
 // Everything a wave reads sits at fixed offsets from its index, so all its
 // loads issue at once: no load waits for another's result.
 struct LaneArgs {
-  const uint32_t* __restrict__ hdr;  // per wave: n_elems | n_nodes << 8 | trips << 16
   const double* __restrict__ geo;    // per wave PL_ELEMS slots of PL_GEO doubles
-  const uint2* __restrict__ nodes;   // per wave PL_NODES: CSR position of row 6a, nb
+  // per wave PL_NODES: CSR position of row 6a of owned node s,
+  // nb | (n_elems | n_nodes << 8 | trips << 16) << 4
+  const uint2* __restrict__ nodes;
   const uint2* __restrict__ lanes;   // per wave 64 lane descriptors
   double* __restrict__ val;
   uint32_t val_bytes;
 };
+
+// inputs of wave patch w: the lane's descriptor, owned node (l mod 16) and
+// the geometry slot of element l / 2
+__device__ inline void load_patch(const LaneArgs& A, uint32_t w, int l, int le, uint2& desc, uint2& nd, double2& g0,
+                                  double2& g1, double2& g2, double2& g3, double2& g4) {
+  desc = A.lanes[size_t(w) * 64 + l];
+  nd = A.nodes[size_t(w) * PL_NODES + (l & (PL_NODES - 1))];
+  const double2* gs = reinterpret_cast<const double2*>(A.geo + (size_t(w) * PL_ELEMS + le) * PL_GEO);
+  g0 = gs[0];
+  g1 = gs[1];
+  g2 = gs[2];
+  g3 = gs[3];
+  g4 = gs[4];
+}
 
 // Fills the wave slots' geometry: local coordinates of nodes 1, 2, 4 about
 // node 3 (the same differences the crate's plate frame takes, quad:340-378,
@@ -125,25 +141,102 @@ __device__ inline double with_sign(double v, int neg) {
   return __builtin_bit_cast(double, x);
 }
 
+// The 13 entries of the flat-plate block (p, q) (layout of plate_layout.hpp:
+// (u,u) (u,v) | (v,u) (v,v) | (w,w) (w,tx) (w,ty) | (tx,.) | (ty,.)) from the
+// six point integrals of the pair, plate.rs:513-672 folded over the 2x2 rule:
+// membrane from cm * M, bending as t^2/12 times the matching membrane entry,
+// MITC4 shear A cs v_p v_q^T + B cs u_p u_q^T with v_a = (sv_a / 2, ev_a),
+// u_a = (su_a / 2, eu_a) (sv, su: sign bits).
+__device__ inline void block_entries(double Mxx, double Mxy, double Myx, double Myy, double Acs, double Bcs,
+                                     double cm00, double cm01, double cm22, double tb, double2 evp, double2 eup,
+                                     double2 evq, double2 euq, int svp, int sup, int svq, int suq, double o[13]) {
+  o[0] = __builtin_fma(cm00, Mxx, cm22 * Myy);
+  o[1] = __builtin_fma(cm01, Mxy, cm22 * Myx);
+  o[2] = __builtin_fma(cm01, Myx, cm22 * Mxy);
+  o[3] = __builtin_fma(cm00, Myy, cm22 * Mxx);
+  const double bxx = tb * o[3], bxy = -(tb * o[2]), byx = -(tb * o[1]), byy = tb * o[0];
+  const double a[3] = {with_sign(0.5 * Acs, svp), Acs * evp.x, Acs * evp.y};
+  const double b[3] = {with_sign(0.5 * Bcs, sup), Bcs * eup.x, Bcs * eup.y};
+  const double vq[3] = {with_sign(0.5, svq), evq.x, evq.y};
+  const double uq[3] = {with_sign(0.5, suq), euq.x, euq.y};
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) o[4 + 3 * i + j] = __builtin_fma(a[i], vq[j], b[i] * uq[j]);
+  o[8] += bxx;
+  o[9] += bxy;
+  o[11] += byx;
+  o[12] += byy;
+}
+
+// Integration points 2h and 2h + 1 of lane half h, in the crate's order
+// (g,g), (-g,g), (-g,-g), (g,-g): both have s = +g (h = 0) or -g (h = 1), and
+// r = s, -s.  The quantities and formulas of fem::plate_point (det J,
+// dh/dx, dh/dy quad:380-653, MITC4 multipliers plate.rs:424-447); the parts
+// that depend on s only are formed once, and the square root of the gamma_rz
+// length at r = -s comes from the other lane of the pair (whose s is -s).
+__device__ inline void lane_points(const double xl[6], int half, fem::PlatePoint& P0, fem::PlatePoint& P1) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double gs = sqrt(double(fem::ONE_THIRD_F32));
+  const double s = half ? -gs : gs;
+  const double q4 = 0.25;
+  const double x1q = x1 * q4, x2q = x2 * q4, x4q = x4 * q4, y1q = y1 * q4, y2q = y2 * q4, y4q = y4 * q4;
+  const double j00 = ((((x1q + x1q * s) - x2q) - x2q * s) + x4q) - x4q * s;
+  const double j01 = ((((y1q + y1q * s) - y2q) - y2q * s) + y4q) - y4q * s;
+  const double dr[4] = {q4 + q4 * s, -(q4 + q4 * s), -(q4 - q4 * s), q4 - q4 * s};
+  const double ax = x1 - x2 + x4, bx = x1 - x2 - x4, cx = x1 + x2 - x4;
+  const double ay = y1 - y2 + y4, by = y1 - y2 - y4, cy = y1 + y2 - y4;
+  const double sq_s = sqrt((ax + s * bx) * (ax + s * bx) + (ay + s * by) * (ay + s * by));
+  const double r0 = s;
+  const double sq_r0 = sqrt((cx + r0 * bx) * (cx + r0 * bx) + (cy + r0 * by) * (cy + r0 * by));
+  const double sq_r1 = swap_pair(sq_r0);  // r = -s: the other lane's r0
+#pragma unroll
+  for (int j = 0; j < 2; ++j) {
+    fem::PlatePoint& o = j ? P1 : P0;
+    const double r = j ? -s : s;
+    const double j10 = ((((x1q + x1q * r) + x2q) - x2q * r) - x4q) - x4q * r;
+    const double j11 = ((((y1q + y1q * r) + y2q) - y2q * r) - y4q) - y4q * r;
+    const double det = j00 * j11 - j01 * j10;
+    const double rdet = 1.0 / det;
+    const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+    const double ds[4] = {q4 + q4 * r, q4 - q4 * r, -(q4 - q4 * r), -(q4 + q4 * r)};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      o.dx[a] = i00 * dr[a] + i01 * ds[a];
+      o.dy[a] = i10 * dr[a] + i11 * ds[a];
+    }
+    const double r8 = 0.125 * rdet;
+    const double mr = (j ? sq_r1 : sq_r0) * r8;
+    const double ms = sq_s * r8;
+    o.al[0] = (1.0 + s) * mr; o.al[1] = o.al[0]; o.al[2] = (1.0 - s) * mr; o.al[3] = o.al[2];
+    o.be[0] = (1.0 + r) * ms; o.be[1] = (1.0 - r) * ms; o.be[2] = o.be[1]; o.be[3] = o.be[0];
+    o.w = det;
+  }
+}
+
 // block index k of item t of a lane descriptor (t < PL_TRIPS)
 __device__ inline int item_block(uint64_t items, int t) { return int(uint32_t(items >> (13 * t + 9)) & 15u); }
 
 
-__global__ __launch_bounds__(64) void plate_lanes_kernel(LaneArgs A) {
+#ifndef PL_WAVES_PER_EU
+#define PL_WAVES_PER_EU 2
+#endif
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER_EU))) void plate_lanes_kernel(
+    LaneArgs A) {
   // element records during steps 1-2, the owned nodes' staged values after
   __shared__ double2 lds2[PL_LDS / 2];
   double* rec = reinterpret_cast<double*>(lds2);
   double* stage = rec;
   const double2* stage2 = lds2;
   const int l = threadIdx.x;
-  const uint32_t w = blockIdx.x;
-  const uint32_t h = A.hdr[w];
-  const int nel = int(h & 255u), nnod = int((h >> 8) & 255u), trips = int(h >> 16);
-  const uint2 desc = A.lanes[size_t(w) * 64 + l];
-  const uint2 nd = A.nodes[size_t(w) * PL_NODES + (l & (PL_NODES - 1))];  // lane s: owned node s
   const int le = l >> 1, half = l & 1;
-  const double2* gs = reinterpret_cast<const double2*>(A.geo + (size_t(w) * PL_ELEMS + le) * PL_GEO);
-  const double2 g0 = gs[0], g1 = gs[1], g2 = gs[2], g3 = gs[3], g4 = gs[4];
+  const uint32_t w = blockIdx.x;
+  uint2 desc, nd;
+  double2 g0, g1, g2, g3, g4;
+  load_patch(A, w, l, le, desc, nd, g0, g1, g2, g3, g4);
+  // nb | (n_elems | n_nodes << 8 | trips << 16) << 4 in every node entry
+  const uint32_t h = uint32_t(__builtin_amdgcn_readfirstlane(int(nd.y))) >> 4;
+  const int nel = int(h & 255u), nnod = int((h >> 8) & 255u), trips = int(h >> 16);
 
   // ---- step 1: element records, two lanes per element
   {
@@ -153,10 +246,11 @@ __global__ __launch_bounds__(64) void plate_lanes_kernel(LaneArgs A) {
     for (int k = 0; k < PL_NINT; ++k) I[k] = 0.0;
     const double xl[6] = {g0.x, g0.y, g1.x, g1.y, g2.x, g2.y};
     if (live) {
+      fem::PlatePoint pts[2];
+      lane_points(xl, half, pts[0], pts[1]);
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        fem::PlatePoint g;
-        fem::plate_point(xl, 2 * half + j, g);
+        const fem::PlatePoint& g = pts[j];
         double wdx[4], wdy[4];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
@@ -200,10 +294,11 @@ __global__ __launch_bounds__(64) void plate_lanes_kernel(LaneArgs A) {
         }
         r[20] = I[20];
         const fem::PlateMat mt = fem::plate_material(g3.x, g3.y, g4.x, g4.y);
+        // bending constants are the membrane ones times t^2 / 12 (plate.rs:532-636)
+        const double tb = (g4.x * g4.x) / 12.0;
         *reinterpret_cast<double2*>(r + R_MAT) = make_double2(mt.cm00, mt.cm01);
-        *reinterpret_cast<double2*>(r + R_MAT + 2) = make_double2(mt.cm22, mt.cb00);
-        *reinterpret_cast<double2*>(r + R_MAT + 4) = make_double2(mt.cb01, mt.cb22);
-        *reinterpret_cast<double2*>(r + R_MAT + 6) = make_double2(mt.cs, 0.0);
+        *reinterpret_cast<double2*>(r + R_MAT + 2) = make_double2(mt.cm22, tb);
+        *reinterpret_cast<double2*>(r + R_MAT + 4) = make_double2(mt.cs, 0.0);
       } else {
         r[21] = I[21];
 #pragma unroll
@@ -253,17 +348,10 @@ __global__ __launch_bounds__(64) void plate_lanes_kernel(LaneArgs A) {
       const double2 euq = *reinterpret_cast<const double2*>(r + R_SH + 4 + 2 * bq);
       const double2 m0 = *reinterpret_cast<const double2*>(r + R_MAT);
       const double2 m1 = *reinterpret_cast<const double2*>(r + R_MAT + 2);
-      const double2 m2 = *reinterpret_cast<const double2*>(r + R_MAT + 4);
-      const double2 m3 = *reinterpret_cast<const double2*>(r + R_MAT + 6);
-      fem::PlateMat mat;
-      mat.cm00 = m0.x; mat.cm01 = m0.y; mat.cm22 = m1.x; mat.cb00 = m1.y; mat.cb01 = m2.x; mat.cb22 = m2.y;
-      mat.cs = m3.x;
-      const double half_cs = mat.cs * 0.5;
-      const double vp[3] = {with_sign(half_cs, bp), mat.cs * evp.x, mat.cs * evp.y};
-      const double up[3] = {with_sign(half_cs, p >> 1), mat.cs * eup.x, mat.cs * eup.y};
-      const double vq[3] = {with_sign(0.5, bq), evq.x, evq.y}, uq[3] = {with_sign(0.5, q >> 1), euq.x, euq.y};
+      const double cs = r[R_MAT + 4];
       double o[13];
-      fem::plate_block_entries(Mxx, Mxy, Myx, Myy, Aa, Bb, mat, vp, up, vq, uq, o);
+      block_entries(Mxx, Mxy, Myx, Myy, Aa * cs, Bb * cs, m0.x, m0.y, m1.x, m1.y, evp, eup, evq, euq, bp, p >> 1,
+                    bq, q >> 1, o);
       if (t == 0) {
         run[0] = 1;
 #pragma unroll
@@ -313,23 +401,26 @@ __global__ __launch_bounds__(64) void plate_lanes_kernel(LaneArgs A) {
   }
   wave_lds_sync();
 
-  // ---- step 3: copy-out, node by node, aligned 16-byte chunks
+  // ---- step 3: copy-out, one wave store per owned node (13 nb + 1 <= 118
+  // values: at most 60 chunks), full chunks as 16-byte stores, the at most
+  // two half chunks at the node's ends as 8-byte stores
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(A.val, 0, int(A.val_bytes), 0x00020000);
+  static_assert(PL_NODE_STRIDE / 2 <= 64, "one chunk per lane and node");
+#pragma unroll 1
   for (int s = 0; s < nnod; ++s) {
     const uint32_t base = uint32_t(__builtin_amdgcn_readlane(int(nd.x), s));
-    const uint32_t len = 13u * uint32_t(__builtin_amdgcn_readlane(int(nd.y), s)) + 1u;
-    const uint32_t c0 = base >> 1, nch = ((base + len + 1u) >> 1) - c0;
-    for (uint32_t c = uint32_t(l); c < nch; c += 64u) {
-      const double2 v = stage2[s * (PL_NODE_STRIDE / 2) + c];
-      const uint32_t g = 2u * (c0 + c);
-      const bool lo_ok = g >= base, hi_ok = g + 1u < base + len;
-      if (lo_ok && hi_ok) {
+    const uint32_t end = base + 13u * (uint32_t(__builtin_amdgcn_readlane(int(nd.y), s)) & 15u) + 1u;
+    const uint32_t g = 2u * ((base >> 1) + uint32_t(l));  // first value of this lane's chunk
+    if (g + 1u < end) {
+      const double2 v = stage2[s * (PL_NODE_STRIDE / 2) + l];
+      if (g >= base) {
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(pl_u32x4, v), rsrc, 8u * g, 0, 0);
-      } else if (lo_ok) {
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.x), rsrc, 8u * g, 0, 0);
-      } else {
+      } else {  // the node starts at an odd position: the chunk's upper half
         __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.y), rsrc, 8u * g + 8u, 0, 0);
       }
+    } else if (g < end) {  // the node ends at an even position: the chunk's lower half
+      const double2 v = stage2[s * (PL_NODE_STRIDE / 2) + l];
+      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.x), rsrc, 8u * g, 0, 0);
     }
   }
 }
@@ -350,15 +441,15 @@ std::string plate_lanes_gather(Device* d, hipStream_t s) {
 std::string upload_plate_lane_maps(Device* d, const PlateLaneMaps& pm) {
   d->pl_waves = pm.n_waves;
   std::string e;
-  if (!(e = dupload(d->pl_hdr, pm.hdr.data(), pm.hdr.size())).empty()) return e;
   if (!(e = dupload(d->pb_conn, pm.sconn.data(), pm.sconn.size())).empty()) return e;
   if (!(e = dupload(d->pb_eid, pm.seid.data(), pm.seid.size())).empty()) return e;
   if (!(e = dupload(d->pl_nodes, pm.nodes.data(), pm.nodes.size())).empty()) return e;
   if (!(e = dupload(d->pl_lanes, pm.lanes.data(), pm.lanes.size())).empty()) return e;
   if (!(e = dalloc(d->pl_geo, size_t(pm.n_waves) * PL_ELEMS * PL_GEO)).empty()) return e;
   e = plate_lanes_gather(d, nullptr);
-  if (e.empty()) HIP_TRY(hipDeviceSynchronize());
-  return e;
+  if (!e.empty()) return e;
+  HIP_TRY(hipDeviceSynchronize());
+  return "";
 }
 
 std::string assemble_plate_lanes(Device* d, double* val, void* stream, bool timed) {
@@ -370,7 +461,6 @@ std::string assemble_plate_lanes(Device* d, double* val, void* stream, bool time
   if (timed) HIP_TRY(hipEventRecord(d->ev0, s));
   if (d->pl_waves > 0) {
     LaneArgs a;
-    a.hdr = d->pl_hdr;
     a.geo = d->pl_geo;
     a.nodes = reinterpret_cast<const uint2*>(d->pl_nodes);
     a.lanes = reinterpret_cast<const uint2*>(d->pl_lanes);
