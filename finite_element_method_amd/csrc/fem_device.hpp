@@ -264,6 +264,12 @@ struct SolveResult {
 // when K_aa's values span under 4 GiB
 std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
                       SolveResult& res, bool wide = true);
+// The same solve with every work buffer in one static device array of
+// STATIC_SCRATCH_BYTES per GPU (no allocation); u_a (n_aa) and r_r (n_bb) are
+// caller device memory.  The model's stored solution is not changed.
+constexpr size_t STATIC_SCRATCH_BYTES = size_t(64) << 20;
+std::string solve_pcg_static(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
+                             double* u_a, double* r_r, SolveResult& res, bool wide = true);
 std::string solution(Device* d, const double** u_a, const double** r_r, int64_t* n_aa, int64_t* n_bb);
 double last_solve_ms(Device* d);
 

@@ -2135,6 +2135,26 @@ int fem_solve_pcg(fem_model* m, int preconditioner, uint64_t max_iter, void* str
   return FEM_OK;
 }
 
+int fem_solve_pcg_noalloc(fem_model* m, int preconditioner, uint64_t max_iter, double* u_a, double* r_r,
+                          void* stream, uint64_t* iterations, int* converged) {
+  int rc = separation_ready(m);
+  if (rc) return rc;
+  if (preconditioner != 0 && preconditioner != 1)
+    return fail(m, FEM_ERR_ARGUMENT, "preconditioner must be 0 (Jacobi) or 1 (node-block Jacobi)");
+  if (max_iter > uint64_t(INT32_MAX)) return fail(m, FEM_ERR_ARGUMENT, "max_iter too large");
+  femdev::SolveResult r;
+  auto e = femdev::solve_pcg_static(m->dev, preconditioner, int(max_iter), m->rel_tol, m->abs_tol, stream, u_a, r_r,
+                                    r, m->wide_loads);
+  if (!e.empty()) {
+    const bool numeric = e.rfind("PCG", 0) == 0;
+    const bool limit = e.find("static scratch") != std::string::npos || e.find("device buffers") != std::string::npos;
+    return fail(m, numeric ? FEM_ERR_VALIDATION : limit ? FEM_ERR_ARGUMENT : FEM_ERR_DEVICE, e);
+  }
+  if (iterations) *iterations = uint64_t(r.iterations);
+  if (converged) *converged = r.converged ? 1 : 0;
+  return FEM_OK;
+}
+
 int fem_solution_device(fem_model* m, const double** u_a, const double** r_r) {
   if (!m) return FEM_ERR_ARGUMENT;
   if (!m->dev) return fail(m, FEM_ERR_STATE, "no solution computed: call fem_solve_pcg");

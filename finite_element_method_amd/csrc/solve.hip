@@ -17,6 +17,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "device_state.hpp"
@@ -450,25 +451,47 @@ unsigned grid_for(int64_t items, int per_block) { return unsigned(std::max<int64
 
 }  // namespace
 
-std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
-                      SolveResult& res, bool wide) {
-  hipStream_t s = static_cast<hipStream_t>(stream);
+// Work buffers of one solve.  get(slot, bytes) returns device memory for a
+// slot; vectors: 0 b, 1 x = u_a, 2 r, 3 z, 4 p, 5 Ap, 6 r_r, 7 r_b; 8 the
+// reduction partials, 9 the scalars, 10 the Jacobi diagonal, 11 block flags,
+// 12 block ids, 13 block starts, 14 scan storage, 15 block inverses.
+struct SolveBuffers {
+  virtual std::string get(int slot, size_t bytes, void** out) = 0;
+  double* host = nullptr;  // S_COUNT doubles of host memory for the scalar reads
+};
+
+namespace {
+
+__global__ void sv_zero(double* __restrict__ x, int n) {
+  if (int(threadIdx.x) < n) x[threadIdx.x] = 0.0;
+}
+
+template <class T>
+std::string take(SolveBuffers& bufs, int slot, size_t n, T*& out) {
+  void* p = nullptr;
+  auto e = bufs.get(slot, std::max<size_t>(n, 1) * sizeof(T), &p);
+  out = static_cast<T*>(p);
+  return e;
+}
+
+// the solve proper: b, preconditioner, PCG updates, r_r, into `bufs`
+std::string solve_core(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, hipStream_t s,
+                       SolveResult& res, bool wide, SolveBuffers& bufs) {
   res = SolveResult();
-  d->sol_done = false;
-  if (!d->sep_done) return "no separation computed";
   const int64_t na = d->sep_rows[0], nb = d->sep_rows[2];
   const size_t una = size_t(std::max<int64_t>(na, 1)), unb = size_t(std::max<int64_t>(nb, 1));
   std::string e;
+  double* vec[8];
   for (int k = 0; k < 6; ++k)  // b, x, r, z, p, Ap
-    if (!(e = dgrow(d->sol_vec[k], d->sol_cap[k], una)).empty()) return e;
-  if (!(e = dgrow(d->sol_rr, d->sol_cap[6], unb)).empty()) return e;
-  if (!(e = dgrow(d->sol_rb, d->sol_cap[7], unb)).empty()) return e;
-  if (!(e = dgrow(d->sol_part, d->sol_cap[8], size_t(2 * SV_PARTS))).empty()) return e;
-  if (!(e = dgrow(d->sol_s, d->sol_cap[9], size_t(S_COUNT))).empty()) return e;
-  if (!d->sol_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&d->sol_host), 16 * sizeof(double)));
-  double *b = d->sol_vec[0], *x = d->sol_vec[1], *r = d->sol_vec[2], *z = d->sol_vec[3], *p = d->sol_vec[4],
-         *ap = d->sol_vec[5];
-  double *pa = d->sol_part, *pb = d->sol_part + SV_PARTS;
+    if (!(e = take(bufs, k, una, vec[k])).empty()) return e;
+  if (!(e = take(bufs, 6, unb, vec[6])).empty()) return e;
+  if (!(e = take(bufs, 7, unb, vec[7])).empty()) return e;
+  double *part = nullptr, *sc = nullptr;
+  if (!(e = take(bufs, 8, size_t(2 * SV_PARTS), part)).empty()) return e;
+  if (!(e = take(bufs, 9, size_t(S_COUNT), sc)).empty()) return e;
+  double *b = vec[0], *x = vec[1], *r = vec[2], *z = vec[3], *p = vec[4], *ap = vec[5], *rr = vec[6], *rb = vec[7];
+  double *pa = part, *pb = part + SV_PARTS;
+  double* host = bufs.host;
   const int64_t* rp[4];
   const int32_t* ci[4];
   const double* v[4];
@@ -485,52 +508,56 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
   }
   // preconditioner
   int64_t nblk = 0;
-  double* zero_s = d->sol_host;
-  for (int k = 0; k < S_COUNT; ++k) zero_s[k] = 0.0;
-  HIP_TRY(hipMemcpyAsync(d->sol_s, zero_s, sizeof(double) * S_COUNT, hipMemcpyHostToDevice, s));
+  double* diag = nullptr;
+  double* inv = nullptr;
+  int32_t* starts = nullptr;
+  sv_zero<<<1, 64, 0, s>>>(sc, S_COUNT);
+  HIP_TRY(hipGetLastError());
   if (precond == 0) {
-    if (!(e = dgrow(d->sol_diag, d->sol_cap[10], una)).empty()) return e;
+    if (!(e = take(bufs, 10, una, diag)).empty()) return e;
     if (na) {
-      sv_diag<<<grid_for(na, B), B, 0, s>>>(rp[0], ci[0], v[0], na, d->sol_diag);
+      sv_diag<<<grid_for(na, B), B, 0, s>>>(rp[0], ci[0], v[0], na, diag);
       HIP_TRY(hipGetLastError());
     }
   } else {
-    if (!(e = dgrow(d->sol_flag, d->sol_cap[11], una + 1)).empty()) return e;
-    if (!(e = dgrow(d->sol_bid, d->sol_cap[12], una + 1)).empty()) return e;
-    if (!(e = dgrow(d->sol_starts, d->sol_cap[13], una + 1)).empty()) return e;
-    sv_block_flags<<<grid_for(na + 1, B), B, 0, s>>>(aa_idx, na, d->sol_flag);
+    uint32_t *flag = nullptr, *bid = nullptr;
+    if (!(e = take(bufs, 11, una + 1, flag)).empty()) return e;
+    if (!(e = take(bufs, 12, una + 1, bid)).empty()) return e;
+    if (!(e = take(bufs, 13, una + 1, starts)).empty()) return e;
+    sv_block_flags<<<grid_for(na + 1, B), B, 0, s>>>(aa_idx, na, flag);
     HIP_TRY(hipGetLastError());
     size_t bytes = 0;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, d->sol_flag, d->sol_bid, int(na + 1), s));
-    if (!(e = dgrow(d->sol_tmp, d->sol_cap[14], bytes)).empty()) return e;
-    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d->sol_tmp, bytes, d->sol_flag, d->sol_bid, int(na + 1), s));
-    sv_block_starts<<<grid_for(na + 1, B), B, 0, s>>>(d->sol_flag, d->sol_bid, na, d->sol_starts);
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, flag, bid, int(na + 1), s));
+    void* tmp = nullptr;
+    if (!(e = bufs.get(14, std::max<size_t>(bytes, 1), &tmp)).empty()) return e;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, bytes, flag, bid, int(na + 1), s));
+    sv_block_starts<<<grid_for(na + 1, B), B, 0, s>>>(flag, bid, na, starts);
     HIP_TRY(hipGetLastError());
     uint32_t nb32 = 0;
-    HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_bid + na, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(host, bid + na, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    std::memcpy(&nb32, d->sol_host, sizeof(uint32_t));
+    std::memcpy(&nb32, host, sizeof(uint32_t));
     nblk = int64_t(nb32);
-    if (!(e = dgrow(d->sol_inv, d->sol_cap[15], size_t(36 * std::max<int64_t>(nblk, 1)))).empty()) return e;
+    if (!(e = take(bufs, 15, size_t(36 * std::max<int64_t>(nblk, 1)), inv)).empty()) return e;
     if (nblk) {
-      sv_block_inverse<<<grid_for(nblk, B), B, 0, s>>>(rp[0], ci[0], v[0], d->sol_starts, nblk, d->sol_inv, d->sol_s);
+      sv_block_inverse<<<grid_for(nblk, B), B, 0, s>>>(rp[0], ci[0], v[0], starts, nblk, inv, sc);
       HIP_TRY(hipGetLastError());
     }
   }
   // start
   if (precond == 0)
-    sv_start_jacobi<<<SV_PARTS, B, 0, s>>>(b, d->sol_diag, na, x, r, z, p, pa, pb);
+    sv_start_jacobi<<<SV_PARTS, B, 0, s>>>(b, diag, na, x, r, z, p, pa, pb);
   else
-    sv_start_block<<<SV_PARTS, B, 0, s>>>(b, d->sol_inv, d->sol_starts, nblk, x, r, z, p, pa, pb);
+    sv_start_block<<<SV_PARTS, B, 0, s>>>(b, inv, starts, nblk, x, r, z, p, pa, pb);
   HIP_TRY(hipGetLastError());
-  sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 0, 0.0, 0.0);
+  sv_final<<<1, 1024, 0, s>>>(pa, pb, sc, 0, 0.0, 0.0);
   HIP_TRY(hipGetLastError());
-  HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_s, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+  HIP_TRY(hipMemcpyAsync(host, sc, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
   HIP_TRY(hipStreamSynchronize(s));
-  if (d->sol_host[S_STATUS] != 0.0) return "PCG(BlockJacobi): singular node block";
-  const double stop = std::max(rel_tol * std::sqrt(d->sol_host[S_BB]), abs_tol);
+  if (host[S_STATUS] != 0.0) return "PCG(BlockJacobi): singular node block";
+  const double stop = std::max(rel_tol * std::sqrt(host[S_BB]), abs_tol);
   int it = 0;
-  bool conv = std::sqrt(d->sol_host[S_RR]) <= stop;
+  bool conv = std::sqrt(host[S_RR]) <= stop;
   if (!conv && max_iter > 0) {
     // Updates are queued in growing batches; the last scalar kernel of an
     // update raises the stop flag on convergence or at max_iter, and every
@@ -543,33 +570,32 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
     for (;;) {
       const int nq = std::min(batch, max_iter - queued);
       for (int q = 0; q < nq; ++q) {
-        if (quad) sv_spmv_it_quad<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, d->sol_s, v_bytes);
-        else sv_spmv_it<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, d->sol_s);
-        sv_dot_it<<<SV_PARTS, B, 0, s>>>(p, ap, na, d->sol_s, pa);
-        sv_final<<<1, 1024, 0, s>>>(pa, nullptr, d->sol_s, 1, stop, double(max_iter));
+        if (quad) sv_spmv_it_quad<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, sc, v_bytes);
+        else sv_spmv_it<<<spmv_blocks, B, 0, s>>>(rp[0], ci[0], v[0], p, ap, na, sc);
+        sv_dot_it<<<SV_PARTS, B, 0, s>>>(p, ap, na, sc, pa);
+        sv_final<<<1, 1024, 0, s>>>(pa, nullptr, sc, 1, stop, double(max_iter));
         if (precond == 0)
-          sv_update_jacobi<<<SV_PARTS, B, 0, s>>>(d->sol_diag, p, ap, d->sol_s, na, x, r, z, pa, pb);
+          sv_update_jacobi<<<SV_PARTS, B, 0, s>>>(diag, p, ap, sc, na, x, r, z, pa, pb);
         else
-          sv_update_block<<<SV_PARTS, B, 0, s>>>(d->sol_inv, d->sol_starts, nblk, p, ap, d->sol_s, x, r, z, pa, pb);
-        sv_final<<<1, 1024, 0, s>>>(pa, pb, d->sol_s, 2, stop, double(max_iter));
-        sv_direction<<<SV_PARTS, B, 0, s>>>(z, d->sol_s, na, p);
+          sv_update_block<<<SV_PARTS, B, 0, s>>>(inv, starts, nblk, p, ap, sc, x, r, z, pa, pb);
+        sv_final<<<1, 1024, 0, s>>>(pa, pb, sc, 2, stop, double(max_iter));
+        sv_direction<<<SV_PARTS, B, 0, s>>>(z, sc, na, p);
       }
       HIP_TRY(hipGetLastError());
       queued += nq;
-      HIP_TRY(hipMemcpyAsync(d->sol_host, d->sol_s, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
+      HIP_TRY(hipMemcpyAsync(host, sc, sizeof(double) * S_COUNT, hipMemcpyDeviceToHost, s));
       HIP_TRY(hipStreamSynchronize(s));
-      if (d->sol_host[S_DONE] != 0.0 || queued >= max_iter) break;
+      if (host[S_DONE] != 0.0 || queued >= max_iter) break;
       batch = std::min(batch * 2, 64);
     }
-    it = int(d->sol_host[S_ITER]);
-    conv = d->sol_host[S_CONV] != 0.0;
+    it = int(host[S_ITER]);
+    conv = host[S_CONV] != 0.0;
   }
   // r_r = (K_ba u_a + K_bb u_b) - r_b
   if (nb) {
-    sv_gather<<<grid_for(nb, B), B, 0, s>>>(d->sep_force, bb_idx, nb, d->sol_rb);
-    sv_spmv<<<grid_for(nb * SPMV_LANES, B), B, 0, s>>>(rp[2], ci[2], v[2], x, nullptr, 1.0, nullptr, d->sol_rr, nb);
-    sv_spmv<<<grid_for(nb * SPMV_LANES, B), B, 0, s>>>(rp[3], ci[3], v[3], u_b, d->sol_rr, 1.0, d->sol_rb, d->sol_rr,
-                                                       nb);
+    sv_gather<<<grid_for(nb, B), B, 0, s>>>(d->sep_force, bb_idx, nb, rb);
+    sv_spmv<<<grid_for(nb * SPMV_LANES, B), B, 0, s>>>(rp[2], ci[2], v[2], x, nullptr, 1.0, nullptr, rr, nb);
+    sv_spmv<<<grid_for(nb * SPMV_LANES, B), B, 0, s>>>(rp[3], ci[3], v[3], u_b, rr, 1.0, rb, rr, nb);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(d->ev5, s));
@@ -579,12 +605,110 @@ std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, doub
   d->sol_ms = ms;
   res.iterations = it;
   res.converged = conv;
-  res.residual = std::sqrt(d->sol_host[S_RR]);
+  res.residual = std::sqrt(host[S_RR]);
   res.stop = stop;
-  d->sol_na = na;
-  d->sol_nb = nb;
+  return "";
+}
+
+// the model's own buffers, grown on demand
+struct ModelBuffers : SolveBuffers {
+  Device* d;
+  explicit ModelBuffers(Device* dev) : d(dev) {}
+  std::string get(int slot, size_t bytes, void** out) override {
+    std::string e;
+    const size_t n8 = (bytes + 7) / 8, n4 = (bytes + 3) / 4;
+    if (slot < 6) { e = dgrow(d->sol_vec[slot], d->sol_cap[slot], n8); *out = d->sol_vec[slot]; }
+    else if (slot == 6) { e = dgrow(d->sol_rr, d->sol_cap[6], n8); *out = d->sol_rr; }
+    else if (slot == 7) { e = dgrow(d->sol_rb, d->sol_cap[7], n8); *out = d->sol_rb; }
+    else if (slot == 8) { e = dgrow(d->sol_part, d->sol_cap[8], n8); *out = d->sol_part; }
+    else if (slot == 9) { e = dgrow(d->sol_s, d->sol_cap[9], n8); *out = d->sol_s; }
+    else if (slot == 10) { e = dgrow(d->sol_diag, d->sol_cap[10], n8); *out = d->sol_diag; }
+    else if (slot == 11) { e = dgrow(d->sol_flag, d->sol_cap[11], n4); *out = d->sol_flag; }
+    else if (slot == 12) { e = dgrow(d->sol_bid, d->sol_cap[12], n4); *out = d->sol_bid; }
+    else if (slot == 13) { e = dgrow(d->sol_starts, d->sol_cap[13], n4); *out = d->sol_starts; }
+    else if (slot == 14) { e = dgrow(d->sol_tmp, d->sol_cap[14], bytes); *out = d->sol_tmp; }
+    else { e = dgrow(d->sol_inv, d->sol_cap[15], n8); *out = d->sol_inv; }
+    return e;
+  }
+};
+
+}  // namespace
+
+std::string solve_pcg(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
+                      SolveResult& res, bool wide) {
+  d->sol_done = false;
+  if (!d->sep_done) return "no separation computed";
+  if (!d->sol_host) HIP_TRY(hipHostMalloc(reinterpret_cast<void**>(&d->sol_host), 16 * sizeof(double)));
+  ModelBuffers bufs(d);
+  bufs.host = d->sol_host;
+  auto e = solve_core(d, precond, max_iter, rel_tol, abs_tol, static_cast<hipStream_t>(stream), res, wide, bufs);
+  if (!e.empty()) return e;
+  d->sol_na = d->sep_rows[0];
+  d->sol_nb = d->sep_rows[2];
   d->sol_done = true;
   return "";
+}
+
+// ---- the solve without allocation: every work buffer in one static array per GPU
+
+namespace {
+
+static __device__ __align__(256) unsigned char scratch[STATIC_SCRATCH_BYTES];
+
+// device address of `scratch` on each GPU, looked up once per device (the
+// array's name is only a host-side stand-in for it)
+std::string scratch_address(void** out) {
+  static std::mutex mu;
+  static void* addr[64] = {};
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) return "device index out of range for the static scratch table";
+  std::lock_guard<std::mutex> lock(mu);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return "hipGetSymbolAddress returned no address for the static scratch";
+    addr[dev] = p;
+  }
+  *out = addr[dev];
+  return "";
+}
+
+// slots 1 (u_a) and 6 (r_r) are the caller's outputs; the others are carved
+// from the static array, each on a 256-byte boundary
+struct StaticBuffers : SolveBuffers {
+  unsigned char* base = nullptr;
+  size_t used = 0;
+  double *u_a = nullptr, *r_r = nullptr;
+  std::string get(int slot, size_t bytes, void** out) override {
+    if (slot == 1) { *out = u_a; return ""; }
+    if (slot == 6) { *out = r_r; return ""; }
+    const size_t off = (used + 255) & ~size_t(255);
+    if (off + bytes > STATIC_SCRATCH_BYTES)
+      return "the solve's work buffers exceed the static scratch of " + std::to_string(STATIC_SCRATCH_BYTES >> 20) +
+             " MiB";
+    *out = base + off;
+    used = off + bytes;
+    return "";
+  }
+};
+
+}  // namespace
+
+std::string solve_pcg_static(Device* d, int precond, int max_iter, double rel_tol, double abs_tol, void* stream,
+                             double* u_a, double* r_r, SolveResult& res, bool wide) {
+  if (!d->sep_done) return "no separation computed";
+  if ((!u_a && d->sep_rows[0]) || (!r_r && d->sep_rows[2])) return "u_a and r_r must be device buffers";
+  void* base = nullptr;
+  auto e = scratch_address(&base);
+  if (!e.empty()) return e;
+  StaticBuffers bufs;
+  bufs.base = static_cast<unsigned char*>(base);
+  bufs.u_a = u_a;
+  bufs.r_r = r_r;
+  double host[S_COUNT];
+  bufs.host = host;
+  return solve_core(d, precond, max_iter, rel_tol, abs_tol, static_cast<hipStream_t>(stream), res, wide, bufs);
 }
 
 std::string solution(Device* d, const double** u_a, const double** r_r, int64_t* n_aa, int64_t* n_bb) {

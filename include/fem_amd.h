@@ -240,6 +240,19 @@ double fem_last_separate_ms(fem_model* m);
  * updates and whether the stopping test was met.  Synchronizes `stream`. */
 int fem_solve_pcg(fem_model* m, int preconditioner, uint64_t max_iter, void* stream, uint64_t* iterations,
                   int* converged);
+/* No counterpart in the crate: fem_solve_pcg for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid hipMalloc).  The same solve of the last fem_separate, bit for bit,
+ * with every work buffer (the PCG vectors, the preconditioner, the reduction
+ * partials and scalars) carved from one static device array of 64 MiB per GPU,
+ * held until the process ends; u_a (n_aa doubles) and r_r (n_bb doubles) are
+ * caller device memory.  Calls no hipMalloc/hipFree.  FEM_ERR_ARGUMENT when the
+ * work buffers exceed the array: about 56 bytes per a-dof with Jacobi, 108
+ * with node-block Jacobi (so up to ~1.1M / ~0.6M a-dofs).  Calls that overlap
+ * on one GPU race on the array.  The handle's stored solution (fem_solution_*)
+ * is not changed.  Synchronizes `stream`. */
+int fem_solve_pcg_noalloc(fem_model* m, int preconditioner, uint64_t max_iter, double* u_a, double* r_r,
+                          void* stream, uint64_t* iterations, int* converged);
 /* u_a (n_aa) and r_r (n_bb) of the last solve: borrowed device pointers
  * (valid until the next solve) or a host copy. */
 int fem_solution_device(fem_model* m, const double** u_a, const double** r_r);
