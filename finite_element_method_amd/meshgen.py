@@ -119,10 +119,15 @@ def _log_uniform(rng, lo, hi, n):
     return np.exp(rng.uniform(np.log(lo), np.log(hi), n))
 
 
-def beam_frame(nx=64, ny=None, nz=None, seed=0, spacing=1.0, E=E_STEEL, nu=0.3, shear=5.0 / 6.0):
+def beam_frame(nx=64, ny=None, nz=None, seed=0, spacing=1.0, E=E_STEEL, nu=0.3, shear=5.0 / 6.0,
+               i12_frac=0.0, equal_frac=0.0):
     """nx*ny*nz nodes at unit spacing, 2-node beams on every axis edge; A, I11,
     I22, It log-uniform over two decades, I12 = 0, local axis 1 a random unit
-    vector whose component perpendicular to the beam is at least 0.1."""
+    vector whose component perpendicular to the beam is at least 0.1.
+
+    i12_frac: fraction of beams with a product of inertia I12 ~ U(-0.4, 0.4) *
+    sqrt(I11 I22) (rotated principal axes, beam.rs:119-160); equal_frac:
+    fraction of beams with I22 = I11 (the crate's rel_tol branch)."""
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
     rng = np.random.default_rng(seed)
@@ -149,7 +154,14 @@ def beam_frame(nx=64, ny=None, nz=None, seed=0, spacing=1.0, E=E_STEEL, nu=0.3, 
     i11 = _log_uniform(rng, 1e-8, 1e-6, ne)
     i22 = _log_uniform(rng, 1e-8, 1e-6, ne)
     it = _log_uniform(rng, 1e-8, 1e-6, ne)
-    props = np.stack([np.full(ne, E), np.full(ne, nu), A, i11, i22, np.zeros(ne), it, np.full(ne, shear),
+    i12 = np.zeros(ne)
+    if equal_frac > 0.0:
+        eq = rng.random(ne) < equal_frac
+        i22 = np.where(eq, i11, i22)
+    if i12_frac > 0.0:
+        on = rng.random(ne) < i12_frac
+        i12 = np.where(on, rng.uniform(-0.4, 0.4, ne) * np.sqrt(i11 * i22), 0.0)
+    props = np.stack([np.full(ne, E), np.full(ne, nu), A, i11, i22, i12, it, np.full(ne, shear),
                       v1[:, 0], v1[:, 1], v1[:, 2]], 1)
     return Mesh("beam", np.arange(1, xyz.shape[0] + 1, dtype=np.uint32), xyz,
                 np.arange(1, ne + 1, dtype=np.uint32), (conn_idx + 1).astype(np.uint32), props,

@@ -17,6 +17,8 @@ CASES = {
     "beam_frame_4x5x3": lambda: meshgen.beam_frame(4, 5, 3, seed=5),
     "beam_frame_5": lambda: meshgen.beam_frame(5, seed=4),
     "beam_frame_3_seed1": lambda: meshgen.beam_frame(3, seed=1),
+    # rotated principal axes (I12 != 0) and equal moments (the rel_tol branch), beam.rs:119-160
+    "beam_frame_5_i12_equal": lambda: meshgen.beam_frame(5, seed=9, i12_frac=0.5, equal_frac=0.3),
     "plate_8": lambda: meshgen.plate_grid(8, seed=1),
     "plate_17x9": lambda: meshgen.plate_grid(17, 9, seed=2),
     "plate_64_shuffled": lambda: meshgen.plate_grid(64, seed=7, shuffle=True),
@@ -26,7 +28,7 @@ CASES = {
 EXPECTED_PATH = {"plate_8": "plate_lanes", "plate_17x9": "plate_lanes", "plate_64_shuffled": "plate_lanes",
                  "plate_12_tilted": "generic", "beam_frame_6": "bar_owner", "truss_lattice_5": "bar_owner",
                  "truss_lattice_11": "bar_owner", "beam_frame_4x5x3": "bar_owner", "beam_frame_5": "bar_owner",
-                 "beam_frame_3_seed1": "bar_owner"}
+                 "beam_frame_3_seed1": "bar_owner", "beam_frame_5_i12_equal": "bar_owner"}
 
 
 def _tilt():

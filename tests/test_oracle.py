@@ -137,6 +137,61 @@ def test_beam_x_aligned_closed_form():
     assert np.allclose(K, ex, rtol=1e-13, atol=1e-13 * np.abs(ex).max())
 
 
+def _roll_frame(c, s):
+    return np.array([[1.0, 0.0, 0.0], [0.0, c, s], [0.0, -s, c]])
+
+
+def test_beam_x_aligned_roll_cases():
+    """Beams along +x (R0 = I, beam.rs:162-258): the roll is acos(p_z / |p|) of
+    the part p of local axis 1 across the beam, plus the principal angle; the
+    direction-cosine branch with cx = 1 gives R = [[1,0,0],[0,c,s],[0,-s,c]],
+    c and s snapped with abs_tol (1e-12)."""
+    L, E, nu, A, it, k = 2.0, 2.1e11, 0.3, 3e-3, 2e-6, 5 / 6
+    big = [E, nu, A, 4e-6, 1e-6, 0.0, it, k]  # I11 > I22: principal angle 0
+    cases = [([0, 0, 1], _roll_frame(1.0, 0.0)),
+             ([0, 0, -1], _roll_frame(-1.0, 0.0)),   # acos(-1) = pi; sin(pi) ~ 1e-16 snaps to 0
+             ([0, 1, 0], _roll_frame(0.0, 1.0)),     # acos(0) = pi/2; cos(pi/2) ~ 6e-17 snaps to 0
+             ([0, -1, 0], _roll_frame(0.0, 1.0)),    # the cosine is sign-blind: the same frame as +y
+             ([0.7, 0, -2], _roll_frame(-1.0, 0.0))]  # the x part of v1 is removed first
+    rng = np.random.default_rng(11)
+    for phi in rng.uniform(0.0, 2 * np.pi, 6):
+        t = np.arccos(np.sin(phi))
+        cases.append(([0.0, np.cos(phi), np.sin(phi)], _roll_frame(np.cos(t), np.sin(t))))
+    for v1, want in cases:
+        m = _beam_model([0, 0, 0], [L, 0, 0], big, v1)
+        R = m.rotation(BEAM, 1).reshape(3, 3)
+        assert np.allclose(R, want, rtol=0, atol=1e-15), (v1, R, want)
+    # I11 < I22: the principal angle is pi_f32 / 2 (beam.rs:147-157), which is
+    # not pi/2: cos(pi_f32 / 2) = -4.37e-8 stays above abs_tol
+    small = [E, nu, A, 1e-6, 4e-6, 0.0, it, k]
+    m = _beam_model([0, 0, 0], [L, 0, 0], small, [0, 0, 1])
+    t = PI_F32 / 2
+    assert np.allclose(m.rotation(BEAM, 1).reshape(3, 3), _roll_frame(np.cos(t), np.sin(t)), rtol=0, atol=1e-15)
+    assert abs(np.cos(t) + 4.371139e-8) < 1e-13
+
+
+def test_beam_principal_branches():
+    """I12 != 0 and I11 == I22 (beam.rs:119-160) in the element: the local
+    bending terms are EI / L with the principal moments, which are the
+    eigenvalues of [[I11, I12], [I12, I22]] (Mohr) up to the crate's rel_tol
+    perturbation when I11 == I22; the frame rolls by the principal angle."""
+    L, E, nu, A, it, k = 2.0, 2.1e11, 0.3, 3e-3, 2e-6, 5 / 6
+    for i11, i22, i12 in [(3e-6, 1e-6, 5e-7), (1e-6, 4e-6, -7e-7), (2e-6, 2e-6, 1e-7), (2e-6, 2e-6, 0.0)]:
+        m = _beam_model([0, 0, 0], [L, 0, 0], [E, nu, A, i11, i22, i12, it, k], [0, 0, 1])
+        i11p, i22p, ang = principal_moments(i11, i22, i12, 1e-4)
+        ev = np.linalg.eigvalsh(np.array([[i11, i12], [i12, i22]]))
+        rel = 1e-9 if i11 != i22 else 2e-4  # the equal-moments branch perturbs one moment by rel_tol
+        assert np.isclose(i11p, ev.max(), rtol=rel) and np.isclose(i22p, ev.min(), rtol=rel)
+        K = m.element_matrix(BEAM, 1, global_frame=False)
+        # theta_v / theta_w bending diagonals at node 1 carry E I22p / L and E I11p / L
+        # (plus the shear terms), the difference of the node-1 / node-2 couplings isolates them
+        ei22 = 0.5 * (K[4, 4] - K[4, 10])
+        ei11 = 0.5 * (K[5, 5] - K[5, 11])
+        assert np.isclose(ei22, E * i22p / L, rtol=1e-12) and np.isclose(ei11, E * i11p / L, rtol=1e-12)
+        R = m.rotation(BEAM, 1).reshape(3, 3)
+        assert np.allclose(R, _roll_frame(np.cos(ang), np.sin(ang)), rtol=0, atol=1e-15), (i11, i22, i12, R, ang)
+
+
 def test_beam_invariants_random_orientation():
     rng = np.random.default_rng(3)
     for _ in range(10):
