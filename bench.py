@@ -3,7 +3,8 @@
 A step is one assembly of the global stiffness values (element stiffness +
 deterministic scatter into the CSR pattern) for the whole mesh, with the mesh,
 the pattern and the slot maps already resident in HBM.  Workload (default):
-Mindlin/MITC4 plate 512 x 512 quads, jittered, FP64.
+Mindlin/MITC4 plate 1024 x 1024 quads, jittered, FP64 (the plate 512 x 512,
+beam frame 64^3, truss lattice 11^3 and batched workloads by --workload).
 
 Prints ONE JSON line.  --gpus above 1 exits with a message (single-GPU build).
 """
@@ -21,10 +22,24 @@ sys.path.insert(0, ROOT)
 
 METRIC = "FP64 elements assembled/s + fwd+bwd sensitivities/s; % of HBM roofline vs CPU"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the forward value kernel, from rocprofv3 FETCH_SIZE and
-# WRITE_SIZE counter runs of this script (scaled by a 1 GiB copy kernel that reads
-# FETCH_SIZE = 0.5 x its bytes and WRITE_SIZE = 1.0 x)
-MEASURED_TRAFFIC = {("plate_512", "plate_blocks"): 348.4e6}
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+PROFILES = os.path.join(ROOT, "profiles")
+
+
+def profile_figure(kernel, workload, key):
+    """A measured figure of `kernel` on `workload` from its committed counter
+    summary profiles/<kernel>_counters__<workload>.md (a line `key: value`),
+    or None when that summary does not exist."""
+    path = os.path.join(PROFILES, "%s_counters__%s.md" % (kernel, workload))
+    try:
+        with open(path) as fh:
+            for line in fh:
+                if line.startswith(key + ":"):
+                    return float(line.split(":", 1)[1].split()[0])
+    except OSError:
+        return None
+    return None
+
 
 WORKLOADS = {
     "plate_512": ("plate_grid", dict(nx=512, seed=0)),
@@ -260,7 +275,7 @@ def cpu_baseline(kind):
     om = meshgen.load_oracle(sample, OracleModel)
     om.csr(structural=True)
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
-    one, bad1 = om.time_values(1, reps=1)
+    one, bad1 = om.time_values(1, reps=5, warmup=1)
     par, badp = om.time_values(threads, reps=5, warmup=1)
     if bad1 or badp:
         raise RuntimeError("CPU baseline values differ from the sequential export")
@@ -273,7 +288,7 @@ def cpu_baseline(kind):
     return {"value": sample.n_elements / par, "unit": "elements/s", "cores": threads, "kind": "port",
             "single_thread_value": sample.n_elements / one, "cpu_model": model, "nproc": os.cpu_count(),
             "sample": desc + ", oracle value recompute (element k, T^T k T, insertion-order CSR sums), "
-                      "median of 5 after 1 warm-up on %d OpenMP threads (%.3f s), one run on one thread "
+                      "median of 5 after 1 warm-up on %d OpenMP threads (%.3f s) and on one thread "
                       "(%.3f s)" % (threads, par, one)}
 
 
@@ -282,7 +297,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="plate_512", choices=sorted(WORKLOADS) + ["batched_256"])
+    ap.add_argument("--workload", default="plate_1024", choices=sorted(WORKLOADS) + ["batched_256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the backward / sensitivity timings")
     ap.add_argument("--no-separate", action="store_true", help="skip the sparse separation timing")
@@ -369,11 +384,19 @@ def main():
                    "dof": int(n_rows), "nnz": int(nnz), "seed": 0, "path": model.assembly_path()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": MEASURED_TRAFFIC.get((args.workload, model.assembly_path())),
+                     "traffic": profile_figure(model.assembly_path(), args.workload, "traffic_bytes_per_launch"),
                      "algorithmic_bytes_per_launch": alg},
         "setup_s": setup_s,
         "wall_s": wall,
     }
+    flops = profile_figure(model.assembly_path(), args.workload, "fp64_flops_per_launch")
+    if flops is not None:
+        # FP64 arithmetic of the same launch (SQ_INSTS_VALU_FLOPS_FP64 and the
+        # F64 MFMA count of the committed counter summary) against the FP64 peak
+        mfma = profile_figure(model.assembly_path(), args.workload, "fp64_mfma_flops_per_launch") or 0.0
+        out["fp64"] = {"flops_per_launch": flops, "achieved_tflops": flops / (ms * 1e-3) / 1e12,
+                       "peak_tflops": FP64_PEAK_TFLOPS, "frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
+                       "mfma_flops_per_launch": mfma, "mfma_frac": mfma / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     if not args.no_grad:
         out["sensitivity"] = sensitivity(model, mesh, nnz, val, ms, args.steps, args.warmup, stream)
     if not args.no_separate:

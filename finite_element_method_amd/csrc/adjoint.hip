@@ -43,7 +43,9 @@ struct AdjArgs {
   const uint32_t* bid;
   const uint64_t* bmask;
   const uint16_t* boff;
-  const uint32_t* kpack;  // flat plates: per (element, p) the block index k of each q, 8 bits each
+  // flat plates, per (element, p): CSR position of row 6 a of corner p's node a,
+  // and the block index k of each q (6 bits each) | nb << 24
+  const uint2* kpack;
   const int64_t* row_ptr;
   double* dparam;  // n x params_per, may be null
   double* xs;      // n x nodes_per x 3, may be null
@@ -176,6 +178,19 @@ __device__ inline void load_flat(const AdjArgs& A, const int64_t rp[6], const ui
   g.v[13] = self ? A.g[rp[5] + bo[5]] : 0.0;
 }
 
+// block k of a node whose nb blocks' values start at gr (fixed flat layout)
+__device__ inline void load_flat_k(const double* gr, int nb, int k, bool self, FlatBlock& g) {
+  g.v[0] = gr[2 * k];
+  g.v[1] = gr[2 * k + 1];
+  g.v[2] = gr[2 * nb + 2 * k];
+  g.v[3] = gr[2 * nb + 2 * k + 1];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) g.v[4 + 3 * i + j] = gr[(4 + 3 * i) * nb + 3 * k + j];
+  g.v[13] = self ? gr[13 * nb] : 0.0;
+}
+
 __device__ inline void expand_flat(const FlatBlock& g, double G[36]) {
 #pragma unroll
   for (int i = 0; i < 36; ++i) G[i] = 0.0;
@@ -297,14 +312,19 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   if (MODE == MODE_U) local_u(A.g, ap, R, up);
   // flat dL/dK blocks: the 13 (14 on the diagonal) values of block k of node
   // ap sit at fixed offsets from its first row (2 per row in the two membrane
-  // rows, 3 per row in the three bending rows, then the drilling entry)
-  int64_t g0 = 0;
+  // rows, 3 per row in the three bending rows, then the drilling entry).  The
+  // row position comes with the block indices, so the value loads wait for
+  // one load only; each block's values are loaded while the previous block is
+  // processed.
+  uint32_t g0 = 0, kp = 0;
   int nb = 0;
-  uint32_t kp = 0;
+  FlatBlock nxt;
   if (MODE == MODE_DLDK && FLAT) {
-    g0 = A.row_ptr[6 * size_t(ap)];
-    nb = int((A.row_ptr[6 * size_t(ap) + 6] - g0) / 13);
-    kp = A.kpack[4 * e + p];
+    const uint2 kr = A.kpack[4 * e + p];
+    g0 = kr.x;
+    kp = kr.y;
+    nb = int(kp >> 24);
+    load_flat_k(A.g + g0, nb, int(kp & 63u), p == 0, nxt);
   }
   double vp_[3], up_[3];
 #pragma unroll
@@ -317,18 +337,8 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   for (int q = 0; q < 4; ++q) {
     double gl[36];
     if (MODE == MODE_DLDK && FLAT) {
-      const int k = int((kp >> (8 * q)) & 255u);
-      const double* gr = A.g + g0;
-      FlatBlock g;
-      g.v[0] = gr[2 * k];
-      g.v[1] = gr[2 * k + 1];
-      g.v[2] = gr[2 * nb + 2 * k];
-      g.v[3] = gr[2 * nb + 2 * k + 1];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int j = 0; j < 3; ++j) g.v[4 + 3 * i + j] = gr[(4 + 3 * i) * nb + 3 * k + j];
-      g.v[13] = p == q ? gr[13 * nb] : 0.0;
+      const FlatBlock g = nxt;
+      if (q < 3) load_flat_k(A.g + g0, nb, int((kp >> (6 * (q + 1))) & 63u), p == q + 1, nxt);
       expand_flat(g, gl);
     } else if (MODE == MODE_DLDK) {
       gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
@@ -572,7 +582,7 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.bid = d->adj_bid[t];
     A.bmask = d->adj_bmask;
     A.boff = d->adj_boff;
-    A.kpack = d->adj_kpack;
+    A.kpack = reinterpret_cast<const uint2*>(d->adj_kpack);
     A.row_ptr = d->row_ptr;
     A.dparam = dparam[t];
     A.xs = (dxyz && t != BEAM) ? d->adj_xs + 3 * d->adj_xoff[t] : nullptr;

@@ -213,7 +213,9 @@ struct AdjointMaps {
   int64_t xoff[3] = {0, 0, 0};    // first gradient row of each element type (beams have none)
   int64_t n_xs_rows = 0;
   bool flat = false;              // plates only, identity frames, standard flat block masks
-  std::vector<uint32_t> kpack;    // flat: per (element, p) the block index within node p's row of each q
+  // flat: per (element, p) two words: the CSR position of row 6 a of corner
+  // p's node a; the block index within a's rows of each q (6 bits each) | nb << 24
+  std::vector<uint32_t> kpack;
 };
 std::string upload_adjoint_maps(Device* d, const AdjointMaps& am);
 // mode 0: g = dL/dK (nnz values, CSR order); mode 1: g = nodal displacements

@@ -1247,20 +1247,24 @@ int ensure_adjoint(fem_model* m) {
     for (size_t e = 0; e < S.size() && am.flat; ++e)
       for (int pq = 0; pq < 16; ++pq)
         if (m->pat_bmask[am.bid[PLATE][16 * e + pq]] != ((pq / 4 == pq % 4) ? MSELF : MOFF)) am.flat = false;
-    // the fixed in-row layout needs 13 nb + 1 values per node and nb < 256
+    // the fixed in-row layout needs 13 nb + 1 values per node, nb < 64 and
+    // 32-bit CSR positions
+    if (m->row_ptr_host.back() >= (int64_t(1) << 32)) am.flat = false;
     for (size_t a = 0; a < N && am.flat; ++a) {
       const uint64_t nb = m->pat_bptr[a + 1] - m->pat_bptr[a];
-      if (nb && (nb > 255 || m->row_ptr_host[6 * a + 6] - m->row_ptr_host[6 * a] != int64_t(13 * nb + 1)))
+      if (nb && (nb > 63 || m->row_ptr_host[6 * a + 6] - m->row_ptr_host[6 * a] != int64_t(13 * nb + 1)))
         am.flat = false;
     }
     if (am.flat) {
-      am.kpack.resize(S.size() * 4);
+      am.kpack.resize(S.size() * 8);
       for (size_t e = 0; e < S.size(); ++e)
         for (int p = 0; p < 4; ++p) {
-          uint32_t w = 0;
-          const uint64_t k0 = m->pat_bptr[S.conn[4 * e + p]];
-          for (int q = 0; q < 4; ++q) w |= uint32_t(am.bid[PLATE][16 * e + 4 * p + q] - k0) << (8 * q);
-          am.kpack[4 * e + p] = w;
+          const uint32_t a = S.conn[4 * e + p];
+          const uint64_t k0 = m->pat_bptr[a];
+          uint32_t w = uint32_t(m->pat_bptr[a + 1] - k0) << 24;
+          for (int q = 0; q < 4; ++q) w |= uint32_t(am.bid[PLATE][16 * e + 4 * p + q] - k0) << (6 * q);
+          am.kpack[8 * e + 2 * p] = uint32_t(m->row_ptr_host[6 * size_t(a)]);
+          am.kpack[8 * e + 2 * p + 1] = w;
         }
     }
   }
