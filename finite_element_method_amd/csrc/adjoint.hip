@@ -210,10 +210,9 @@ __device__ inline void load_boff(const AdjArgs& A, uint32_t k, uint32_t bo[6]) {
   for (int i = 0; i < 6; ++i) bo[i] = A.boff[6 * size_t(k) + i];
 }
 
-// FLAT: every plate frame is the identity (constants instead of loads).  In
-// dL/dK mode every block is also standard: then the frame-rotation term
-// vanishes identically (the pattern has no in-plane / out-of-plane
-// couplings) and the blocks have fixed layouts.
+// FLAT: every plate frame is the identity.  Every dL/dK block is then
+// standard: the frame-rotation term vanishes identically (the pattern has no
+// in-plane / out-of-plane couplings) and the blocks have fixed layouts.
 // point data of the block's 64 elements: lane p of an element computes point p
 constexpr int PA_POINT = 9;  // double2 per (element, point): plate_point's 17 values and a pad
 __device__ inline void store_point(double2* dst, const fem::PlatePoint& g) {
@@ -242,7 +241,69 @@ __device__ inline fem::PlatePoint load_point(const double2* src) {
   return g;
 }
 
-template <int MODE, bool FLAT>
+// element data of plate e: nodes, frame, local coordinates relative to node 3
+// (xl = x, y of nodes 1, 2, 4), parameters
+struct PlateElem {
+  uint32_t nd[4];
+  double R[9];
+  double loc[4][3];
+  double xl[6];
+  double E, nu, t, k;
+};
+
+// FLAT: every plate frame is the identity (constants instead of loads)
+template <bool FLAT>
+__device__ inline void load_plate_elem(const AdjArgs& A, size_t e, PlateElem& o) {
+#pragma unroll
+  for (int a = 0; a < 4; ++a) o.nd[a] = A.conn[4 * e + a];
+  if (FLAT) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o.R[k] = (k % 4 == 0) ? 1.0 : 0.0;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) o.R[k] = A.R[9 * e + k];
+  }
+  const size_t n3 = 3 * size_t(o.nd[2]);
+  const double x3 = A.xyz[n3], y3 = A.xyz[n3 + 1], z3 = A.xyz[n3 + 2];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const size_t na = 3 * size_t(o.nd[a]);
+    const double dd[3] = {A.xyz[na] - x3, A.xyz[na + 1] - y3, A.xyz[na + 2] - z3};
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+      o.loc[a][i] = (o.R[i * 3 + 0] * dd[0] + o.R[i * 3 + 1] * dd[1]) + o.R[i * 3 + 2] * dd[2];
+    if (FLAT) o.loc[a][2] = 0.0;
+  }
+  o.xl[0] = o.loc[0][0]; o.xl[1] = o.loc[0][1];
+  o.xl[2] = o.loc[1][0]; o.xl[3] = o.loc[1][1];
+  o.xl[4] = o.loc[3][0]; o.xl[5] = o.loc[3][1];
+  o.E = A.params[4 * e];
+  o.nu = A.params[4 * e + 1];
+  o.t = A.params[4 * e + 2];
+  o.k = A.params[4 * e + 3];
+}
+
+// sums over the four lanes of an element (dead lanes take part), then lane p
+// writes parameter p and node p's coordinate gradient
+__device__ inline void plate_write(const AdjArgs& A, size_t e, int p, bool live, const PlateElem& el, double P[7],
+                                   double gxl[6], double hx, double hy) {
+#pragma unroll
+  for (int i = 0; i < 7; ++i) P[i] = quad_sum(P[i]);
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gxl[i] = quad_sum(gxl[i]);
+  hx = quad_sum(hx);
+  hy = quad_sum(hy);
+  if (!live) return;
+  double gp[4], gX[4][3];
+  fem::plate_adjoint_finish(P, gxl, hx, hy, el.loc, el.R, el.E, el.nu, el.t, el.k, gp, gX);
+  if (A.dparam) A.dparam[4 * e + p] = fem::sel4(gp[0], gp[1], gp[2], gp[3], p);
+  if (A.xs) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) A.xs[(4 * e + p) * 3 + i] = fem::sel4(gX[0][i], gX[1][i], gX[2][i], gX[3][i], p);
+  }
+}
+
+template <bool FLAT>
 __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   __shared__ double2 pts[64 * 4 * PA_POINT];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
@@ -250,33 +311,13 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   const int p = int(threadIdx.x & 3u);
   const bool live = e0 < A.n;
   const size_t e = live ? e0 : A.n - 1;  // dead lanes still take part in the lane sums
-  uint32_t nd[4];
-#pragma unroll
-  for (int a = 0; a < 4; ++a) nd[a] = A.conn[4 * e + a];
-  double R[9];
-  if (FLAT) {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) R[k] = (k % 4 == 0) ? 1.0 : 0.0;
-  } else {
-#pragma unroll
-    for (int k = 0; k < 9; ++k) R[k] = A.R[9 * e + k];
-  }
+  PlateElem el;
+  load_plate_elem<FLAT>(A, e, el);
+  const uint32_t* nd = el.nd;
+  const double* R = el.R;
   const bool ident = FLAT || is_identity(R);
-  double loc[4][3];
-  {
-    const double x3 = A.xyz[3 * size_t(nd[2])], y3 = A.xyz[3 * size_t(nd[2]) + 1], z3 = A.xyz[3 * size_t(nd[2]) + 2];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-      const double dd[3] = {A.xyz[3 * size_t(nd[a])] - x3, A.xyz[3 * size_t(nd[a]) + 1] - y3,
-                            A.xyz[3 * size_t(nd[a]) + 2] - z3};
-#pragma unroll
-      for (int i = 0; i < 3; ++i) loc[a][i] = (R[i * 3 + 0] * dd[0] + R[i * 3 + 1] * dd[1]) + R[i * 3 + 2] * dd[2];
-      if (FLAT) loc[a][2] = 0.0;
-    }
-  }
-  const double xl[6] = {loc[0][0], loc[0][1], loc[1][0], loc[1][1], loc[3][0], loc[3][1]};
-  const double E = A.params[4 * e], nu = A.params[4 * e + 1], t = A.params[4 * e + 2], kk = A.params[4 * e + 3];
-  const fem::PlateMat mat = fem::plate_material(E, nu, t, kk);
+  const double* xl = el.xl;
+  const fem::PlateMat mat = fem::plate_material(el.E, el.nu, el.t, el.k);
   fem::PlateShear sv;
   fem::plate_shear_vectors(xl, sv);
   const uint32_t ap = p == 0 ? nd[0] : (p == 1 ? nd[1] : (p == 2 ? nd[2] : nd[3]));
@@ -308,8 +349,6 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   for (int i = 0; i < 7; ++i) P[i] = 0.0;
 #pragma unroll
   for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
-  double up[6];
-  if (MODE == MODE_U) local_u(A.g, ap, R, up);
   // flat dL/dK blocks: the 13 (14 on the diagonal) values of block k of node
   // ap sit at fixed offsets from its first row (2 per row in the two membrane
   // rows, 3 per row in the three bending rows, then the drilling entry).  The
@@ -319,7 +358,7 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   uint32_t g0 = 0, kp = 0;
   int nb = 0;
   FlatBlock nxt;
-  if (MODE == MODE_DLDK && FLAT) {
+  if (FLAT) {
     const uint2 kr = A.kpack[4 * e + p];
     g0 = kr.x;
     kp = kr.y;
@@ -336,20 +375,13 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
 #pragma unroll 1
   for (int q = 0; q < 4; ++q) {
     double gl[36];
-    if (MODE == MODE_DLDK && FLAT) {
+    if (FLAT) {
       const FlatBlock g = nxt;
       if (q < 3) load_flat_k(A.g + g0, nb, int((kp >> (6 * (q + 1))) & 63u), p == q + 1, nxt);
       expand_flat(g, gl);
-    } else if (MODE == MODE_DLDK) {
+    } else {
       gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
       if (!ident) to_local(gl, R);
-    } else {
-      double uq[6];
-      local_u(A.g, q == 0 ? nd[0] : (q == 1 ? nd[1] : (q == 2 ? nd[2] : nd[3])), R, uq);
-#pragma unroll
-      for (int i = 0; i < 6; ++i)
-#pragma unroll
-        for (int j = 0; j < 6; ++j) gl[i * 6 + j] = -(up[i] * uq[j]);
     }
     double Mq[6], vq[3], uq_[3], cq[6];
 #pragma unroll
@@ -359,8 +391,8 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
       vq[i] = fem::sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], q);
       uq_[i] = fem::sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], q);
     }
-    // the frame term vanishes for flat dL/dK blocks only (dense -u u^T blocks keep it)
-    fem::plate_block_adjoint_q(Mq, mat, vp_, up_, vq, uq_, p, q, gl, P, cq, gxl, hx, hy, !(FLAT && MODE == MODE_DLDK));
+    // flat dL/dK blocks carry no in-plane / out-of-plane coupling: no frame term
+    fem::plate_block_adjoint_q(Mq, mat, vp_, up_, vq, uq_, p, q, gl, P, cq, gxl, hx, hy, !FLAT);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
@@ -383,21 +415,47 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
     }
     fem::plate_point_reverse(xl, p, mine, gxl);
   }
-  // element sums over the four lanes
+  plate_write(A, e, p, live, el, P, gxl, hx, hy);
+}
+
+// Compliance sensitivities of plates by strain energy: four lanes per element,
+// lane p = integration point p (and node p's drilling term and outputs).  Lane
+// p loads node p's displacements; a quad broadcast gives every lane all four.
+template <bool FLAT>
+__global__ __launch_bounds__(256) void plate_compliance_kernel(AdjArgs A) {
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const size_t e0 = size_t(b) * 64 + (threadIdx.x >> 2);
+  const int p = int(threadIdx.x & 3u);
+  const bool live = e0 < A.n;
+  const size_t e = live ? e0 : A.n - 1;  // dead lanes still take part in the lane sums
+  PlateElem el;
+  load_plate_elem<FLAT>(A, e, el);
+  const fem::PlateMat mat = fem::plate_material(el.E, el.nu, el.t, el.k);
+  fem::PlateShear sv;
+  fem::plate_shear_vectors(el.xl, sv);
+  const uint32_t ap = p == 0 ? el.nd[0] : (p == 1 ? el.nd[1] : (p == 2 ? el.nd[2] : el.nd[3]));
+  double up[6];
+  if (FLAT) {
 #pragma unroll
-  for (int i = 0; i < 7; ++i) P[i] = quad_sum(P[i]);
-#pragma unroll
-  for (int i = 0; i < 6; ++i) gxl[i] = quad_sum(gxl[i]);
-  hx = quad_sum(hx);
-  hy = quad_sum(hy);
-  if (!live) return;
-  double gp[4], gX[4][3];
-  fem::plate_adjoint_finish(P, gxl, hx, hy, loc, R, E, nu, t, kk, gp, gX);
-  if (A.dparam) A.dparam[4 * e + p] = fem::sel4(gp[0], gp[1], gp[2], gp[3], p);
-  if (A.xs) {
-#pragma unroll
-    for (int i = 0; i < 3; ++i) A.xs[(4 * e + p) * 3 + i] = fem::sel4(gX[0][i], gX[1][i], gX[2][i], gX[3][i], p);
+    for (int i = 0; i < 6; ++i) up[i] = A.g[6 * size_t(ap) + i];
+  } else {
+    local_u(A.g, ap, el.R, up);
   }
+  double U[4][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    U[0][i] = quad_bcast<0>(up[i]);
+    U[1][i] = quad_bcast<1>(up[i]);
+    U[2][i] = quad_bcast<2>(up[i]);
+    U[3][i] = quad_bcast<3>(up[i]);
+  }
+  double P[7], gxl[6], hx = 0.0, hy = 0.0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) P[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
+  fem::plate_compliance_point(el.xl, p, mat, sv, U, P, gxl, hx, hy);
+  plate_write(A, e, p, live, el, P, gxl, hx, hy);
 }
 
 // ---------------------------------------------------------------- trusses
@@ -592,10 +650,10 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
                     : 0u;
     if (t == PLATE) {
       const unsigned blocks = unsigned((n + 63) / 64);
-      if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<MODE_DLDK, true><<<blocks, 256, 0, s>>>(A);
-      else if (mode == MODE_DLDK) plate_adjoint_kernel<MODE_DLDK, false><<<blocks, 256, 0, s>>>(A);
-      else if (d->adj_flat) plate_adjoint_kernel<MODE_U, true><<<blocks, 256, 0, s>>>(A);
-      else plate_adjoint_kernel<MODE_U, false><<<blocks, 256, 0, s>>>(A);
+      if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<true><<<blocks, 256, 0, s>>>(A);
+      else if (mode == MODE_DLDK) plate_adjoint_kernel<false><<<blocks, 256, 0, s>>>(A);
+      else if (d->adj_flat) plate_compliance_kernel<true><<<blocks, 256, 0, s>>>(A);
+      else plate_compliance_kernel<false><<<blocks, 256, 0, s>>>(A);
     } else {
       const unsigned blocks = unsigned((n + 255) / 256);
       if (t == TRUSS) {

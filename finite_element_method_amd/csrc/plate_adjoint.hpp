@@ -300,4 +300,122 @@ __host__ __device__ inline void plate_adjoint_finish(const double P[7], const do
     for (int i = 0; i < 3; ++i) gX[a][i] = (R[0 * 3 + i] * ga[a][0] + R[1 * 3 + i] * ga[a][1]) + R[2 * 3 + i] * ga[a][2];
 }
 
+// Compliance by strain energy: the share of integration point ip in
+// L = -u_l^T k u_l, with U[a] = (u, v, w, theta_x, theta_y, theta_z) the
+// local-frame displacements of node a, plus the drilling term of node ip.
+// Adds to P, gxl, hx, hy what plate_block_adjoint_q adds over the sixteen
+// blocks for G = -u_p u_q^T, without forming a block: each block term is a
+// bilinear form in nodal values, so its contraction with u u^T is the point's
+// quadratic form in the strains
+//   membrane eps = (ux, vy, uy + vx), bending kappa = (ay, -bx, ax - by)
+//   (a = theta_x, b = theta_y), shear gamma_r = mr rho, gamma_s = ms sigma
+// (rho, sigma: the MITC4 tying sums of the shear vectors against U), and
+// every derivative follows from the strains, the stress resultants and the
+// natural-coordinate derivatives of U:
+//   * P from the squared strains;
+//   * the point adjoints of plate_point_reverse: det from the energy density,
+//     the inverse Jacobian from resultant x natural derivative, mr and ms
+//     from the shear strains;
+//   * the shear vectors from the tying sums;
+//   * hx, hy = -2 sum_a (x_a x f_a)_{x,y} over the translation and rotation
+//     triples, f = k u_l the local nodal forces (their point share here).
+__host__ __device__ inline void plate_compliance_point(const double xl[6], int ip, const PlateMat& m,
+                                                       const PlateShear& sv, const double U[4][6], double P[7],
+                                                       double gxl[6], double& hx, double& hy) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double gs = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? gs : -gs;
+  const double s = (ip < 2) ? gs : -gs;
+  const double q4 = 0.25;
+  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
+  const double usx = (x1 * ps - x2 * ps) + x4 * ms_, usy = (y1 * ps - y2 * ps) + y4 * ms_;
+  const double urx = (x1 * pr + x2 * mr_) - x4 * pr, ury = (y1 * pr + y2 * mr_) - y4 * pr;
+  const double j00 = usx * q4, j01 = usy * q4, j10 = urx * q4, j11 = ury * q4;
+  const double det = j00 * j11 - j01 * j10;
+  const double rdet = 1.0 / det;
+  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+  const double mr = sqrt(urx * urx + ury * ury) * 0.125 * rdet, ms = sqrt(usx * usx + usy * usy) * 0.125 * rdet;
+  const double dr[4] = {q4 * ps, -q4 * ps, -q4 * ms_, q4 * ms_};
+  const double ds[4] = {q4 * pr, q4 * mr_, -q4 * mr_, -q4 * pr};
+  const double fa[4] = {ps, ps, ms_, ms_}, fb[4] = {pr, mr_, mr_, pr};
+  // natural derivatives of u, v, theta_x, theta_y
+  const int comp[4] = {0, 1, 3, 4};
+  double fr[4], fs[4];
+#pragma unroll
+  for (int f = 0; f < 4; ++f) {
+    fr[f] = ((dr[0] * U[0][comp[f]] + dr[1] * U[1][comp[f]]) + dr[2] * U[2][comp[f]]) + dr[3] * U[3][comp[f]];
+    fs[f] = ((ds[0] * U[0][comp[f]] + ds[1] * U[1][comp[f]]) + ds[2] * U[2][comp[f]]) + ds[3] * U[3][comp[f]];
+  }
+  const double ux = i00 * fr[0] + i01 * fs[0], uy = i10 * fr[0] + i11 * fs[0];
+  const double vx = i00 * fr[1] + i01 * fs[1], vy = i10 * fr[1] + i11 * fs[1];
+  const double ax = i00 * fr[2] + i01 * fs[2], ay = i10 * fr[2] + i11 * fs[2];
+  const double bx = i00 * fr[3] + i01 * fs[3], by = i10 * fr[3] + i11 * fs[3];
+  // stress resultants: derivatives of the energy density phi with respect to
+  // ux (Nux), vy (Nvy), uy and vx (Nxy), ay (May), bx (Mbx), ax (Mxy; by: -Mxy)
+  const double Nux = 2.0 * (m.cm00 * ux + m.cm01 * vy);
+  const double Nvy = 2.0 * (m.cm00 * vy + m.cm01 * ux);
+  const double Nxy = 2.0 * m.cm22 * (uy + vx);
+  const double May = 2.0 * (m.cb00 * ay - m.cb01 * bx);
+  const double Mbx = 2.0 * (m.cb00 * bx - m.cb01 * ay);
+  const double Mxy = 2.0 * m.cb22 * (ax - by);
+  const double phi = 0.5 * (((Nux * ux + Nvy * vy) + Nxy * (uy + vx)) + ((May * ay + Mbx * bx) + Mxy * (ax - by)));
+  // transverse shear: tying sums of the shear vectors against (w, theta_x, theta_y)
+  double sp[4], tp[4];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    sp[a] = (sv.v[a][0] * U[a][2] + sv.v[a][1] * U[a][3]) + sv.v[a][2] * U[a][4];
+    tp[a] = (sv.u[a][0] * U[a][2] + sv.u[a][1] * U[a][3]) + sv.u[a][2] * U[a][4];
+  }
+  const double rho = ps * (sp[0] + sp[1]) + ms_ * (sp[2] + sp[3]);
+  const double sig = pr * (tp[0] + tp[3]) + mr_ * (tp[1] + tp[2]);
+  const double gr = mr * rho, gz = ms * sig;
+  const double w = det;
+  // constitutive partials
+  P[0] -= w * (ux * ux + vy * vy);
+  P[1] -= w * (2.0 * ux * vy);
+  P[2] -= w * ((uy + vx) * (uy + vx));
+  P[3] -= w * (ay * ay + bx * bx);
+  P[4] += w * (2.0 * ay * bx);
+  P[5] -= w * ((ax - by) * (ax - by));
+  P[6] -= w * (gr * gr + gz * gz);
+  // point adjoints -> local coordinates
+  const double cs2 = 2.0 * m.cs;
+  double s7[7];
+  s7[0] = -(phi + m.cs * (gr * gr + gz * gz));
+  s7[1] = -w * (((Nux * fr[0] + Nxy * fr[1]) + Mxy * fr[2]) + Mbx * fr[3]);
+  s7[2] = -w * (((Nux * fs[0] + Nxy * fs[1]) + Mxy * fs[2]) + Mbx * fs[3]);
+  s7[3] = -w * (((Nxy * fr[0] + Nvy * fr[1]) + May * fr[2]) - Mxy * fr[3]);
+  s7[4] = -w * (((Nxy * fs[0] + Nvy * fs[1]) + May * fs[2]) - Mxy * fs[3]);
+  s7[5] = -w * cs2 * gr * rho;
+  s7[6] = -w * cs2 * gz * sig;
+  plate_point_reverse(xl, ip, s7, gxl);
+  // shear vectors, and this point's share of the nodal forces f = k u_l
+  const double cr = -w * cs2 * gr * mr, cz = -w * cs2 * gz * ms;  // dL/d(tying sums) per unit multiplier
+  const double hw = 0.5 * w;
+  double ex = 0.0, ey = 0.0;
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    const double dsp = cr * fa[a], dtp = cz * fb[a];
+    plate_shear_vector_adjoint(a, dsp * U[a][3], dsp * U[a][4], dtp * U[a][3], dtp * U[a][4], gxl);
+    const double dxa = i00 * dr[a] + i01 * ds[a], dya = i10 * dr[a] + i11 * ds[a];
+    const double cv = (w * m.cs) * gr * (fa[a] * mr), cu = (w * m.cs) * gz * (fb[a] * ms);
+    const double fu = hw * (Nux * dxa + Nxy * dya);
+    const double fv = hw * (Nxy * dxa + Nvy * dya);
+    const double fw = cv * sv.v[a][0] + cu * sv.u[a][0];
+    const double fta = hw * (Mxy * dxa + May * dya) + (cv * sv.v[a][1] + cu * sv.u[a][1]);
+    const double ftb = hw * (Mbx * dxa - Mxy * dya) + (cv * sv.v[a][2] + cu * sv.u[a][2]);
+    ex += (U[a][1] * fw - U[a][2] * fv) - U[a][5] * ftb;
+    ey += (U[a][2] * fu - U[a][0] * fw) + U[a][5] * fta;
+  }
+  // drilling force of node ip (KROT6 on its theta_z)
+  const double tzx = ip == 0 ? U[0][3] : (ip == 1 ? U[1][3] : (ip == 2 ? U[2][3] : U[3][3]));
+  const double tzy = ip == 0 ? U[0][4] : (ip == 1 ? U[1][4] : (ip == 2 ? U[2][4] : U[3][4]));
+  const double tzz = ip == 0 ? U[0][5] : (ip == 1 ? U[1][5] : (ip == 2 ? U[2][5] : U[3][5]));
+  const double fz = KROT6 * tzz;
+  ex += tzy * fz;
+  ey -= tzx * fz;
+  hx -= 2.0 * ex;
+  hy -= 2.0 * ey;
+}
+
 }  // namespace fem
