@@ -151,68 +151,22 @@ __device__ inline void local_u(const double* __restrict__ u, uint32_t a, const d
       o[3 * h + i] = (R[i * 3 + 0] * v[3 * h + 0] + R[i * 3 + 1] * v[3 * h + 1]) + R[i * 3 + 2] * v[3 * h + 2];
 }
 
+template <int CTRL>
+__device__ inline double quad_perm(double v) {
+  typedef unsigned int qp_u32x2 __attribute__((ext_vector_type(2)));
+  const qp_u32x2 x = __builtin_bit_cast(qp_u32x2, v);
+  qp_u32x2 r;
+  r.x = uint32_t(__builtin_amdgcn_mov_dpp(int(x.x), CTRL, 0xF, 0xF, false));
+  r.y = uint32_t(__builtin_amdgcn_mov_dpp(int(x.y), CTRL, 0xF, 0xF, false));
+  return __builtin_bit_cast(double, r);
+}
 __device__ inline double quad_sum(double v) {
-  v += __shfl_xor(v, 1, 4);
-  v += __shfl_xor(v, 2, 4);
+  v += quad_perm<0xB1>(v);
+  v += quad_perm<0x4E>(v);
   return v;
 }
 
 // ---------------------------------------------------------------- plates
-// Four lanes per element (lane p = row block of node p), 64 elements per block.
-// Flat standard block (13 values, 14 on the diagonal) held compactly:
-// (u,u) (u,v) (v,u) (v,v) | (w,tx,ty) x (w,tx,ty) row-major | (tz,tz)
-struct FlatBlock {
-  double v[14];
-};
-
-__device__ inline void load_flat(const AdjArgs& A, const int64_t rp[6], const uint32_t bo[6], bool self,
-                                 FlatBlock& g) {
-  g.v[0] = A.g[rp[0] + bo[0]];
-  g.v[1] = A.g[rp[0] + bo[0] + 1];
-  g.v[2] = A.g[rp[1] + bo[1]];
-  g.v[3] = A.g[rp[1] + bo[1] + 1];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) g.v[4 + 3 * i + j] = A.g[rp[2 + i] + bo[2 + i] + j];
-  g.v[13] = self ? A.g[rp[5] + bo[5]] : 0.0;
-}
-
-// block k of a node whose nb blocks' values start at gr (fixed flat layout)
-__device__ inline void load_flat_k(const double* gr, int nb, int k, bool self, FlatBlock& g) {
-  g.v[0] = gr[2 * k];
-  g.v[1] = gr[2 * k + 1];
-  g.v[2] = gr[2 * nb + 2 * k];
-  g.v[3] = gr[2 * nb + 2 * k + 1];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) g.v[4 + 3 * i + j] = gr[(4 + 3 * i) * nb + 3 * k + j];
-  g.v[13] = self ? gr[13 * nb] : 0.0;
-}
-
-__device__ inline void expand_flat(const FlatBlock& g, double G[36]) {
-#pragma unroll
-  for (int i = 0; i < 36; ++i) G[i] = 0.0;
-  G[0] = g.v[0];
-  G[1] = g.v[1];
-  G[6] = g.v[2];
-  G[7] = g.v[3];
-#pragma unroll
-  for (int i = 0; i < 3; ++i)
-#pragma unroll
-    for (int j = 0; j < 3; ++j) G[(2 + i) * 6 + 2 + j] = g.v[4 + 3 * i + j];
-  G[35] = g.v[13];
-}
-
-__device__ inline void load_boff(const AdjArgs& A, uint32_t k, uint32_t bo[6]) {
-#pragma unroll
-  for (int i = 0; i < 6; ++i) bo[i] = A.boff[6 * size_t(k) + i];
-}
-
-// FLAT: every plate frame is the identity.  Every dL/dK block is then
-// standard: the frame-rotation term vanishes identically (the pattern has no
-// in-plane / out-of-plane couplings) and the blocks have fixed layouts.
 // point data of the block's 64 elements: lane p of an element computes point p
 constexpr int PA_POINT = 9;  // double2 per (element, point): plate_point's 17 values and a pad
 __device__ inline void store_point(double2* dst, const fem::PlatePoint& g) {
@@ -303,7 +257,8 @@ __device__ inline void plate_write(const AdjArgs& A, size_t e, int p, bool live,
   }
 }
 
-template <bool FLAT>
+// Plates in any frame: four lanes per element (lane p = row block of node p),
+// 64 elements per block; gradient blocks gathered through the block maps.
 __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   __shared__ double2 pts[64 * 4 * PA_POINT];
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
@@ -312,10 +267,10 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   const bool live = e0 < A.n;
   const size_t e = live ? e0 : A.n - 1;  // dead lanes still take part in the lane sums
   PlateElem el;
-  load_plate_elem<FLAT>(A, e, el);
+  load_plate_elem<false>(A, e, el);
   const uint32_t* nd = el.nd;
   const double* R = el.R;
-  const bool ident = FLAT || is_identity(R);
+  const bool ident = is_identity(R);
   const double* xl = el.xl;
   const fem::PlateMat mat = fem::plate_material(el.E, el.nu, el.t, el.k);
   fem::PlateShear sv;
@@ -349,22 +304,6 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   for (int i = 0; i < 7; ++i) P[i] = 0.0;
 #pragma unroll
   for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
-  // flat dL/dK blocks: the 13 (14 on the diagonal) values of block k of node
-  // ap sit at fixed offsets from its first row (2 per row in the two membrane
-  // rows, 3 per row in the three bending rows, then the drilling entry).  The
-  // row position comes with the block indices, so the value loads wait for
-  // one load only; each block's values are loaded while the previous block is
-  // processed.
-  uint32_t g0 = 0, kp = 0;
-  int nb = 0;
-  FlatBlock nxt;
-  if (FLAT) {
-    const uint2 kr = A.kpack[4 * e + p];
-    g0 = kr.x;
-    kp = kr.y;
-    nb = int(kp >> 24);
-    load_flat_k(A.g + g0, nb, int(kp & 63u), p == 0, nxt);
-  }
   double vp_[3], up_[3];
 #pragma unroll
   for (int i = 0; i < 3; ++i) {
@@ -375,14 +314,8 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
 #pragma unroll 1
   for (int q = 0; q < 4; ++q) {
     double gl[36];
-    if (FLAT) {
-      const FlatBlock g = nxt;
-      if (q < 3) load_flat_k(A.g + g0, nb, int((kp >> (6 * (q + 1))) & 63u), p == q + 1, nxt);
-      expand_flat(g, gl);
-    } else {
-      gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
-      if (!ident) to_local(gl, R);
-    }
+    gather_block(A, A.bid[16 * e + 4 * p + q], ap, gl);
+    if (!ident) to_local(gl, R);
     double Mq[6], vq[3], uq_[3], cq[6];
 #pragma unroll
     for (int k = 0; k < 6; ++k) Mq[k] = fem::sel4(acc[4 * k], acc[4 * k + 1], acc[4 * k + 2], acc[4 * k + 3], q);
@@ -391,8 +324,7 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
       vq[i] = fem::sel4(sv.v[0][i], sv.v[1][i], sv.v[2][i], sv.v[3][i], q);
       uq_[i] = fem::sel4(sv.u[0][i], sv.u[1][i], sv.u[2][i], sv.u[3][i], q);
     }
-    // flat dL/dK blocks carry no in-plane / out-of-plane coupling: no frame term
-    fem::plate_block_adjoint_q(Mq, mat, vp_, up_, vq, uq_, p, q, gl, P, cq, gxl, hx, hy, !FLAT);
+    fem::plate_block_adjoint_q(Mq, mat, vp_, up_, vq, uq_, p, q, gl, P, cq, gxl, hx, hy, true);
 #pragma unroll
     for (int k = 0; k < 6; ++k)
 #pragma unroll
@@ -416,6 +348,296 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
     fem::plate_point_reverse(xl, p, mine, gxl);
   }
   plate_write(A, e, p, live, el, P, gxl, hx, hy);
+}
+
+// plate_write for a flat plate: the frame is the identity and the local
+// coordinates relative to node 3 are xl (z = 0)
+__device__ inline void plate_write_flat(const AdjArgs& A, size_t e, int p, bool live, const double xl[6], double E,
+                                        double nu, double t, double k, double P[7], double gxl[6]) {
+  PlateElem el;
+  el.loc[0][0] = xl[0]; el.loc[0][1] = xl[1];
+  el.loc[1][0] = xl[2]; el.loc[1][1] = xl[3];
+  el.loc[2][0] = 0.0; el.loc[2][1] = 0.0;
+  el.loc[3][0] = xl[4]; el.loc[3][1] = xl[5];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) el.loc[a][2] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 9; ++i) el.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
+  el.E = E;
+  el.nu = nu;
+  el.t = t;
+  el.k = k;
+  plate_write(A, e, p, live, el, P, gxl, 0.0, 0.0);
+}
+
+// ------------------------------------------------ flat plates, dL/dK
+// One wave per block, 16 elements, four lanes per element.  Lane p is row
+// block p while the gradient blocks are read and integration point p while
+// the integrals are swept back; the element's data meet in LDS:
+//   pts:  the four points' data (w = det J, dh/dx, dh/dy, the two distinct
+//         MITC4 multipliers of each direction);
+//   rows: per row p and column q six values, first the integrals (Mxx, Mxy,
+//         Myx, Myy, A, B), then their coefficients dL/d(integral).
+// The sweep of point ip needs only the sums of the (p, q) and (q, p)
+// coefficients, and the point's value adjoint is half the gradient vectors'
+// product with the point data (the integrand is a quadratic form in them).
+constexpr int PF_PT = 14;              // doubles per point
+constexpr int PF_PTS = 4 * PF_PT + 2;  // per element, padded so that elements start in different banks
+constexpr int PF_ROWS = 4 * 24 + 2;    // per element: 4 rows x 4 columns x 6 values, padded
+
+// compiler fence: the accumulators are final at it and LDS reads after it
+// are not issued before it, so the coefficients in registers stay few
+__device__ inline void sweep_fence(double g[16]) {
+#pragma unroll
+  for (int i = 0; i < 16; i += 4) asm volatile("" : "+v"(g[i]), "+v"(g[i + 1]), "+v"(g[i + 2]), "+v"(g[i + 3])::"memory");
+}
+
+// block k of a flat plate node whose rows' values start at gr (nb blocks per
+// row): 2 values per row in the two membrane rows (at 2 k and 2 nb + 2 k), 3
+// per row in the three bending rows (at (4 + 3 i) nb + 3 k), then, on the
+// diagonal block only, the drilling entry (at 13 nb).  The drilling entry has
+// no derivative: 13 values are loaded, (u,u) (u,v) (v,u) (v,v) and the
+// (w, theta_x, theta_y) x (w, theta_x, theta_y) block row-major.
+__device__ inline void load_flat_k13(const double* gr, int nb, int k, double v[13]) {
+  v[0] = gr[2 * k];
+  v[1] = gr[2 * k + 1];
+  v[2] = gr[2 * nb + 2 * k];
+  v[3] = gr[2 * nb + 2 * k + 1];
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) v[4 + 3 * i + j] = gr[(4 + 3 * i) * nb + 3 * k + j];
+}
+
+// shear-vector adjoint of node a given as masks (a varies across lanes): v_a
+// carries e12 for a <= 1, e43 otherwise; u_a carries e14 for a = 0, 3, e23 otherwise
+__device__ inline void shear_vector_adjoint_lane(int a, double gv1, double gv2, double gu1, double gu2,
+                                                 double gxl[6]) {
+  const double q = 0.25;
+  const double v12 = a <= 1 ? q : 0.0, v43 = a <= 1 ? 0.0 : q;
+  const double u14 = (a == 0 || a == 3) ? q : 0.0, u23 = (a == 0 || a == 3) ? 0.0 : q;
+  gxl[0] += gv2 * v12 + gu2 * u14;
+  gxl[1] -= gv1 * v12 + gu1 * u14;
+  gxl[2] += gu2 * u23 - gv2 * v12;
+  gxl[3] += gv1 * v12 - gu1 * u23;
+  gxl[4] += gv2 * v43 - gu2 * u14;
+  gxl[5] += gu1 * u14 - gv1 * v43;
+}
+
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void plate_adjoint_flat_kernel(AdjArgs A) {
+  __shared__ double2 pts[16 * PF_PTS / 2];
+  __shared__ double2 rows[16 * PF_ROWS / 2];
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int le = int(threadIdx.x >> 2);
+  const int p = int(threadIdx.x & 3u);
+  const size_t e0 = size_t(b) * 16 + le;
+  const bool live = e0 < A.n;
+  const size_t e = live ? e0 : A.n - 1;  // dead lanes still take part in the lane sums
+  // gradient-block positions first: the value loads wait on this one load only
+  const uint2 kr = A.kpack[4 * e + p];
+  const uint32_t g0 = kr.x, kp = kr.y;
+  const int nb = int(kp >> 24);
+  double nxt[13];
+  load_flat_k13(A.g + g0, nb, int(kp & 63u), nxt);
+  double xl[6], E, nu, t, kk;
+  {
+    PlateElem el;
+    load_plate_elem<true>(A, e, el);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) xl[i] = el.xl[i];
+    E = el.E;
+    nu = el.nu;
+    t = el.t;
+    kk = el.k;
+  }
+  const fem::PlateMat mat = fem::plate_material(E, nu, t, kk);
+  double* const ep = reinterpret_cast<double*>(pts) + le * PF_PTS;
+  double* const er = reinterpret_cast<double*>(rows) + le * PF_ROWS;
+  // point p's data
+  {
+    fem::PlatePoint g;
+    fem::plate_point(xl, p, g);
+    double2* d = reinterpret_cast<double2*>(ep + p * PF_PT);
+    d[0] = make_double2(g.w, g.dx[0]);
+    d[1] = make_double2(g.dx[1], g.dx[2]);
+    d[2] = make_double2(g.dx[3], g.dy[0]);
+    d[3] = make_double2(g.dy[1], g.dy[2]);
+    d[4] = make_double2(g.dy[3], g.al[0]);
+    d[5] = make_double2(g.al[2], g.be[0]);
+    d[6] = make_double2(g.be[1], 0.0);
+  }
+  __syncthreads();
+  // integrals of row p (plate_row_accumulate with the multipliers' classes:
+  // al_0 = al_1, al_2 = al_3, be_0 = be_3, be_1 = be_2)
+  {
+    double acc[24];
+#pragma unroll
+    for (int i = 0; i < 24; ++i) acc[i] = 0.0;
+#pragma unroll
+    for (int ip = 0; ip < 4; ++ip) {
+      const double2* d = reinterpret_cast<const double2*>(ep + ip * PF_PT);
+      const double2 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5], d6 = d[6];
+      const double w = d0.x;
+      const double dx[4] = {d0.y, d1.x, d1.y, d2.x}, dy[4] = {d2.y, d3.x, d3.y, d4.x};
+      const double al[4] = {d4.y, d4.y, d5.x, d5.x}, be[4] = {d5.y, d6.x, d6.x, d5.y};
+      const double wdx = w * ep[ip * PF_PT + 1 + p], wdy = w * ep[ip * PF_PT + 5 + p];
+      const double wal = w * (p <= 1 ? al[0] : al[2]), wbe = w * ((p == 0 || p == 3) ? be[0] : be[1]);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        acc[q] = __builtin_fma(wdx, dx[q], acc[q]);
+        acc[4 + q] = __builtin_fma(wdx, dy[q], acc[4 + q]);
+        acc[8 + q] = __builtin_fma(wdy, dx[q], acc[8 + q]);
+        acc[12 + q] = __builtin_fma(wdy, dy[q], acc[12 + q]);
+        acc[16 + q] = __builtin_fma(wal, al[q], acc[16 + q]);
+        acc[20 + q] = __builtin_fma(wbe, be[q], acc[20 + q]);
+      }
+    }
+    double2* r = reinterpret_cast<double2*>(er + p * 24);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      r[3 * q] = make_double2(acc[q], acc[4 + q]);
+      r[3 * q + 1] = make_double2(acc[8 + q], acc[12 + q]);
+      r[3 * q + 2] = make_double2(acc[16 + q], acc[20 + q]);
+    }
+  }
+  // the element's shear-vector edges (plate_shear_vectors)
+  const double e12x = -(xl[1] - xl[3]) * 0.25, e12y = (xl[0] - xl[2]) * 0.25;
+  const double e43x = -(xl[5]) * 0.25, e43y = (xl[4]) * 0.25;
+  const double e14x = -(xl[1] - xl[5]) * 0.25, e14y = (xl[0] - xl[4]) * 0.25;
+  const double e23x = -(xl[3]) * 0.25, e23y = (xl[2]) * 0.25;
+  const double vp[3] = {(p == 0 || p == 3) ? 0.5 : -0.5, p <= 1 ? e12x : e43x, p <= 1 ? e12y : e43y};
+  const double up[3] = {p <= 1 ? 0.5 : -0.5, (p == 0 || p == 3) ? e14x : e23x, (p == 0 || p == 3) ? e14y : e23y};
+  double P[7], gxl[6];
+#pragma unroll
+  for (int i = 0; i < 7; ++i) P[i] = 0.0;
+#pragma unroll
+  for (int i = 0; i < 6; ++i) gxl[i] = 0.0;
+  double gv1 = 0.0, gv2 = 0.0, gu1 = 0.0, gu2 = 0.0;  // dL/d(v_p), dL/d(u_p), components 1 and 2
+  // blocks (p, q): q is wave-uniform; block q + 1's values load during block q
+#pragma unroll 1
+  for (int q = 0; q < 4; ++q) {
+    double G[13];
+#pragma unroll
+    for (int i = 0; i < 13; ++i) G[i] = nxt[i];
+    if (q < 3) load_flat_k13(A.g + g0, nb, int((kp >> (6 * (q + 1))) & 63u), nxt);
+    double2* r = reinterpret_cast<double2*>(er + p * 24 + 6 * q);
+    const double2 m0 = r[0], m1 = r[1], m2 = r[2];
+    const double Mxx = m0.x, Mxy = m0.y, Myx = m1.x, Myy = m1.y, Aq = m2.x, Bq = m2.y;
+    const double vq[3] = {(q == 0 || q == 3) ? 0.5 : -0.5, q <= 1 ? e12x : e43x, q <= 1 ? e12y : e43y};
+    const double uq[3] = {q <= 1 ? 0.5 : -0.5, (q == 0 || q == 3) ? e14x : e23x, (q == 0 || q == 3) ? e14y : e23y};
+    const double Guu = G[0], Guv = G[1], Gvu = G[2], Gvv = G[3];
+    const double Gxx = G[8], Gxy = G[9], Gyx = G[11], Gyy = G[12];
+    // constitutive partials
+    P[0] += Guu * Mxx + Gvv * Myy;
+    P[1] += Guv * Mxy + Gvu * Myx;
+    P[2] += ((Guu * Myy + Guv * Myx) + Gvu * Mxy) + Gvv * Mxx;
+    P[3] += Gxx * Myy + Gyy * Mxx;
+    P[4] -= Gxy * Myx + Gyx * Mxy;
+    P[5] += ((Gxx * Mxx - Gxy * Mxy) - Gyx * Myx) + Gyy * Myy;
+    // shear block Gs = G[4..12] over (w, theta_x, theta_y)
+    double Gv[3], Gu[3], vG[3], uG[3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      Gv[i] = (G[4 + 3 * i] * vq[0] + G[5 + 3 * i] * vq[1]) + G[6 + 3 * i] * vq[2];
+      Gu[i] = (G[4 + 3 * i] * uq[0] + G[5 + 3 * i] * uq[1]) + G[6 + 3 * i] * uq[2];
+      vG[i] = (vp[0] * G[4 + i] + vp[1] * G[7 + i]) + vp[2] * G[10 + i];
+      uG[i] = (up[0] * G[4 + i] + up[1] * G[7 + i]) + up[2] * G[10 + i];
+    }
+    const double sv_ = (vp[0] * Gv[0] + vp[1] * Gv[1]) + vp[2] * Gv[2];
+    const double su_ = (up[0] * Gu[0] + up[1] * Gu[1]) + up[2] * Gu[2];
+    P[6] += Aq * sv_ + Bq * su_;
+    // coefficients of column q's integrals
+    const double cxx = ((mat.cm00 * Guu + mat.cm22 * Gvv) + mat.cb22 * Gxx) + mat.cb00 * Gyy;
+    const double cxy = ((mat.cm01 * Guv + mat.cm22 * Gvu) - mat.cb22 * Gxy) - mat.cb01 * Gyx;
+    const double cyx = ((mat.cm22 * Guv + mat.cm01 * Gvu) - mat.cb01 * Gxy) - mat.cb22 * Gyx;
+    const double cyy = ((mat.cm22 * Guu + mat.cm00 * Gvv) + mat.cb00 * Gxx) + mat.cb22 * Gyy;
+    r[0] = make_double2(cxx, cxy);
+    r[1] = make_double2(cyx, cyy);
+    r[2] = make_double2(mat.cs * sv_, mat.cs * su_);
+    // shear vectors: d/dv_p = cs A Gs v_q, d/dv_q = cs A v_p^T Gs (same with u, B)
+    const double sa = mat.cs * Aq, sb = mat.cs * Bq;
+    gv1 += sa * Gv[1];
+    gv2 += sa * Gv[2];
+    gu1 += sb * Gu[1];
+    gu2 += sb * Gu[2];
+    shear_vector_adjoint_lane(q, sa * vG[1], sa * vG[2], sb * uG[1], sb * uG[2], gxl);
+  }
+  shear_vector_adjoint_lane(p, gv1, gv2, gu1, gu2, gxl);
+  __syncthreads();
+  // sweep of point p over the sums of the (a, b) and (b, a) coefficients:
+  // gdx_a = dl/d(dx_a), ... of l = w * sum_ab [dx_a cxx dx_b + dx_a cxy dy_b
+  // + dy_a cyx dx_b + dy_a cyy dy_b + al_a cA al_b + be_a cB be_b]
+  {
+    const double2* d = reinterpret_cast<const double2*>(ep + p * PF_PT);
+    const double2 d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3], d4 = d[4], d5 = d[5], d6 = d[6];
+    const double w = d0.x;
+    const double dx[4] = {d0.y, d1.x, d1.y, d2.x}, dy[4] = {d2.y, d3.x, d3.y, d4.x};
+    const double al[4] = {d4.y, d4.y, d5.x, d5.x}, be[4] = {d5.y, d6.x, d6.x, d5.y};
+    double gv[16];  // dl/d(dx_a), dl/d(dy_a), dl/d(al_a), dl/d(be_a)
+    double* const gdx = gv;
+    double* const gdy = gv + 4;
+    double* const gal = gv + 8;
+    double* const gbe = gv + 12;
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const double2* c = reinterpret_cast<const double2*>(er + a * 24 + 6 * a);
+      const double2 c0 = c[0], c1 = c[1], c2 = c[2];
+      const double sxy = c0.y + c1.x;
+      gdx[a] = 2.0 * c0.x * dx[a] + sxy * dy[a];
+      gdy[a] = sxy * dx[a] + 2.0 * c1.y * dy[a];
+      gal[a] = 2.0 * c2.x * al[a];
+      gbe[a] = 2.0 * c2.y * be[a];
+    }
+    // pairs in three groups of two, each group's reads issued after the
+    // previous group's use (all 96 coefficients at once would not fit)
+#pragma unroll
+    for (int pr = 0; pr < 6; ++pr) {
+      const int a = pr < 3 ? 0 : (pr < 5 ? 1 : 2);
+      const int bb = pr < 3 ? pr + 1 : (pr < 5 ? pr - 1 : 3);
+      if ((pr & 1) == 0) sweep_fence(gv);
+      {
+        const double2* cab = reinterpret_cast<const double2*>(er + a * 24 + 6 * bb);
+        const double2* cba = reinterpret_cast<const double2*>(er + bb * 24 + 6 * a);
+        const double2 x0 = cab[0], x1 = cab[1], x2 = cab[2], y0 = cba[0], y1 = cba[1], y2 = cba[2];
+        const double sxx = x0.x + y0.x;   // dx_a dx_b
+        const double sab = x0.y + y1.x;   // dx_a dy_b
+        const double sba = x1.x + y0.y;   // dy_a dx_b
+        const double syy = x1.y + y1.y;   // dy_a dy_b
+        const double sA = x2.x + y2.x, sB = x2.y + y2.y;
+        gdx[a] += sxx * dx[bb] + sab * dy[bb];
+        gdx[bb] += sxx * dx[a] + sba * dy[a];
+        gdy[a] += sba * dx[bb] + syy * dy[bb];
+        gdy[bb] += sab * dx[a] + syy * dy[a];
+        gal[a] += sA * al[bb];
+        gal[bb] += sA * al[a];
+        gbe[a] += sB * be[bb];
+        gbe[bb] += sB * be[a];
+      }
+    }
+    const double gs = sqrt(double(fem::ONE_THIRD_F32));
+    const double rr = (p == 0 || p == 3) ? gs : -gs;
+    const double ss = (p < 2) ? gs : -gs;
+    const double q4 = 0.25;
+    const double dr[4] = {q4 + q4 * ss, -(q4 + q4 * ss), -(q4 - q4 * ss), q4 - q4 * ss};
+    const double ds[4] = {q4 + q4 * rr, q4 - q4 * rr, -(q4 - q4 * rr), -(q4 + q4 * rr)};
+    const double fa[4] = {1.0 + ss, 1.0 + ss, 1.0 - ss, 1.0 - ss}, fb[4] = {1.0 + rr, 1.0 - rr, 1.0 - rr, 1.0 + rr};
+    double s7[7] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      s7[0] += ((dx[a] * gdx[a] + dy[a] * gdy[a]) + al[a] * gal[a]) + be[a] * gbe[a];
+      s7[1] += gdx[a] * dr[a];
+      s7[2] += gdx[a] * ds[a];
+      s7[3] += gdy[a] * dr[a];
+      s7[4] += gdy[a] * ds[a];
+      s7[5] += gal[a] * fa[a];
+      s7[6] += gbe[a] * fb[a];
+    }
+    s7[0] *= 0.5;
+#pragma unroll
+    for (int k = 1; k < 7; ++k) s7[k] *= w;
+    fem::plate_point_reverse(xl, p, s7, gxl);
+  }
+  plate_write_flat(A, e, p, live, xl, E, nu, t, kk, P, gxl);
 }
 
 // Compliance sensitivities of plates by strain energy: four lanes per element,
@@ -650,8 +872,9 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
                     : 0u;
     if (t == PLATE) {
       const unsigned blocks = unsigned((n + 63) / 64);
-      if (mode == MODE_DLDK && d->adj_flat) plate_adjoint_kernel<true><<<blocks, 256, 0, s>>>(A);
-      else if (mode == MODE_DLDK) plate_adjoint_kernel<false><<<blocks, 256, 0, s>>>(A);
+      if (mode == MODE_DLDK && d->adj_flat)
+        plate_adjoint_flat_kernel<<<unsigned((n + 15) / 16), 64, 0, s>>>(A);
+      else if (mode == MODE_DLDK) plate_adjoint_kernel<<<blocks, 256, 0, s>>>(A);
       else if (d->adj_flat) plate_compliance_kernel<true><<<blocks, 256, 0, s>>>(A);
       else plate_compliance_kernel<false><<<blocks, 256, 0, s>>>(A);
     } else {
