@@ -310,6 +310,9 @@ def main():
                          "calibrates the FETCH_SIZE / WRITE_SIZE counters of a profiled run)")
     ap.add_argument("--patch-nodes", type=int, default=0, help="fused plate: owned nodes per patch (0 = default)")
     ap.add_argument("--wg-per-cu", type=int, default=-1, help="fused plate: resident workgroups per CU (0 = one per patch)")
+    ap.add_argument("--assembly-events", type=int, default=0,
+                    help="1: the library records its own events around every assembly launch (fem_last_assembly_ms); "
+                         "the timed loop is bracketed by the bench's events either way")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU only; --gpus %d is not supported "
@@ -341,6 +344,7 @@ def main():
         model.set_option("separate_lanes", args.separate_lanes)
     if args.plate_kernel >= 0:
         model.set_option("plate_kernel", args.plate_kernel)
+    model.set_option("time_assembly", args.assembly_events)
     t0 = time.perf_counter()
     model.build_pattern()
     setup_s = time.perf_counter() - t0

@@ -103,6 +103,7 @@ struct fem_model {
                           // 3 = balanced lanes (plate_lanes; falls back to plate_blocks)
   bool force_generic = false;
   bool wide_loads = true;  // backward block gathers and PCG SpMV: 16-byte loads when the values fit 32-bit offsets
+  bool time_assembly = true;  // events around each assembly launch (fem_last_assembly_ms)
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
   std::vector<uint64_t> pat_bptr;
@@ -1903,12 +1904,13 @@ int fem_assemble_into(fem_model* m, double* val_device, void* stream) {
   if (!m || (!val_device && m->nnz)) return FEM_ERR_ARGUMENT;
   int rc = ensure_ready(m);
   if (rc) return rc;
-  std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream)
-                  : m->path == "plate_lanes" ? femdev::assemble_plate_lanes(m->dev, val_device, stream)
-                  : m->path == "plate_blocks" ? femdev::assemble_plate_blocks(m->dev, val_device, stream)
-                  : m->path == "plate_corners" ? femdev::assemble_plate_corners(m->dev, val_device, stream)
-                  : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream)
-                                           : femdev::assemble_generic(m->dev, val_device, stream);
+  const bool t = m->time_assembly;
+  std::string e = m->path == "plate_fused" ? femdev::assemble_plate_fused(m->dev, val_device, stream, t)
+                  : m->path == "plate_lanes" ? femdev::assemble_plate_lanes(m->dev, val_device, stream, t)
+                  : m->path == "plate_blocks" ? femdev::assemble_plate_blocks(m->dev, val_device, stream, t)
+                  : m->path == "plate_corners" ? femdev::assemble_plate_corners(m->dev, val_device, stream, t)
+                  : m->path == "bar_owner" ? femdev::assemble_bar_owner(m->dev, val_device, stream, t)
+                                           : femdev::assemble_generic(m->dev, val_device, stream, t);
   if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
   if (val_device == femdev::values(m->dev)) m->own_vals = true;
   return FEM_OK;
@@ -2263,6 +2265,11 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
   } else if (k == "wide_loads") {
     m->wide_loads = value != 0;
     return FEM_OK;  // speed only
+  } else if (k == "time_assembly") {
+    // 0: no events around the assembly launches (two fewer packets between
+    // back-to-back assemblies); fem_last_assembly_ms then keeps the last timed one
+    m->time_assembly = value != 0;
+    return FEM_OK;
   } else {
     return fail(m, FEM_ERR_ARGUMENT, "unknown option " + k);
   }

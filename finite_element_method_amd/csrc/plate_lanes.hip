@@ -279,20 +279,26 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER
         I[R_B + 2] = __builtin_fma(wbe1, be1, I[R_B + 2]);
       }
     }
-    // points {0, 1} + points {2, 3}, the same sum in both lanes of the pair
+    // points {0, 1} + points {2, 3}: lane half 0 writes integrals 0..20, half 1
+    // integrals 21..41, so each lane sends the partner the partial the partner
+    // writes and adds the one it receives to its own (IEEE addition commutes:
+    // the sum is the same whichever lane forms it).  S[k] = integral k of this
+    // lane's half (k + 21 h).
     double* r = rec + le * PL_REC;
+    double S[PL_NINT / 2];
 #pragma unroll
-    for (int k = 0; k < PL_NINT; ++k) {
-      const double o = swap_pair(I[k]);
-      I[k] = I[k] + o;  // IEEE addition commutes: both lanes of the pair hold the same sum
+    for (int k = 0; k < PL_NINT / 2; ++k) {
+      const double mine = half ? I[k + PL_NINT / 2] : I[k];
+      const double theirs = swap_pair(half ? I[k] : I[k + PL_NINT / 2]);
+      S[k] = mine + theirs;
     }
     if (live) {
       if (half == 0) {
 #pragma unroll
         for (int k = 0; k < 20; k += 2) {
-          *reinterpret_cast<double2*>(r + k) = make_double2(I[k], I[k + 1]);
+          *reinterpret_cast<double2*>(r + k) = make_double2(S[k], S[k + 1]);
         }
-        r[20] = I[20];
+        r[20] = S[20];
         const fem::PlateMat mt = fem::plate_material(g3.x, g3.y, g4.x, g4.y);
         // bending constants are the membrane ones times t^2 / 12 (plate.rs:532-636)
         const double tb = (g4.x * g4.x) / 12.0;
@@ -300,10 +306,10 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER
         *reinterpret_cast<double2*>(r + R_MAT + 2) = make_double2(mt.cm22, tb);
         *reinterpret_cast<double2*>(r + R_MAT + 4) = make_double2(mt.cs, 0.0);
       } else {
-        r[21] = I[21];
+        r[21] = S[0];
 #pragma unroll
         for (int k = 22; k < PL_NINT; k += 2) {
-          *reinterpret_cast<double2*>(r + k) = make_double2(I[k], I[k + 1]);
+          *reinterpret_cast<double2*>(r + k) = make_double2(S[k - 21], S[k - 20]);
         }
         // shear edge vectors (plate.rs:452-508): e12, e43, e14, e23
         const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
