@@ -12,8 +12,8 @@
 //
 // Determinism: per-element results are written once; the four lanes of a plate
 // element combine with a fixed butterfly; nodal coordinate gradients are
-// per-(element, node) rows summed per node in element-insertion order by
-// node_gather_kernel.  No atomics.
+// per-(element, node) rows, written to node-major slots and summed per node in
+// element-insertion order by node_gather_kernel.  No atomics.
 #include <hip/hip_runtime.h>
 
 #include <string>
@@ -48,7 +48,8 @@ struct AdjArgs {
   const uint2* kpack;
   const int64_t* row_ptr;
   double* dparam;  // n x params_per, may be null
-  double* xs;      // n x nodes_per x 3, may be null
+  double* xs;      // coordinate-gradient rows in node-major slots (3 doubles each), may be null
+  const uint32_t* xpos;  // per (element, node) of this type: its slot in xs
   double rel_tol;
   uint32_t g_bytes;  // dL/dK: bytes of g when below 4 GiB (wide range-checked loads), else 0
 };
@@ -252,8 +253,9 @@ __device__ inline void plate_write(const AdjArgs& A, size_t e, int p, bool live,
   fem::plate_adjoint_finish(P, gxl, hx, hy, el.loc, el.R, el.E, el.nu, el.t, el.k, gp, gX);
   if (A.dparam) A.dparam[4 * e + p] = fem::sel4(gp[0], gp[1], gp[2], gp[3], p);
   if (A.xs) {
+    double* x = A.xs + 3 * size_t(A.xpos[4 * e + p]);
 #pragma unroll
-    for (int i = 0; i < 3; ++i) A.xs[(4 * e + p) * 3 + i] = fem::sel4(gX[0][i], gX[1][i], gX[2][i], gX[3][i], p);
+    for (int i = 0; i < 3; ++i) x[i] = fem::sel4(gX[0][i], gX[1][i], gX[2][i], gX[3][i], p);
   }
 }
 
@@ -350,24 +352,22 @@ __global__ __launch_bounds__(256) void plate_adjoint_kernel(AdjArgs A) {
   plate_write(A, e, p, live, el, P, gxl, hx, hy);
 }
 
-// plate_write for a flat plate: the frame is the identity and the local
-// coordinates relative to node 3 are xl (z = 0)
-__device__ inline void plate_write_flat(const AdjArgs& A, size_t e, int p, bool live, const double xl[6], double E,
-                                        double nu, double t, double k, double P[7], double gxl[6]) {
-  PlateElem el;
-  el.loc[0][0] = xl[0]; el.loc[0][1] = xl[1];
-  el.loc[1][0] = xl[2]; el.loc[1][1] = xl[3];
-  el.loc[2][0] = 0.0; el.loc[2][1] = 0.0;
-  el.loc[3][0] = xl[4]; el.loc[3][1] = xl[5];
+// plate_write for a flat plate (identity frame, nodes in z = 0, no frame term)
+__device__ inline void plate_write_flat(const AdjArgs& A, size_t e, int p, bool live, double E, double nu, double t,
+                                        double k, double P[7], double gxl[6]) {
 #pragma unroll
-  for (int a = 0; a < 4; ++a) el.loc[a][2] = 0.0;
+  for (int i = 0; i < 7; ++i) P[i] = quad_sum(P[i]);
 #pragma unroll
-  for (int i = 0; i < 9; ++i) el.R[i] = (i % 4 == 0) ? 1.0 : 0.0;
-  el.E = E;
-  el.nu = nu;
-  el.t = t;
-  el.k = k;
-  plate_write(A, e, p, live, el, P, gxl, 0.0, 0.0);
+  for (int i = 0; i < 6; ++i) gxl[i] = quad_sum(gxl[i]);
+  if (!live) return;
+  double gp[4], gX[4][3];
+  fem::plate_adjoint_finish_flat(P, gxl, E, nu, t, k, gp, gX);
+  if (A.dparam) A.dparam[4 * e + p] = fem::sel4(gp[0], gp[1], gp[2], gp[3], p);
+  if (A.xs) {
+    double* x = A.xs + 3 * size_t(A.xpos[4 * e + p]);
+#pragma unroll
+    for (int i = 0; i < 3; ++i) x[i] = fem::sel4(gX[0][i], gX[1][i], gX[2][i], gX[3][i], p);
+  }
 }
 
 // ------------------------------------------------ flat plates, dL/dK
@@ -437,26 +437,25 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void pl
   const uint2 kr = A.kpack[4 * e + p];
   const uint32_t g0 = kr.x, kp = kr.y;
   const int nb = int(kp >> 24);
-  double nxt[13];
+  double nxt[13], nxt2[13];
   load_flat_k13(A.g + g0, nb, int(kp & 63u), nxt);
-  double xl[6], E, nu, t, kk;
+  load_flat_k13(A.g + g0, nb, int((kp >> 6) & 63u), nxt2);
+  double xl[6];
+  fem::PlateMat mat;
   {
     PlateElem el;
     load_plate_elem<true>(A, e, el);
 #pragma unroll
     for (int i = 0; i < 6; ++i) xl[i] = el.xl[i];
-    E = el.E;
-    nu = el.nu;
-    t = el.t;
-    kk = el.k;
+    mat = fem::plate_material(el.E, el.nu, el.t, el.k);
   }
-  const fem::PlateMat mat = fem::plate_material(E, nu, t, kk);
   double* const ep = reinterpret_cast<double*>(pts) + le * PF_PTS;
   double* const er = reinterpret_cast<double*>(rows) + le * PF_ROWS;
-  // point p's data
+  // point p's data; the reciprocals its reverse sweep needs stay with lane p
+  double rdet, rqr, rqs;
   {
     fem::PlatePoint g;
-    fem::plate_point(xl, p, g);
+    fem::plate_point_x(xl, p, g, rdet, rqr, rqs);
     double2* d = reinterpret_cast<double2*>(ep + p * PF_PT);
     d[0] = make_double2(g.w, g.dx[0]);
     d[1] = make_double2(g.dx[1], g.dx[2]);
@@ -518,8 +517,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void pl
   for (int q = 0; q < 4; ++q) {
     double G[13];
 #pragma unroll
-    for (int i = 0; i < 13; ++i) G[i] = nxt[i];
-    if (q < 3) load_flat_k13(A.g + g0, nb, int((kp >> (6 * (q + 1))) & 63u), nxt);
+    for (int i = 0; i < 13; ++i) {
+      G[i] = nxt[i];
+      nxt[i] = nxt2[i];
+    }
+    if (q < 2) load_flat_k13(A.g + g0, nb, int((kp >> (6 * (q + 2))) & 63u), nxt2);
     double2* r = reinterpret_cast<double2*>(er + p * 24 + 6 * q);
     const double2 m0 = r[0], m1 = r[1], m2 = r[2];
     const double Mxx = m0.x, Mxy = m0.y, Myx = m1.x, Myy = m1.y, Aq = m2.x, Bq = m2.y;
@@ -635,9 +637,11 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void pl
     s7[0] *= 0.5;
 #pragma unroll
     for (int k = 1; k < 7; ++k) s7[k] *= w;
-    fem::plate_point_reverse(xl, p, s7, gxl);
+    fem::plate_point_reverse_r(xl, p, s7, rdet, rqr, rqs, gxl);
   }
-  plate_write_flat(A, e, p, live, xl, E, nu, t, kk, P, gxl);
+  // the parameters again (not held through the sweeps)
+  plate_write_flat(A, e, p, live, A.params[4 * e], A.params[4 * e + 1], A.params[4 * e + 2], A.params[4 * e + 3], P,
+                   gxl);
 }
 
 // Compliance sensitivities of plates by strain energy: four lanes per element,
@@ -725,10 +729,12 @@ __global__ __launch_bounds__(256) void truss_adjoint_kernel(AdjArgs A) {
     for (int i = 0; i < 3; ++i) A.dparam[3 * e + i] = gp[i];
   }
   if (A.xs) {
+    double* x0 = A.xs + 3 * size_t(A.xpos[2 * e]);
+    double* x1 = A.xs + 3 * size_t(A.xpos[2 * e + 1]);
 #pragma unroll
     for (int i = 0; i < 3; ++i) {
-      A.xs[(2 * e) * 3 + i] = -gd[i];
-      A.xs[(2 * e + 1) * 3 + i] = gd[i];
+      x0[i] = -gd[i];
+      x1[i] = gd[i];
     }
   }
 }
@@ -799,20 +805,18 @@ __global__ __launch_bounds__(256) void beam_adjoint_kernel(AdjArgs A) {
   }
 }
 
-// nodal coordinate gradient = sum of the node's (element, node) rows in
-// element-insertion order
+// nodal coordinate gradient = sum of the node's (element, node) rows, which
+// sit in consecutive slots in element-insertion order
 __global__ __launch_bounds__(256) void node_gather_kernel(const uint32_t* __restrict__ inc_ptr,
-                                                          const uint32_t* __restrict__ inc,
                                                           const double* __restrict__ xs, size_t n_nodes,
                                                           double* __restrict__ dxyz) {
   const size_t a = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (a >= n_nodes) return;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (uint32_t k = inc_ptr[a]; k < inc_ptr[a + 1]; ++k) {
-    const size_t r = inc[k];
-    s0 += xs[3 * r];
-    s1 += xs[3 * r + 1];
-    s2 += xs[3 * r + 2];
+    s0 += xs[3 * size_t(k)];
+    s1 += xs[3 * size_t(k) + 1];
+    s2 += xs[3 * size_t(k) + 2];
   }
   dxyz[3 * a] = s0;
   dxyz[3 * a + 1] = s1;
@@ -832,7 +836,7 @@ std::string upload_adjoint_maps(Device* d, const AdjointMaps& am) {
   if (!e.empty()) return e;
   e = dupload(d->adj_inc_ptr, am.inc_ptr.data(), am.inc_ptr.size());
   if (!e.empty()) return e;
-  e = dupload(d->adj_inc, am.inc.data(), am.inc.size());
+  e = dupload(d->adj_xpos, am.xpos.data(), am.xpos.size());
   if (!e.empty()) return e;
   for (int t = 0; t < 3; ++t) d->adj_xoff[t] = am.xoff[t];
   d->adj_flat = am.flat;
@@ -865,7 +869,8 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.kpack = reinterpret_cast<const uint2*>(d->adj_kpack);
     A.row_ptr = d->row_ptr;
     A.dparam = dparam[t];
-    A.xs = (dxyz && t != BEAM) ? d->adj_xs + 3 * d->adj_xoff[t] : nullptr;
+    A.xs = (dxyz && t != BEAM) ? d->adj_xs : nullptr;
+    A.xpos = d->adj_xpos + d->adj_xoff[t];
     A.rel_tol = d->rel_tol;
     A.g_bytes = (mode == MODE_DLDK && wide && size_t(d->nnz) * 8 < (size_t(1) << 32))
                     ? uint32_t(size_t(d->nnz) * 8)
@@ -893,7 +898,7 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
   }
   if (dxyz && d->n_nodes) {
     const unsigned blocks = unsigned((d->n_nodes + 255) / 256);
-    node_gather_kernel<<<blocks, 256, 0, s>>>(d->adj_inc_ptr, d->adj_inc, d->adj_xs, d->n_nodes, dxyz);
+    node_gather_kernel<<<blocks, 256, 0, s>>>(d->adj_inc_ptr, d->adj_xs, d->n_nodes, dxyz);
     HIP_TRY(hipGetLastError());
   }
   HIP_TRY(hipEventRecord(d->ev3, s));

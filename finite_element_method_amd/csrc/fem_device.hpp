@@ -209,7 +209,10 @@ struct AdjointMaps {
   std::vector<uint64_t> bmask;    // per block: union structural mask (bit 6 i + j)
   std::vector<uint16_t> boff;     // per block and row i: first entry of the block within the CSR row
   std::vector<uint32_t> inc_ptr;  // N + 1
-  std::vector<uint32_t> inc;      // per node: its (element, node) gradient rows, element-insertion order
+  // per (element, node) gradient row: its slot in node-major order (the rows
+  // of node a fill slots inc_ptr[a] .. inc_ptr[a + 1] - 1 in element-insertion
+  // order), so the kernels write each row where the node sum reads it
+  std::vector<uint32_t> xpos;
   int64_t xoff[3] = {0, 0, 0};    // first gradient row of each element type (beams have none)
   int64_t n_xs_rows = 0;
   bool flat = false;              // plates only, identity frames, standard flat block masks

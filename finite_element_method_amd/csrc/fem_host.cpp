@@ -1292,13 +1292,13 @@ int ensure_adjoint(fem_model* m) {
     for (int p = 0; p < nn; ++p) am.inc_ptr[m->set[r.t].conn[size_t(r.pos) * nn + p] + 1]++;
   }
   for (size_t a = 0; a < N; ++a) am.inc_ptr[a + 1] += am.inc_ptr[a];
-  am.inc.resize(am.inc_ptr[N]);
+  am.xpos.assign(size_t(am.n_xs_rows), 0);
   {
     std::vector<uint32_t> f(am.inc_ptr.begin(), am.inc_ptr.end() - 1);
     for (auto& r : order) {
       const int nn = femdev::NODES_PER[r.t];
       for (int p = 0; p < nn; ++p)
-        am.inc[f[m->set[r.t].conn[size_t(r.pos) * nn + p]]++] = uint32_t(am.xoff[r.t] + int64_t(r.pos) * nn + p);
+        am.xpos[size_t(am.xoff[r.t] + int64_t(r.pos) * nn + p)] = f[m->set[r.t].conn[size_t(r.pos) * nn + p]]++;
     }
   }
   auto e = femdev::upload_adjoint_maps(m->dev, am);

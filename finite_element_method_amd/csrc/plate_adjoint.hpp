@@ -108,6 +108,78 @@ __host__ __device__ inline void plate_point_reverse(const double xl[6], int ip, 
   gxl[5] += kuy * ms_ - kry * pr;
 }
 
+// plate_point (element_math.hpp) that also returns the reciprocals the
+// reverse sweep of the point needs (plate_point_reverse_r)
+__host__ __device__ inline void plate_point_x(const double xl[6], int ip, PlatePoint& o, double& rdet_o,
+                                              double& rqr, double& rqs) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double g = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? g : -g;
+  const double s = (ip < 2) ? g : -g;
+  const double q4 = 0.25;
+  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
+  const double usx = (x1 * ps - x2 * ps) + x4 * ms_, usy = (y1 * ps - y2 * ps) + y4 * ms_;
+  const double urx = (x1 * pr + x2 * mr_) - x4 * pr, ury = (y1 * pr + y2 * mr_) - y4 * pr;
+  const double j00 = usx * q4, j01 = usy * q4, j10 = urx * q4, j11 = ury * q4;
+  const double det = j00 * j11 - j01 * j10;
+  const double rdet = 1.0 / det;
+  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+  const double dr[4] = {q4 * ps, -q4 * ps, -q4 * ms_, q4 * ms_};
+  const double ds[4] = {q4 * pr, q4 * mr_, -q4 * mr_, -q4 * pr};
+#pragma unroll
+  for (int a = 0; a < 4; ++a) {
+    o.dx[a] = i00 * dr[a] + i01 * ds[a];
+    o.dy[a] = i10 * dr[a] + i11 * ds[a];
+  }
+  const double sqr = sqrt(urx * urx + ury * ury), sqs = sqrt(usx * usx + usy * usy);
+  const double mr = sqr * 0.125 * rdet, ms = sqs * 0.125 * rdet;
+  o.al[0] = ps * mr; o.al[1] = o.al[0]; o.al[2] = ms_ * mr; o.al[3] = o.al[2];
+  o.be[0] = pr * ms; o.be[1] = mr_ * ms; o.be[2] = o.be[1]; o.be[3] = o.be[0];
+  o.w = det;
+  rdet_o = rdet;
+  rqr = sqr > 0.0 ? 1.0 / sqr : 0.0;
+  rqs = sqs > 0.0 ? 1.0 / sqs : 0.0;
+}
+
+// plate_point_reverse with the point's reciprocals given: rdet = 1 / det J and
+// rqr, rqs = 1 / |(cx + r bx, cy + r by)|, 1 / |(ax + s bx, ay + s by)| (the
+// MITC4 lengths; 0 where the length is 0), as the forward point formed them
+__host__ __device__ inline void plate_point_reverse_r(const double xl[6], int ip, const double s7[7], double rdet,
+                                                      double rqr, double rqs, double gxl[6]) {
+  const double x1 = xl[0], y1 = xl[1], x2 = xl[2], y2 = xl[3], x4 = xl[4], y4 = xl[5];
+  const double gs = sqrt(double(ONE_THIRD_F32));
+  const double r = (ip == 0 || ip == 3) ? gs : -gs;
+  const double s = (ip < 2) ? gs : -gs;
+  const double q4 = 0.25;
+  const double pr = 1.0 + r, mr_ = 1.0 - r, ps = 1.0 + s, ms_ = 1.0 - s;
+  const double usx = (x1 * ps - x2 * ps) + x4 * ms_, usy = (y1 * ps - y2 * ps) + y4 * ms_;
+  const double urx = (x1 * pr + x2 * mr_) - x4 * pr, ury = (y1 * pr + y2 * mr_) - y4 * pr;
+  const double j00 = usx * q4, j01 = usy * q4, j10 = urx * q4, j11 = ury * q4;
+  const double i00 = j11 * rdet, i01 = -j01 * rdet, i10 = -j10 * rdet, i11 = j00 * rdet;
+  const double sqr = (urx * urx + ury * ury) * rqr, sqs = (usx * usx + usy * usy) * rqs;  // the lengths
+  const double m_r = sqr * 0.125 * rdet, m_s = sqs * 0.125 * rdet;
+  double g_det = s7[0];
+  const double g_i00 = s7[1], g_i01 = s7[2], g_i10 = s7[3], g_i11 = s7[4], g_mr = s7[5], g_ms = s7[6];
+  g_det -= (g_mr * m_r + g_ms * m_s) * rdet;
+  const double g_sqr = g_mr * 0.125 * rdet, g_sqs = g_ms * 0.125 * rdet;
+  double g_j11 = g_i00 * rdet, g_j01 = -g_i01 * rdet, g_j10 = -g_i10 * rdet, g_j00 = g_i11 * rdet;
+  g_det -= (((g_i00 * i00 + g_i01 * i01) + g_i10 * i10) + g_i11 * i11) * rdet;
+  g_j00 += g_det * j11;
+  g_j11 += g_det * j00;
+  g_j01 -= g_det * j10;
+  g_j10 -= g_det * j01;
+  const double kux = g_j00 * q4 + g_sqs * usx * rqs;
+  const double kuy = g_j01 * q4 + g_sqs * usy * rqs;
+  const double krx = g_j10 * q4 + g_sqr * urx * rqr;
+  const double kry = g_j11 * q4 + g_sqr * ury * rqr;
+  gxl[0] += kux * ps + krx * pr;
+  gxl[1] += kuy * ps + kry * pr;
+  gxl[2] += -kux * ps + krx * mr_;
+  gxl[3] += -kuy * ps + kry * mr_;
+  gxl[4] += kux * ms_ - krx * pr;
+  gxl[5] += kuy * ms_ - kry * pr;
+}
+
 // adjoint of the shear vectors of node a (components 1 and 2; component 0 is
 // the constant +-1/2): gv = dL/dv_a, gu = dL/du_a, mapped onto xl through
 //   e12 = (-(y1-y2), x1-x2)/4, e43 = (-y4, x4)/4, e14 = (-(y1-y4), x1-x4)/4, e23 = (-y2, x2)/4
@@ -298,6 +370,38 @@ __host__ __device__ inline void plate_adjoint_finish(const double P[7], const do
   for (int a = 0; a < 4; ++a)
 #pragma unroll
     for (int i = 0; i < 3; ++i) gX[a][i] = (R[0 * 3 + i] * ga[a][0] + R[1 * 3 + i] * ga[a][1]) + R[2 * 3 + i] * ga[a][2];
+}
+
+// plate_adjoint_finish for a plate whose frame is the identity and whose nodes
+// lie in z = 0 with no frame-rotation sums (hx = hy = 0): the rotation term
+// vanishes and the world gradient is the local one.  The constants' reciprocals
+// come from two divisions.
+__host__ __device__ inline void plate_adjoint_finish_flat(const double P[7], const double gxl[6], double E, double nu,
+                                                          double t, double k, double grad_param[4], double gX[4][3]) {
+  const double inv_n = 1.0 / ((1.0 - nu) * (1.0 + nu));  // 1 / (1 - nu^2)
+  const double inv_etk = 1.0 / ((E * t) * k);
+  const double cm = E * t * inv_n;
+  const double cb = E * (t * t * t) * (1.0 / 12.0) * inv_n;
+  const double inv_1pn = (1.0 - nu) * inv_n;  // 1 / (1 + nu)
+  const double cs = E * t * k * 0.5 * inv_1pn;
+  const double h = (1.0 - nu) / 2.0;
+  const double Lm = cm * ((P[0] + nu * P[1]) + h * P[2]);
+  const double Lb = cb * ((P[3] + nu * P[4]) + h * P[5]);
+  const double Ls = cs * P[6];
+  const double dcm = cm * 2.0 * nu * inv_n, dcb = cb * 2.0 * nu * inv_n;
+  grad_param[0] = ((Lm + Lb) + Ls) * (t * k * inv_etk);
+  grad_param[1] = (((P[0] * dcm + P[1] * (cm + nu * dcm)) + P[2] * (h * dcm - 0.5 * cm)) +
+                   ((P[3] * dcb + P[4] * (cb + nu * dcb)) + P[5] * (h * dcb - 0.5 * cb))) -
+                  P[6] * cs * inv_1pn;
+  grad_param[2] = ((Lm + 3.0 * Lb) + Ls) * (E * k * inv_etk);
+  grad_param[3] = Ls * (E * t * inv_etk);
+  const double g3x = -((gxl[0] + gxl[2]) + gxl[4]), g3y = -((gxl[1] + gxl[3]) + gxl[5]);
+  gX[0][0] = gxl[0]; gX[0][1] = gxl[1];
+  gX[1][0] = gxl[2]; gX[1][1] = gxl[3];
+  gX[2][0] = g3x;    gX[2][1] = g3y;
+  gX[3][0] = gxl[4]; gX[3][1] = gxl[5];
+#pragma unroll
+  for (int a = 0; a < 4; ++a) gX[a][2] = 0.0;
 }
 
 // Compliance by strain energy: the share of integration point ip in
