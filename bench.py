@@ -82,7 +82,7 @@ def time_steps(fn, steps, warmup, stream):
     return ev0.elapsed_time(ev1) / steps
 
 
-def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream):
+def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream, workload=None):
     """Backward from dL/dK ~ N(0, 1) on the pattern, compliance sensitivities
     with u ~ N(0, 1e-3), and forward + backward, all on the same mesh."""
     import torch
@@ -103,7 +103,7 @@ def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream):
     bb = backward_bytes(mesh, nnz)
     cb = backward_bytes(mesh, nnz, compliance=True)
     best_ms = min(both_ms, fused_ms)
-    return {"bwd_ms": bwd_ms, "compliance_ms": comp_ms, "fwd_then_bwd_ms": both_ms,
+    out = {"bwd_ms": bwd_ms, "compliance_ms": comp_ms, "fwd_then_bwd_ms": both_ms,
             "fwd_bwd_two_streams_ms": fused_ms, "fwd_plus_bwd_ms": best_ms,
             "fwd_plus_bwd_over_fwd": best_ms / fwd_ms,
             "sensitivities_per_s": mesh.n_elements / (best_ms * 1e-3),
@@ -113,6 +113,15 @@ def sensitivity(model, mesh, nnz, val, fwd_ms, steps, warmup, stream):
             "compliance_roofline": {"bound": "hbm", "achieved": cb / (comp_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
                                     "unit": "GB/s", "frac": cb / (comp_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                                     "algorithmic_bytes_per_launch": cb}}
+    # FP64 arithmetic of the plate backward and compliance launches (committed
+    # counter summary) against the FP64 peak, at the measured step times
+    for key, ms in (("bwd", bwd_ms), ("compliance", comp_ms)):
+        fl = profile_figure("plate_adjoint", workload, key + "_fp64_flops_per_launch") if workload else None
+        if fl is not None:
+            tf = fl / (ms * 1e-3) / 1e12
+            out[key + "_fp64"] = {"flops_per_launch": fl, "achieved_tflops": tf, "peak_tflops": FP64_PEAK_TFLOPS,
+                                  "frac": tf / FP64_PEAK_TFLOPS}
+    return out
 
 
 def separation(model, mesh, nnz, steps, warmup):
@@ -402,7 +411,7 @@ def main():
                        "peak_tflops": FP64_PEAK_TFLOPS, "frac": flops / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS,
                        "mfma_flops_per_launch": mfma, "mfma_frac": mfma / (ms * 1e-3) / 1e12 / FP64_PEAK_TFLOPS}
     if not args.no_grad:
-        out["sensitivity"] = sensitivity(model, mesh, nnz, val, ms, args.steps, args.warmup, stream)
+        out["sensitivity"] = sensitivity(model, mesh, nnz, val, ms, args.steps, args.warmup, stream, args.workload)
     if not args.no_separate:
         out["separation"] = separation(model, mesh, nnz, args.steps, args.warmup)
         if args.pcg_iters > 0:
