@@ -823,6 +823,86 @@ std::vector<V> plate_local_k(const Model<V>& m, const Plate<V>& e, unsigned part
     for (int i = 0; i < 4; ++i) k[(6 * i + 5) * n + 6 * i + 5] += V(1.0f);  // KROT6, plate.rs:716-721
   return k;
 }
+// ---------------------------------------------------------------- magnitudes
+// Term magnitudes of the element matrices (test infrastructure for the parity
+// tolerance): every product of the stiffness formulas above taken with the
+// absolute value of each factor and summed, i.e. sum_terms prod |factor|, and
+// rotated as |T|^T |k| |T|.  Any two evaluations of an entry that sum the same
+// terms in different orders differ by at most 2 gamma_n times this magnitude,
+// gamma_n ~ n u for chains of n roundings (Higham, Accuracy and Stability of
+// Numerical Algorithms, 2nd ed., sec. 3.1), whatever cancellation the signed
+// sum has.  Geometric factors (local coordinates, Jacobian, frame) enter as
+// computed.
+template <class V>
+std::vector<V> rotate_tkt_abs(const std::vector<V>& k, const std::array<V, 9>& R, int n) {
+  std::array<V, 9> Ra;
+  for (int i = 0; i < 9; ++i) Ra[i] = std::fabs(R[i]);
+  return rotate_tkt(k, Ra, n);
+}
+template <class V>
+std::vector<V> abs_all(std::vector<V> a) {
+  for (auto& x : a) x = std::fabs(x);
+  return a;
+}
+// beam_local_k with every term |B_t[i]| |B_t[j]| |scale_t|
+template <class V>
+std::vector<V> beam_local_k_abs(const Beam<V>& e, V length) {
+  const int n = 12;
+  V r = V(0), alpha = V(2);
+  V inv_j = V(1) / bar_jacobian(length, r);
+  V det_j = bar_jacobian(length, r);
+  V G = e.E / (V(2) * (V(1) + e.nu));
+  V cs[6] = {e.A * e.E, G * e.A * e.k, G * e.A * e.k, G * e.it, e.E * e.i22p, e.E * e.i11p};
+  // rows of the six terms (beam_local_k's bu .. btw): entries at disjoint positions
+  std::vector<V> B(6 * n, V(0));
+  const V d1 = std::fabs(bar_dh1(r) * inv_j), d2 = std::fabs(bar_dh2(r) * inv_j);
+  const V h1 = std::fabs(bar_h1(r)), h2 = std::fabs(bar_h2(r));
+  B[0 * n + 0] = d1; B[0 * n + 6] = d2;
+  B[1 * n + 1] = d1; B[1 * n + 7] = d2; B[1 * n + 5] = h1; B[1 * n + 11] = h2;
+  B[2 * n + 2] = d1; B[2 * n + 8] = d2; B[2 * n + 4] = h1; B[2 * n + 10] = h2;
+  B[3 * n + 3] = d1; B[3 * n + 9] = d2;
+  B[4 * n + 4] = d1; B[4 * n + 10] = d2;
+  B[5 * n + 5] = d1; B[5 * n + 11] = d2;
+  std::vector<V> k(n * n, V(0));
+  for (int t = 0; t < 6; ++t) {
+    const V scale = std::fabs(cs[t] * det_j * alpha);
+    for (int i = 0; i < n; ++i)
+      for (int j = 0; j < n; ++j) k[i * n + j] += (B[t * n + i] * B[t * n + j]) * scale;
+  }
+  return k;
+}
+// plate_local_k with |B|^T |C| |B| |w| per integration point and the drilling term
+template <class V>
+std::vector<V> plate_local_k_abs(const Model<V>& m, const Plate<V>& e) {
+  const int n = 24;
+  QuadLocal<V> q = quad_local(m, e);
+  V ips[4][2] = {{e.ip_rs[0], e.ip_rs[0]}, {e.ip_rs[1], e.ip_rs[0]}, {e.ip_rs[1], e.ip_rs[1]}, {e.ip_rs[0], e.ip_rs[1]}};
+  std::vector<V> k(n * n, V(0));
+  for (int ip = 0; ip < 4; ++ip) {
+    V det;
+    std::vector<V> bm, bb, bs;
+    plate_b(q, ips[ip][0], ips[ip][1], bm, bb, bs, det);
+    V nu = e.nu;
+    V cm_mul = e.E * e.t / (V(1) - nu * nu);
+    V cb_mul = e.E * (e.t * e.t * e.t) / (V(12) * (V(1) - nu * nu));
+    V cs_mul = e.E * e.t * e.k / (V(2) * (V(1) + nu));
+    std::vector<V> base = {V(1), nu, V(0), nu, V(1), V(0), V(0), V(0), (V(1) - nu) / V(2)};
+    std::vector<V> cm(9), cb(9);
+    for (int i = 0; i < 9; ++i) { cm[i] = std::fabs(base[i] * cm_mul); cb[i] = std::fabs(base[i] * cb_mul); }
+    std::vector<V> cs = {std::fabs(cs_mul), V(0), V(0), std::fabs(cs_mul)};
+    auto term = [&](const std::vector<V>& B, const std::vector<V>& C, int rows) {
+      auto Ba = abs_all(B);
+      auto Bt = transpose(Ba, rows, n);
+      return matmul(matmul(Bt, C, n, rows, rows), Ba, n, rows, n);
+    };
+    auto km = term(bm, cm, 3), kb = term(bb, cb, 3), ks = term(bs, cs, 2);
+    const V w = std::fabs(det);
+    for (int i = 0; i < n * n; ++i) k[i] += (km[i] + kb[i] + ks[i]) * w;
+  }
+  for (int i = 0; i < 4; ++i) k[(6 * i + 5) * n + 6 * i + 5] += V(1.0f);
+  return k;
+}
+
 std::vector<uint8_t> plate_local_pattern() {
   std::vector<uint8_t> P(576, 0);
   for (int a = 0; a < 4; ++a)
@@ -1051,6 +1131,47 @@ int64_t export_csr(Model<V>& m, int mode) {
   }
   for (int64_t r = 0; r < ndof; ++r) m.row_ptr[r + 1] += m.row_ptr[r];
   return int64_t(m.val.size());
+}
+
+// Term magnitude of every entry of the structural CSR (export_csr mode 1: the
+// same entries in the same order): the sum over the entry's contributions of
+// the element magnitude matrices (|T|^T k^ |T|, k^ from *_local_k_abs).
+template <class V>
+void export_abs(const Model<V>& m, std::vector<double>& out) {
+  std::vector<V> ta(m.t_val.size(), V(0));
+  for (const auto& er : m.erange) {
+    std::vector<V> K;
+    if (er.type == 0) {
+      const auto& e = m.trusses.at(er.number);
+      K = rotate_tkt_abs(abs_all(truss_local_k(e, norm3(element_vector(m, e.n1, e.n2)))), e.R, 6);
+    } else if (er.type == 1) {
+      const auto& e = m.beams.at(er.number);
+      K = rotate_tkt_abs(beam_local_k_abs(e, norm3(element_vector(m, e.n1, e.n2))), e.R, 12);
+    } else {
+      const auto& e = m.plates.at(er.number);
+      K = rotate_tkt_abs(plate_local_k_abs(m, e), e.R, 24);
+    }
+    for (size_t t = er.t0; t < er.t1; ++t) ta[t] = K[m.t_local[t]];
+  }
+  std::vector<size_t> order;
+  for (size_t i = 0; i < m.t_row.size(); ++i)
+    if (m.t_structural[i]) order.push_back(i);
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) {
+    return m.t_row[a] != m.t_row[b] ? m.t_row[a] < m.t_row[b] : m.t_col[a] < m.t_col[b];
+  });
+  out.clear();
+  size_t i = 0;
+  while (i < order.size()) {
+    size_t j = i;
+    V acc = ta[order[i]];
+    while (j + 1 < order.size() && m.t_row[order[j + 1]] == m.t_row[order[i]] &&
+           m.t_col[order[j + 1]] == m.t_col[order[i]]) {
+      ++j;
+      acc += ta[order[j]];
+    }
+    out.push_back(double(acc));
+    i = j + 1;
+  }
 }
 
 // ---------------------------------------------------------------- boundary conditions
@@ -1610,6 +1731,15 @@ void om_export_copy(AnyModel* am, int64_t* row_ptr, int32_t* col_idx, double* va
     std::memcpy(val, m.val.data(), m.val.size() * sizeof(double));
   };
   if (am->f32) cp(*am->f); else cp(*am->d);
+}
+// term magnitudes of the structural CSR's entries (om_export(., 1) order);
+// returns the number of entries written
+int64_t om_export_magnitude(AnyModel* am, double* out, int64_t cap) {
+  std::vector<double> v;
+  if (am->f32) export_abs(*am->f, v); else export_abs(*am->d, v);
+  if (int64_t(v.size()) > cap) return -1;
+  std::memcpy(out, v.data(), v.size() * sizeof(double));
+  return int64_t(v.size());
 }
 int64_t om_num_contributions(AnyModel* am) { return int64_t(am->f32 ? am->f->t_val.size() : am->d->t_val.size()); }
 // largest |value| among non-structural contributions, relative to the largest |value| overall

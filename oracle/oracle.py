@@ -59,6 +59,8 @@ def lib():
         L.om_ndof.restype = i64
         L.om_ndof.argtypes = [vp]
         L.om_export_copy.argtypes = [vp, vp, vp, vp]
+        L.om_export_magnitude.restype = i64
+        L.om_export_magnitude.argtypes = [vp, vp, i64]
         L.om_num_contributions.restype = i64
         L.om_num_contributions.argtypes = [vp]
         L.om_structural_violations.restype = i64
@@ -376,6 +378,19 @@ class OracleModel:
         val = np.zeros(nnz, dtype=np.float64)
         lib().om_export_copy(self._h, _ptr(row_ptr), _ptr(col), _ptr(val))
         return row_ptr, col, val
+
+    def csr_magnitude(self):
+        """Term magnitude of every entry of csr(structural=True), in its order:
+        the sum over the entry's element contributions of the stiffness terms
+        taken with the absolute value of each factor (|T|^T k^ |T|).  Two
+        evaluations of an entry that sum the same terms in different orders
+        differ by at most about 2 n u times this (u = 2^-53, n roundings per
+        term), however much the signed sum cancels."""
+        nnz = lib().om_export(self._h, 1)
+        out = np.zeros(nnz, dtype=np.float64)
+        if lib().om_export_magnitude(self._h, _ptr(out), nnz) != nnz:
+            raise OracleError("magnitude export size mismatch")
+        return out
 
     def add_displacement(self, number, dof, value):
         """FEM::add_displacement (methods_for_bc_data_handle.rs:176-203); dof 0..5 = X..ThZ."""

@@ -23,7 +23,7 @@ import scipy.sparse as sp
 from finite_element_method_amd import PLATE, FEM, FEMBatch, meshgen
 from oracle.oracle import OracleModel
 from tests import adjoint_ref as ar
-from tests.parity import compare
+from tests.parity import compare, oracle_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -50,7 +50,7 @@ def test_plate_512_whole_csr_matches_oracle(gpu):
     m = FEM.from_mesh(mesh)
     got = m.stiffness_csr()
     print("plate 512 path", m.assembly_path(), "nnz", len(got[2]), flush=True)
-    ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)
+    ref = oracle_reference(meshgen.load_oracle(mesh, OracleModel))
     r = compare(got, ref)
     print("plate 512 whole CSR", r, flush=True)
     assert r["pattern_equal"] and r["max_ratio"] <= 1.0
@@ -93,7 +93,7 @@ def test_frame64_translations(frame64):
 
 def test_frame64_whole_csr_matches_oracle(frame64):
     mesh, m, got = frame64
-    ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)
+    ref = oracle_reference(meshgen.load_oracle(mesh, OracleModel))
     r = compare(got, ref)
     print("beam frame 64 whole CSR", r, flush=True)
     assert r["pattern_equal"] and r["max_ratio"] <= 1.0
@@ -115,7 +115,7 @@ def test_batched_256(gpu):
         alone = model.stiffness_csr()
         assert _same(got, alone), i
         if i in check:
-            r = compare(got, meshgen.load_oracle(mesh, OracleModel).csr(structural=True))
+            r = compare(got, oracle_reference(meshgen.load_oracle(mesh, OracleModel)))
             print("  mesh", i, mesh.kind, mesh.n_elements, model.assembly_path(), r, flush=True)
             assert r["pattern_equal"] and r["max_ratio"] <= 1.0, (i, r)
 

@@ -12,7 +12,7 @@ import pytest
 from finite_element_method_amd import FEM, meshgen
 from finite_element_method_amd.meshgen import Mesh
 from oracle.oracle import OracleModel
-from tests.parity import compare
+from tests.parity import compare, oracle_reference
 
 pytestmark = pytest.mark.gpu
 
@@ -53,7 +53,7 @@ def _fan(k, seed):
                                     (8, "generic")])
 def test_plate_fan_matches_oracle(gpu, k, path):
     mesh = _fan(k, seed=k)
-    ref = meshgen.load_oracle(mesh, OracleModel).csr(structural=True)
+    ref = oracle_reference(meshgen.load_oracle(mesh, OracleModel))
     m = FEM.from_mesh(mesh)
     r = compare(m.stiffness_csr(), ref)
     print("fan", k, m.assembly_path(), r, flush=True)
