@@ -597,7 +597,10 @@ class FEMBatch:
             raise DeviceError(lib().fem_batch_last_error(self._h).decode())
 
     def autotune(self):
-        """Times candidate launch shapes, keeps the fastest, captures the graph."""
+        """Times candidate launch shapes and the groups' schedule (concurrent
+        or serial), keeps the fastest, captures the graph.  Tuned results are
+        cached per (gfx arch, batch signature) in the process and in the tune
+        cache file (FEM_AMD_TUNE_CACHE, default ~/.cache/fem_amd/tune.txt)."""
         self._call(lib().fem_batch_autotune(self._h))
         return self.config()
 

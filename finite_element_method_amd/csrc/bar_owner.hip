@@ -54,6 +54,8 @@ struct BarArgs {
   const uint32_t* __restrict__ inc_ptr;
   const uint32_t* __restrict__ inc;
   const uint2* __restrict__ ranges;  // per workgroup: first node, node count
+  const double* __restrict__ slot;   // bar kernel: staged entries (BO_SLOT doubles each), range r from slot_ptr[r]
+  const uint32_t* __restrict__ slot_ptr;
   const int64_t* __restrict__ row_ptr;
   double* __restrict__ val;
 };
@@ -128,12 +130,21 @@ __device__ inline void truss_constants(const BarArgs& A, size_t e, int p, double
   o[4] = (bp * bq) * scale;
 }
 
-// stage one (owned node, incident element) entry: the element constants
-// (truss.rs:132-189, beam.rs:495-605) and its two blocks' masks and offsets
-__device__ inline void stage_entry(const BarArgs& A, BarLds& L, int slot, uint32_t u) {
+// One (owned node, incident element) entry's staged constants, as a slot of
+// BO_SLOT doubles: the element constants (truss.rs:132-189, beam.rs:495-605;
+// BO_DATA doubles), the masks of its blocks (p, p) and (p, 1 - p), their
+// per-row offsets (12 x u16) and the element type.  The slots are formed once
+// per model change by bar_slot_kernel, so that the assembly reads each entry
+// with one load instead of a chain of dependent ones.
+constexpr int BO_SLOT = 24;
+static_assert(BO_DATA + 2 + 3 + 1 <= BO_SLOT, "slot layout");
+
+__device__ inline void form_slot(const BarArgs& A, uint32_t u, double* __restrict__ slot) {
   const int type = int(u >> 31), p = int(u & 1u);
   const size_t e = (u >> 1) & 0x3FFFFFFFu;
-  double* o = L.data + slot * BO_DATA;
+  double o[BO_DATA];
+#pragma unroll
+  for (int k = 0; k < BO_DATA; ++k) o[k] = 0.0;
   if (type == 0) {
     truss_constants(A, e, p, o);
   } else {
@@ -157,14 +168,64 @@ __device__ inline void stage_entry(const BarArgs& A, BarLds& L, int slot, uint32
     o[16] = bq;
   }
   const uint32_t ks = A.bid[type][4 * e + 3 * p], kn = A.bid[type][4 * e + 2 * p + (1 - p)];
-  L.mask[slot][0] = A.bmask[ks];
-  L.mask[slot][1] = A.bmask[kn];
+  uint16_t off[12];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    L.off[slot][0][i] = A.boff[6 * size_t(ks) + i];
-    L.off[slot][1][i] = A.boff[6 * size_t(kn) + i];
+    off[i] = A.boff[6 * size_t(ks) + i];
+    off[6 + i] = A.boff[6 * size_t(kn) + i];
   }
-  L.type[slot] = uint32_t(type);
+  double w[BO_SLOT];
+#pragma unroll
+  for (int k = 0; k < BO_DATA; ++k) w[k] = o[k];
+  w[BO_DATA] = __builtin_bit_cast(double, A.bmask[ks]);
+  w[BO_DATA + 1] = __builtin_bit_cast(double, A.bmask[kn]);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const uint64_t v = uint64_t(off[4 * m]) | uint64_t(off[4 * m + 1]) << 16 | uint64_t(off[4 * m + 2]) << 32 |
+                       uint64_t(off[4 * m + 3]) << 48;
+    w[BO_DATA + 2 + m] = __builtin_bit_cast(double, v);
+  }
+  w[BO_DATA + 5] = __builtin_bit_cast(double, uint64_t(type));
+#pragma unroll
+  for (int k = BO_DATA + 6; k < BO_SLOT; ++k) w[k] = 0.0;
+  double2* dst = reinterpret_cast<double2*>(slot);
+#pragma unroll
+  for (int k = 0; k < BO_SLOT / 2; ++k) dst[k] = make_double2(w[2 * k], w[2 * k + 1]);
+}
+
+// slot s <- entry slot_inc[s]
+__global__ __launch_bounds__(256) void bar_slot_kernel(BarArgs A, const uint32_t* __restrict__ slot_inc, size_t n,
+                                                       double* __restrict__ slot) {
+  const size_t s = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
+  if (s >= n) return;
+  form_slot(A, A.inc[slot_inc[s]], slot + BO_SLOT * s);
+}
+
+// a slot into the workgroup's staged entry `slot_idx`
+__device__ inline void load_slot(const double* __restrict__ src, BarLds& L, int slot_idx) {
+  const double2* s2 = reinterpret_cast<const double2*>(src);
+  double w[BO_SLOT];
+#pragma unroll
+  for (int k = 0; k < BO_SLOT / 2; ++k) {
+    const double2 v = s2[k];
+    w[2 * k] = v.x;
+    w[2 * k + 1] = v.y;
+  }
+  double* o = L.data + slot_idx * BO_DATA;
+#pragma unroll
+  for (int k = 0; k < BO_DATA; ++k) o[k] = w[k];
+  L.mask[slot_idx][0] = __builtin_bit_cast(uint64_t, w[BO_DATA]);
+  L.mask[slot_idx][1] = __builtin_bit_cast(uint64_t, w[BO_DATA + 1]);
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const uint64_t v = __builtin_bit_cast(uint64_t, w[BO_DATA + 2 + m]);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int idx = 4 * m + j;
+      L.off[slot_idx][idx / 6][idx % 6] = uint16_t(v >> (16 * j));
+    }
+  }
+  L.type[slot_idx] = uint32_t(__builtin_bit_cast(uint64_t, w[BO_DATA + 5]));
 }
 
 // row i of an entry's blocks (p, p) -> self and (p, 1 - p) -> nb
@@ -222,7 +283,7 @@ __global__ __launch_bounds__(BO_THREADS) void bar_owner_kernel(BarArgs A) {
   const int64_t base = A.row_ptr[6 * size_t(a0)];
   const int nv = int(A.row_ptr[6 * size_t(a0 + na)] - base);  // <= BO_VALUES (cut on the host)
   if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
-  if (t < int(ne)) stage_entry(A, L, t, A.inc[e0 + t]);
+  if (t < int(ne)) load_slot(A.slot + BO_SLOT * size_t(A.slot_ptr[blockIdx.x] + t), L, t);
   __syncthreads();
   const int j = t / 6, i = t % 6;
   if (j < int(na)) {
@@ -342,6 +403,46 @@ __global__ __launch_bounds__(BO_THREADS) void truss_owner_kernel(BarArgs A) {
 
 }  // namespace
 
+namespace {
+
+BarArgs bar_args(Device* d, double* val) {
+  BarArgs A;
+  A.xyz = d->xyz;
+  for (int t = 0; t < 2; ++t) {
+    A.conn[t] = d->conn[t];
+    A.params[t] = d->params[t];
+    A.R[t] = d->R[t];
+    A.bid[t] = d->bo_bid[t];
+  }
+  A.derived = d->derived;
+  A.tdir = d->truss_dir;
+  A.bmask = d->bo_bmask;
+  A.boff = d->bo_boff;
+  A.inc_ptr = d->bo_inc_ptr;
+  A.inc = d->bo_inc;
+  A.ranges = reinterpret_cast<const uint2*>(d->bo_ranges);
+  A.slot = d->bo_slot;
+  A.slot_ptr = d->bo_slot_ptr;
+  A.row_ptr = d->row_ptr;
+  A.val = val;
+  return A;
+}
+
+// (re)forms the bar kernel's entry slots from the current coordinates,
+// frames and parameters
+std::string fill_bar_slots(Device* d, hipStream_t s) {
+  if (d->bo_n_slots) {
+    BarArgs A = bar_args(d, nullptr);
+    bar_slot_kernel<<<unsigned((d->bo_n_slots + 255) / 256), 256, 0, s>>>(A, d->bo_slot_inc, d->bo_n_slots,
+                                                                          d->bo_slot);
+    HIP_TRY(hipGetLastError());
+  }
+  d->bo_slot_dirty = false;
+  return "";
+}
+
+}  // namespace
+
 std::string upload_bar_maps(Device* d, const BarMaps& bm) {
   std::string e;
   for (int t = 0; t < 2; ++t)
@@ -352,30 +453,26 @@ std::string upload_bar_maps(Device* d, const BarMaps& bm) {
   if (!(e = dupload(d->bo_ranges, bm.ranges.data(), bm.ranges.size())).empty()) return e;
   d->bo_n_ranges = bm.n_bar_ranges;
   d->bo_n_truss_ranges = bm.ranges.size() / 2 - bm.n_bar_ranges;
-  return dupload(d->bo_inc, bm.inc.data(), bm.inc.size());
+  if (!(e = dupload(d->bo_inc, bm.inc.data(), bm.inc.size())).empty()) return e;
+  if (!(e = dupload(d->bo_slot_inc, bm.slot_inc.data(), bm.slot_inc.size())).empty()) return e;
+  if (!(e = dupload(d->bo_slot_ptr, bm.slot_ptr.data(), bm.slot_ptr.size())).empty()) return e;
+  d->bo_n_slots = bm.slot_inc.size();
+  if (!(e = dalloc(d->bo_slot, size_t(BO_SLOT) * d->bo_n_slots)).empty()) return e;
+  e = fill_bar_slots(d, nullptr);
+  if (!e.empty()) return e;
+  HIP_TRY(hipDeviceSynchronize());
+  return "";
 }
 
 std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed) {
   hipStream_t s = static_cast<hipStream_t>(stream);
+  if (d->bo_slot_dirty) {  // coordinates or parameters changed since the slots were formed
+    auto e = fill_bar_slots(d, s);
+    if (!e.empty()) return e;
+  }
   if (timed) HIP_TRY(hipEventRecord(d->ev0, s));
   if (d->bo_n_ranges + d->bo_n_truss_ranges) {
-    BarArgs A;
-    A.xyz = d->xyz;
-    for (int t = 0; t < 2; ++t) {
-      A.conn[t] = d->conn[t];
-      A.params[t] = d->params[t];
-      A.R[t] = d->R[t];
-      A.bid[t] = d->bo_bid[t];
-    }
-    A.derived = d->derived;
-    A.tdir = d->truss_dir;
-    A.bmask = d->bo_bmask;
-    A.boff = d->bo_boff;
-    A.inc_ptr = d->bo_inc_ptr;
-    A.inc = d->bo_inc;
-    A.ranges = reinterpret_cast<const uint2*>(d->bo_ranges);
-    A.row_ptr = d->row_ptr;
-    A.val = val;
+    BarArgs A = bar_args(d, val);
     if (d->bo_n_ranges) {
       bar_owner_kernel<<<unsigned(d->bo_n_ranges), BO_THREADS, 0, s>>>(A);
       HIP_TRY(hipGetLastError());

@@ -123,6 +123,15 @@ std::string copy_to_host(void* dst, const void* src_device, size_t bytes) {
   return "";
 }
 
+std::string device_arch() {
+  int dev = 0;
+  hipDeviceProp_t prop;
+  if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return "unknown";
+  std::string a(prop.gcnArchName);
+  const size_t c = a.find(':');
+  return c == std::string::npos ? a : a.substr(0, c);
+}
+
 double last_ms_between(Timer* t) {
   float f = 0.f;
   if (!t || hipEventSynchronize(t->b) != hipSuccess || hipEventElapsedTime(&f, t->a, t->b) != hipSuccess) return -1.0;

@@ -52,6 +52,10 @@ struct Device {
   uint16_t* bo_boff = nullptr;
   uint32_t *bo_inc_ptr = nullptr, *bo_inc = nullptr, *bo_ranges = nullptr;
   size_t bo_n_ranges = 0, bo_n_truss_ranges = 0;  // bar-kernel ranges, then truss-kernel ranges
+  uint32_t *bo_slot_inc = nullptr, *bo_slot_ptr = nullptr;
+  double* bo_slot = nullptr;   // staged entry constants of the bar-kernel ranges
+  size_t bo_n_slots = 0;
+  bool bo_slot_dirty = false;  // coordinates or bar parameters changed since bo_slot was filled
   // adjoint maps
   uint32_t* adj_bid[3] = {nullptr, nullptr, nullptr};
   uint64_t* adj_bmask = nullptr;

@@ -58,6 +58,7 @@ void destroy_device(Device* d) {
   for (int t = 0; t < 3; ++t) f(d->adj_bid[t]);
   for (int t = 0; t < 2; ++t) f(d->bo_bid[t]);
   f(d->bo_bmask); f(d->bo_boff); f(d->bo_inc_ptr); f(d->bo_inc); f(d->bo_ranges);
+  f(d->bo_slot_inc); f(d->bo_slot_ptr); f(d->bo_slot);
   f(d->adj_bmask); f(d->adj_boff); f(d->adj_inc_ptr); f(d->adj_xpos); f(d->adj_xs); f(d->adj_kpack);
   f(d->sep_cons); f(d->sep_pack); f(d->sep_cls); f(d->sep_tmp); f(d->sep_disp); f(d->sep_force); f(d->sep_ra); f(d->sep_ub);
   f(d->sep_v); f(d->sep_flag); f(d->sep_pos); f(d->sep_key); f(d->sep_cnt); f(d->sep_off); f(d->sep_aa); f(d->sep_bb);
@@ -181,6 +182,7 @@ std::string replace_params(Device* d, int type, const double* src, double* host_
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemcpyAsync(d->params[type], src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   if (type == PLATE) d->pl_geo_dirty = true;
+  else d->bo_slot_dirty = true;
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(hipMemcpy(host_copy, d->params[type], n * sizeof(double), hipMemcpyDeviceToHost));
   return "";
@@ -192,6 +194,7 @@ std::string replace_nodes(Device* d, const double* src, double* host_copy, void*
   hipStream_t s = static_cast<hipStream_t>(stream);
   HIP_TRY(hipMemcpyAsync(d->xyz, src, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   d->pl_geo_dirty = true;
+  d->bo_slot_dirty = true;
   HIP_TRY(hipStreamSynchronize(s));
   HIP_TRY(hipMemcpy(host_copy, d->xyz, n * sizeof(double), hipMemcpyDeviceToHost));
   return "";

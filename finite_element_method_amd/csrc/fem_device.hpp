@@ -164,6 +164,9 @@ struct BarMaps {
   // nodes touch trusses only, to the truss kernel (<= 85 nodes, <= 512 entries)
   std::vector<uint32_t> ranges;
   size_t n_bar_ranges = 0;
+  // bar-kernel entries in range order: slot s holds the staged constants of
+  // entry slot_inc[s] (an index into inc); range r owns slots slot_ptr[r] ..
+  std::vector<uint32_t> slot_inc, slot_ptr;
 };
 constexpr int BAR_WG_NODES = 42, BAR_WG_ENTRIES = 256, BAR_WG_VALUES = 4096;
 constexpr int TRUSS_WG_NODES = 85, TRUSS_WG_ENTRIES = 512;
@@ -201,6 +204,8 @@ std::string timer_stop(Timer* t, void* stream, double& ms);  // synchronizes
 std::string timer_end(Timer* t, void* stream);  // records only
 double last_ms_between(Timer* t);                // synchronizes on the end event
 std::string copy_to_host(void* dst, const void* src_device, size_t bytes);
+// architecture name of the current device ("gfx950" without feature suffixes)
+std::string device_arch();
 int64_t* row_ptr_mut(Device* d);
 
 // Adjoint maps (adjoint.hip)

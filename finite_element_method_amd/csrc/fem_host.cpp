@@ -4,6 +4,11 @@
 // include/fem_amd.h.  All element arithmetic runs on the device
 // (fem_device.hip, plate_fused.hip); the host only validates and indexes.
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <sstream>
+#include <sys/stat.h>
 #include <array>
 #include <map>
 #include <charconv>
@@ -150,6 +155,9 @@ struct fem_batch {
   femdev::Timer* timer = nullptr;
   bool ready = false, timed = false;
   std::string config = "untuned";
+  // 0: every group on its own graph branch (concurrent); 1: the groups one
+  // after another on one stream
+  int schedule = 0;
 };
 
 namespace {
@@ -1138,6 +1146,12 @@ std::string build_pattern(fem_model* m) {
         a0 += na;
       }
       bm.n_bar_ranges = bm.ranges.size() / 2;
+      bm.slot_ptr.assign(1, 0);
+      for (size_t r = 0; r < bm.n_bar_ranges; ++r) {
+        const uint32_t r0 = bm.ranges[2 * r], rn = bm.ranges[2 * r + 1];
+        for (uint32_t k = bm.inc_ptr[r0]; k < bm.inc_ptr[r0 + rn]; ++k) bm.slot_inc.push_back(k);
+        bm.slot_ptr.push_back(uint32_t(bm.slot_inc.size()));
+      }
       bm.ranges.insert(bm.ranges.end(), truss_ranges.begin(), truss_ranges.end());
     }
     if (ok && blocks.size() < (size_t(1) << 32)) {
@@ -1448,9 +1462,18 @@ std::string batch_ensure_branches(fem_batch* b) {
   return "";
 }
 
-// the groups' launches as parallel branches of one captured graph
+// the groups' launches as parallel branches of one captured graph, or one
+// after another (schedule 1)
 std::string batch_launch_branches(fem_batch* b) {
   const size_t ng = b->groups.size();
+  if (b->schedule == 1) {
+    for (size_t k = 0; k < ng; ++k) {
+      auto* g = b->groups[k];
+      auto e = launch_values(g, femdev::values(g->dev), b->stream, false);
+      if (!e.empty()) return e;
+    }
+    return "";
+  }
   for (size_t k = 1; k < ng; ++k) {
     auto e = femdev::stream_wait(b->side[k - 1], b->stream, b->fork_ev[k - 1]);
     if (!e.empty()) return e;
@@ -1498,6 +1521,73 @@ int time_group(fem_batch* b, fem_model* g, int reps, double& ms) {
   std::sort(t.begin(), t.end());
   ms = t[t.size() / 2];
   return FEM_OK;
+}
+
+// median of `reps` timed replays of the batch's graph (one warm-up)
+int time_graph(fem_batch* b, int reps, double& ms) {
+  std::vector<double> t;
+  for (int r = -1; r < reps; ++r) {
+    auto e = femdev::timer_start(b->timer, b->stream);
+    if (e.empty()) e = femdev::graph_launch(b->graph, b->stream);
+    double x = 0;
+    if (e.empty()) e = femdev::timer_stop(b->timer, b->stream, x);
+    if (!e.empty()) return batch_fail(b, FEM_ERR_DEVICE, e);
+    if (r >= 0) t.push_back(x);
+  }
+  std::sort(t.begin(), t.end());
+  ms = t[t.size() / 2];
+  return FEM_OK;
+}
+
+// Persistent autotune cache: one line per (gfx arch, batch signature),
+// "arch signature schedule pn,wg,bs;pn,wg,bs;...", in the file named by
+// FEM_AMD_TUNE_CACHE or else $XDG_CACHE_HOME/fem_amd/tune.txt
+// ($HOME/.cache/fem_amd/tune.txt).  Unwritable locations are skipped.
+std::string tune_cache_path() {
+  if (const char* p = std::getenv("FEM_AMD_TUNE_CACHE")) return p;
+  std::string base;
+  if (const char* x = std::getenv("XDG_CACHE_HOME")) base = x;
+  else if (const char* h = std::getenv("HOME")) base = std::string(h) + "/.cache";
+  else return "";
+  return base + "/fem_amd/tune.txt";
+}
+
+bool tune_cache_find(const std::string& key, int& schedule, std::vector<std::array<int, 3>>& cfg) {
+  const std::string path = tune_cache_path();
+  if (path.empty()) return false;
+  std::ifstream in(path);
+  std::string line;
+  bool found = false;
+  while (std::getline(in, line)) {  // the last entry of a key wins
+    std::istringstream ls(line);
+    std::string arch, sig, cfgs;
+    int sched = 0;
+    if (!(ls >> arch >> sig >> sched >> cfgs) || arch + " " + sig != key) continue;
+    std::vector<std::array<int, 3>> c;
+    std::istringstream cs(cfgs);
+    std::string g;
+    bool ok = true;
+    while (std::getline(cs, g, ';')) {
+      std::array<int, 3> v{};
+      if (std::sscanf(g.c_str(), "%d,%d,%d", &v[0], &v[1], &v[2]) != 3) { ok = false; break; }
+      c.push_back(v);
+    }
+    if (ok) { schedule = sched; cfg = c; found = true; }
+  }
+  return found;
+}
+
+void tune_cache_store(const std::string& key, int schedule, const std::vector<std::array<int, 3>>& cfg) {
+  const std::string path = tune_cache_path();
+  if (path.empty()) return;
+  for (size_t i = 1; i < path.size(); ++i)  // parent directories
+    if (path[i] == '/') mkdir(path.substr(0, i).c_str(), 0755);
+  std::ofstream out(path, std::ios::app);
+  if (!out) return;
+  out << key << ' ' << schedule << ' ';
+  for (size_t k = 0; k < cfg.size(); ++k)
+    out << (k ? ";" : "") << cfg[k][0] << ',' << cfg[k][1] << ',' << cfg[k][2];
+  out << '\n';
 }
 
 }  // namespace
@@ -2288,12 +2378,14 @@ int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out) {
   // Groups, so that every mesh takes the same value path as when assembled
   // alone: flat plate-only meshes (every frame the identity: fused plate
   // kernel), split so that each group's values stay below the fused kernel's
-  // 32-bit byte offsets (at most 118 values per node at valence <= 8); truss
-  // and beam meshes (node-owner kernel); all other meshes (generic path).
+  // 32-bit byte offsets (at most 118 values per node at valence <= 8); meshes
+  // of bars with beams (node-owner kernel); truss-only meshes (the same
+  // kernels' truss ranges, in a group of their own so that its launch runs
+  // beside the beam group's); all other meshes (generic path).
   const uint64_t plate_cap = (uint64_t(1) << 32) / 8 / 118;  // nodes per plate group
   std::vector<int> grp(n, -1);
   std::vector<uint64_t> gnodes;
-  int cur_plate = -1, bars = -1, generic = -1;
+  int cur_plate = -1, bars = -1, trusses = -1, generic = -1;
   auto open_group = [&](int& g) {
     if (g < 0) {
       g = int(gnodes.size());
@@ -2308,6 +2400,8 @@ int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out) {
     if (plates_only && nn <= plate_cap && plates_flat_identity(m)) {
       if (cur_plate >= 0 && gnodes[size_t(cur_plate)] + nn > plate_cap) cur_plate = -1;
       grp[i] = open_group(cur_plate);
+    } else if (!m->set[PLATE].size() && !m->set[BEAM].size()) {
+      grp[i] = open_group(trusses);
     } else if (!m->set[PLATE].size()) {
       grp[i] = open_group(bars);
     } else {
@@ -2369,18 +2463,26 @@ int fem_batch_size(const fem_batch* b, size_t* n_meshes, size_t* n_groups) {
 
 int fem_batch_autotune(fem_batch* b) {
   if (!b) return FEM_ERR_ARGUMENT;
-  // tuned shapes are cached per batch signature for the life of the process
-  static std::map<std::string, std::vector<std::array<int, 3>>> cache;
+  // tuned shapes are cached per (gfx arch, batch signature): for the life of
+  // the process and in the tune cache file
+  static std::map<std::string, std::pair<int, std::vector<std::array<int, 3>>>> cache;
   std::string sig;
   for (auto* g : b->groups)
     sig += std::to_string(g->set[0].size()) + "," + std::to_string(g->set[1].size()) + "," +
            std::to_string(g->set[2].size()) + "," + std::to_string(g->node_number.size()) + ";";
   sig += std::to_string(b->meshes.size());
+  const std::string key = femdev::device_arch() + " " + sig;
   std::vector<std::array<int, 3>> best;  // per group: patch nodes, workgroups per CU, generic block
-  auto it = cache.find(sig);
+  int best_schedule = -1;                // -1: to be timed
+  auto it = cache.find(key);
   if (it != cache.end()) {
-    best = it->second;
+    best_schedule = it->second.first;
+    best = it->second.second;
+  } else if (tune_cache_find(key, best_schedule, best) && best.size() == b->groups.size()) {
+    cache[key] = {best_schedule, best};
   } else {
+    best.clear();
+    best_schedule = -1;
     for (auto* g : b->groups) {
       std::array<int, 3> cfg = {g->pf_nodes, g->pf_wg_per_cu, 256};
       double best_ms = 1e300;
@@ -2428,7 +2530,6 @@ int fem_batch_autotune(fem_batch* b) {
       }
       best.push_back(cfg);
     }
-    cache[sig] = best;
   }
   // apply
   b->config.clear();
@@ -2460,6 +2561,23 @@ int fem_batch_autotune(fem_batch* b) {
                    (g->path == "generic" ? " block=" + std::to_string(best[k][2]) : std::string()) + "; ";
     }
   }
+  if (best_schedule < 0) {
+    // the schedule: concurrent branches or one stream, timed on the whole graph
+    double best_ms = 1e300;
+    for (int sched : {0, 1}) {
+      b->schedule = sched;
+      rc = batch_capture(b);
+      if (rc) return rc;
+      double ms = 0;
+      rc = time_graph(b, 5, ms);
+      if (rc) return rc;
+      if (ms < best_ms) { best_ms = ms; best_schedule = sched; }
+    }
+    cache[key] = {best_schedule, best};
+    tune_cache_store(key, best_schedule, best);
+  }
+  b->schedule = best_schedule;
+  b->config += best_schedule == 0 ? "groups concurrent" : "groups serial";
   return batch_capture(b);
 }
 

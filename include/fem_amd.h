@@ -297,10 +297,15 @@ int fem_batch_create(fem_model* const* models, size_t n, fem_batch** out);
 void fem_batch_destroy(fem_batch* b);
 const char* fem_batch_last_error(const fem_batch* b);
 int fem_batch_size(const fem_batch* b, size_t* n_meshes, size_t* n_groups);
-/* Times candidate launch shapes per group (fused plate kernel: owned nodes per
- * patch and resident workgroups per CU; generic path: block size), keeps the
- * fastest and captures the graph.  Results are cached per batch signature for
- * the life of the process.  Synchronous. */
+/* Times candidate launch shapes per group (row-staging plate kernel: owned
+ * nodes per patch and resident workgroups per CU; block-lane plate kernels:
+ * workgroups per CU; generic path: block size) and the schedule of the groups
+ * (concurrent graph branches, or one after another on one stream) on whole
+ * graph replays, keeps the fastest and captures the graph.  Results are cached
+ * per (gfx architecture, batch signature) for the life of the process and in a
+ * cache file: FEM_AMD_TUNE_CACHE, else $XDG_CACHE_HOME/fem_amd/tune.txt, else
+ * $HOME/.cache/fem_amd/tune.txt (one line per entry; skipped when not
+ * writable).  A cached entry is applied without timing.  Synchronous. */
 int fem_batch_autotune(fem_batch* b);
 /* One hipGraphLaunch of every group's value kernels on `stream`. */
 int fem_batch_assemble(fem_batch* b, void* stream);

@@ -31,7 +31,7 @@ def test_batch_matches_single_assembly_and_oracle(gpu):
     meshes = _meshes()
     models = [FEM.from_mesh(m) for m in meshes]
     batch = FEMBatch(models)
-    assert batch.groups() == 3  # flat plates | trusses and beams | the rest
+    assert batch.groups() == 4  # flat plates | bars with beams | truss-only meshes | the rest
     batch.assemble()
     for i, (mesh, model) in enumerate(zip(meshes, models)):
         print("mesh", i, mesh.kind, mesh.n_elements, flush=True)
@@ -62,3 +62,44 @@ def test_batch_autotune_and_replay(gpu):
     # a second batch with the same signature reuses the tuned shapes
     batch2 = FEMBatch(models)
     assert batch2.autotune() == cfg
+
+
+_CACHE_SCRIPT = r"""
+import sys
+sys.path.insert(0, %r)
+from finite_element_method_amd import FEM, FEMBatch, meshgen
+meshes = [meshgen.plate_grid(5, seed=31), meshgen.truss_lattice(3, seed=32), meshgen.beam_frame(3, seed=33)]
+batch = FEMBatch([FEM.from_mesh(m) for m in meshes])
+print("CONFIG", batch.autotune(), flush=True)
+"""
+
+
+def test_tune_cache_persists(gpu, tmp_path):
+    """The tuned shapes and schedule are stored per (gfx arch, batch signature)
+    in the tune cache file, and a later process applies a stored entry
+    without timing: flipping the stored schedule flips the next process's."""
+    import os
+    import subprocess
+    import sys
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    cache = tmp_path / "tune.txt"
+    env = dict(os.environ, FEM_AMD_TUNE_CACHE=str(cache))
+    script = _CACHE_SCRIPT % root
+
+    def run():
+        out = subprocess.run([sys.executable, "-c", script], env=env, capture_output=True, text=True, timeout=300)
+        assert out.returncode == 0, out.stderr[-2000:]
+        return [ln for ln in out.stdout.splitlines() if ln.startswith("CONFIG")][0]
+
+    first = run()
+    lines = cache.read_text().splitlines()
+    assert len(lines) == 1
+    arch, sig, sched, cfgs = lines[0].split()
+    assert arch.startswith("gfx") and sched in ("0", "1") and len(cfgs.split(";")) == 3
+    assert ("groups serial" if sched == "1" else "groups concurrent") in first
+    flipped = "0" if sched == "1" else "1"
+    cache.write_text(" ".join([arch, sig, flipped, cfgs]) + "\n")
+    second = run()
+    assert ("groups serial" if flipped == "1" else "groups concurrent") in second
+    assert cache.read_text().splitlines() == [" ".join([arch, sig, flipped, cfgs])]  # nothing re-tuned
