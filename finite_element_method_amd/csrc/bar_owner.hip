@@ -5,19 +5,20 @@
 // record, rows 0..2 only); the others to bar_owner_kernel:
 //
 // One workgroup per range of owned nodes (<= 42 nodes, <= 256 incident
-// (node, element) entries, <= 4096 CSR values).  Phase 1: one lane per entry
-// stages the element's constants and its blocks' masks and row offsets in LDS,
-// all global loads in parallel.  Phase 2: one lane per (owned node a, CSR row
-// i) walks a's entries in element-insertion order and computes, for each, row
-// i of the element's global blocks (p, p) and (p, 1 - p) (p = a's position in
-// the element).  The diagonal block (a, a) is summed in registers over all
-// incident elements (insertion order, starting from the first contribution);
-// an off-diagonal block (a, b) has exactly one contributing element (checked
-// when the maps are built).  The rows are formed in LDS and the workgroup's
-// owned rows, one contiguous CSR range, are then written out by consecutive
-// lanes (a scattered row-by-row write costs one memory request per value).
-// Every CSR value is written once: no contribution buffer, no atomics,
-// bitwise reproducible.
+// (node, element) entries, <= 6144 CSR values).  Each entry's constants (the
+// element frame and term scales, its blocks' masks and row offsets) sit in a
+// 160-byte slot formed once per model change (bar_slot_kernel), in the order
+// the workgroup reads them.  One lane per (owned node a, CSR row i) walks a's
+// entries in element-insertion order, loading the next entry's slot while it
+// computes row i of the current entry's global blocks (p, p) and (p, 1 - p)
+// (p = a's position in the element).  The diagonal block (a, a) is summed in
+// registers over all incident elements (insertion order, starting from the
+// first contribution); an off-diagonal block (a, b) has exactly one
+// contributing element (checked when the maps are built).  The rows are formed
+// in LDS and the workgroup's owned rows, one contiguous CSR range, are then
+// written out by consecutive lanes as 16-byte stores (a scattered row-by-row
+// write costs one memory request per value).  Every CSR value is written once:
+// no contribution buffer, no atomics, bitwise reproducible.
 //
 // Beam blocks use the rank-one form of beam.rs:495-605: with T = diag(R, R),
 //   row i of T^T (sum_t s_t v_t,p v_t,q^T) T = sum_t s_t (T^T v_t,p)_i (T^T v_t,q)^T.
@@ -36,8 +37,8 @@ namespace {
 constexpr int BO_THREADS = 256;
 constexpr int BO_NODES = 42;     // owned nodes per workgroup (6 row lanes each)
 constexpr int BO_ENTRIES = 256;  // (owned node, incident element) entries per workgroup
-constexpr int BO_DATA = 18;      // doubles per staged entry
-constexpr int BO_VALUES = 4096;  // CSR values of the owned rows per workgroup
+constexpr int BO_DATA = 16;      // element constants per entry slot
+constexpr int BO_VALUES = 6144;  // CSR values of the owned rows per workgroup
 constexpr int TO_NODES = 85;     // truss kernel: owned nodes per workgroup (3 row lanes each)
 constexpr int TO_ENTRIES = 512;  // truss kernel: entries per workgroup
 
@@ -60,15 +61,13 @@ struct BarArgs {
   double* __restrict__ val;
 };
 
+// The entries' constants are read from their slots (global memory); LDS
+// holds the owned rows only, so that three workgroups fit per CU.
 struct BarLds {
-  double data[BO_ENTRIES * BO_DATA];   // per entry: R (9), term scales (6), bp, bq, pad / truss: r0, kpp, kpq
-  uint64_t mask[BO_ENTRIES][2];        // masks of the entry's blocks (p, p) and (p, 1 - p)
-  uint16_t off[BO_ENTRIES][2][6];      // per-row offsets of those blocks
-  uint32_t type[BO_ENTRIES];
   uint32_t first[BO_NODES + 1];        // entry range of each owned node
   double out[BO_VALUES];               // the owned rows
 };
-static_assert(2 * sizeof(BarLds) <= 160 * 1024, "two workgroups per CU");
+static_assert(3 * sizeof(BarLds) <= 160 * 1024, "three workgroups per CU");
 static_assert(BO_NODES == BAR_WG_NODES && BO_ENTRIES == BAR_WG_ENTRIES && BO_VALUES == BAR_WG_VALUES, "range cut");
 
 // Truss-only ranges: rows 3..5 are empty and an entry needs five constants,
@@ -130,21 +129,24 @@ __device__ inline void truss_constants(const BarArgs& A, size_t e, int p, double
   o[4] = (bp * bq) * scale;
 }
 
-// One (owned node, incident element) entry's staged constants, as a slot of
-// BO_SLOT doubles: the element constants (truss.rs:132-189, beam.rs:495-605;
-// BO_DATA doubles), the masks of its blocks (p, p) and (p, 1 - p), their
-// per-row offsets (12 x u16) and the element type.  The slots are formed once
-// per model change by bar_slot_kernel, so that the assembly reads each entry
-// with one load instead of a chain of dependent ones.
-constexpr int BO_SLOT = 24;
-static_assert(BO_DATA + 2 + 3 + 1 <= BO_SLOT, "slot layout");
+// One (owned node, incident element) entry's constants, as a slot of BO_SLOT
+// doubles: the element constants (BO_DATA doubles; beam: R (9), the six term
+// scales, b_p = -b_q; truss: r (3), k_pp, k_pq; truss.rs:132-189,
+// beam.rs:495-605), then the masks of its blocks (p, p) and (p, 1 - p) (36
+// bits each) with the offsets of rows 0 and 1 in the bits above them (12 bits
+// each: a row holds at most 6 x 257 values) and the element type in bit 63,
+// then the offsets of rows 2..5 of both blocks (16 bits each).  The slots are
+// formed once per model change by bar_slot_kernel, so that the assembly reads
+// each entry with ten 16-byte loads instead of a chain of dependent ones.
+constexpr int BO_SLOT = 20;
+static_assert(BO_DATA + 4 == BO_SLOT, "slot layout");
 
 __device__ inline void form_slot(const BarArgs& A, uint32_t u, double* __restrict__ slot) {
   const int type = int(u >> 31), p = int(u & 1u);
   const size_t e = (u >> 1) & 0x3FFFFFFFu;
-  double o[BO_DATA];
+  double o[BO_DATA + 1];
 #pragma unroll
-  for (int k = 0; k < BO_DATA; ++k) o[k] = 0.0;
+  for (int k = 0; k <= BO_DATA; ++k) o[k] = 0.0;
   if (type == 0) {
     truss_constants(A, e, p, o);
   } else {
@@ -165,7 +167,7 @@ __device__ inline void form_slot(const BarArgs& A, uint32_t u, double* __restric
     o[13] = E * i22p * det_j * alpha;
     o[14] = E * i11p * det_j * alpha;
     o[15] = bp;
-    o[16] = bq;
+    (void)bq;  // -bp exactly
   }
   const uint32_t ks = A.bid[type][4 * e + 3 * p], kn = A.bid[type][4 * e + 2 * p + (1 - p)];
   uint16_t off[12];
@@ -177,17 +179,15 @@ __device__ inline void form_slot(const BarArgs& A, uint32_t u, double* __restric
   double w[BO_SLOT];
 #pragma unroll
   for (int k = 0; k < BO_DATA; ++k) w[k] = o[k];
-  w[BO_DATA] = __builtin_bit_cast(double, A.bmask[ks]);
-  w[BO_DATA + 1] = __builtin_bit_cast(double, A.bmask[kn]);
+  // masks (36 bits) with the offsets of rows 0, 1 (12 bits each) above them and
+  // the type in bit 63; the offsets of rows 2..5 as 16-bit fields
+  const uint64_t m0 = A.bmask[ks], m1 = A.bmask[kn];
+  w[BO_DATA] = __builtin_bit_cast(double, m0 | uint64_t(off[0]) << 36 | uint64_t(off[1]) << 48 | uint64_t(type) << 63);
+  w[BO_DATA + 1] = __builtin_bit_cast(double, m1 | uint64_t(off[6]) << 36 | uint64_t(off[7]) << 48);
 #pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const uint64_t v = uint64_t(off[4 * m]) | uint64_t(off[4 * m + 1]) << 16 | uint64_t(off[4 * m + 2]) << 32 |
-                       uint64_t(off[4 * m + 3]) << 48;
-    w[BO_DATA + 2 + m] = __builtin_bit_cast(double, v);
-  }
-  w[BO_DATA + 5] = __builtin_bit_cast(double, uint64_t(type));
-#pragma unroll
-  for (int k = BO_DATA + 6; k < BO_SLOT; ++k) w[k] = 0.0;
+  for (int b = 0; b < 2; ++b)
+    w[BO_DATA + 2 + b] = __builtin_bit_cast(double, uint64_t(off[6 * b + 2]) | uint64_t(off[6 * b + 3]) << 16 |
+                                                        uint64_t(off[6 * b + 4]) << 32 | uint64_t(off[6 * b + 5]) << 48);
   double2* dst = reinterpret_cast<double2*>(slot);
 #pragma unroll
   for (int k = 0; k < BO_SLOT / 2; ++k) dst[k] = make_double2(w[2 * k], w[2 * k + 1]);
@@ -201,41 +201,38 @@ __global__ __launch_bounds__(256) void bar_slot_kernel(BarArgs A, const uint32_t
   form_slot(A, A.inc[slot_inc[s]], slot + BO_SLOT * s);
 }
 
-// a slot into the workgroup's staged entry `slot_idx`
-__device__ inline void load_slot(const double* __restrict__ src, BarLds& L, int slot_idx) {
-  const double2* s2 = reinterpret_cast<const double2*>(src);
-  double w[BO_SLOT];
+// slot k of a range into registers (ten 16-byte loads)
+__device__ inline void load_slot(const double* __restrict__ slots, int k, double o[BO_SLOT]) {
+  const double2* s2 = reinterpret_cast<const double2*>(slots + BO_SLOT * size_t(k));
 #pragma unroll
-  for (int k = 0; k < BO_SLOT / 2; ++k) {
-    const double2 v = s2[k];
-    w[2 * k] = v.x;
-    w[2 * k + 1] = v.y;
+  for (int m = 0; m < BO_SLOT / 2; ++m) {
+    const double2 v = s2[m];
+    o[2 * m] = v.x;
+    o[2 * m + 1] = v.y;
   }
-  double* o = L.data + slot_idx * BO_DATA;
-#pragma unroll
-  for (int k = 0; k < BO_DATA; ++k) o[k] = w[k];
-  L.mask[slot_idx][0] = __builtin_bit_cast(uint64_t, w[BO_DATA]);
-  L.mask[slot_idx][1] = __builtin_bit_cast(uint64_t, w[BO_DATA + 1]);
-#pragma unroll
-  for (int m = 0; m < 3; ++m) {
-    const uint64_t v = __builtin_bit_cast(uint64_t, w[BO_DATA + 2 + m]);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int idx = 4 * m + j;
-      L.off[slot_idx][idx / 6][idx % 6] = uint16_t(v >> (16 * j));
-    }
-  }
-  L.type[slot_idx] = uint32_t(__builtin_bit_cast(uint64_t, w[BO_DATA + 5]));
+}
+
+// slot fields: element type, mask of block b (0: (p, p), 1: (p, 1 - p)) and
+// the offset of its row i within the owned node's row
+__device__ inline uint32_t slot_type(const double* o) {
+  return uint32_t(__builtin_bit_cast(uint64_t, o[BO_DATA]) >> 63);
+}
+__device__ inline uint64_t slot_mask(const double* o, int b) {
+  return __builtin_bit_cast(uint64_t, b ? o[BO_DATA + 1] : o[BO_DATA]) & ((uint64_t(1) << 36) - 1);
+}
+__device__ inline int slot_off(const double* o, int b, int i) {
+  const uint64_t lo = __builtin_bit_cast(uint64_t, b ? o[BO_DATA + 1] : o[BO_DATA]);
+  const uint64_t hi = __builtin_bit_cast(uint64_t, b ? o[BO_DATA + 3] : o[BO_DATA + 2]);
+  return i < 2 ? int((lo >> (36 + 12 * i)) & 0xFFFu) : int((hi >> (16 * (i - 2))) & 0xFFFFu);
 }
 
 // row i of an entry's blocks (p, p) -> self and (p, 1 - p) -> nb
-__device__ inline void entry_rows(const BarLds& L, int slot, int i, double self[6], double nb[6]) {
-  const double* o = L.data + slot * BO_DATA;
+__device__ inline void entry_rows(const double* __restrict__ o, uint32_t type, int i, double self[6], double nb[6]) {
 #pragma unroll
   for (int j = 0; j < 6; ++j) { self[j] = 0.0; nb[j] = 0.0; }
-  if (L.type[slot] == 0) {
+  if (type == 0) {
     if (i >= 3) return;
-    const double ri = o[i];
+    const double ri = i == 0 ? o[0] : (i == 1 ? o[1] : o[2]);
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
       self[j] = (ri * o[3]) * o[j];
@@ -246,7 +243,7 @@ __device__ inline void entry_rows(const BarLds& L, int slot, int i, double self[
   double R[9];
 #pragma unroll
   for (int m = 0; m < 9; ++m) R[m] = o[m];
-  const double bp = o[15], bq = o[16];
+  const double bp = o[15], bq = -bp;
 #pragma unroll
   for (int t = 0; t < 6; ++t) {
     double vp[6], vq[6];
@@ -263,48 +260,66 @@ __device__ inline void entry_rows(const BarLds& L, int slot, int i, double self[
 }
 
 // row i of one block into the staged rows (r0: the row's start within them)
-__device__ inline void write_row(BarLds& L, uint64_t mask, const uint16_t* off, int i, int r0, const double v[6]) {
+__device__ inline void write_row(BarLds& L, uint64_t mask, int off_i, int i, int r0, const double v[6]) {
   const uint32_t mi = uint32_t(mask >> (6 * i)) & 63u;
-  const int d0 = r0 + int(off[i]);
+  const int d0 = r0 + off_i;
 #pragma unroll
   for (int j = 0; j < 6; ++j)
     if ((mi >> j) & 1u) L.out[d0 + __builtin_popcount(mi & ((1u << j) - 1u))] = v[j];
 }
 
-// One workgroup per range of owned nodes: stage the incident entries (one
-// lane each, all loads in parallel), form the rows with one lane per (owned
-// node, row), then write the range out with consecutive lanes.
+// One workgroup per range of owned nodes: one lane per (owned node, row) forms
+// the rows in LDS, each entry's slot loaded during the previous entry's
+// arithmetic; then consecutive lanes write the range out.
 __global__ __launch_bounds__(BO_THREADS) void bar_owner_kernel(BarArgs A) {
   __shared__ BarLds L;
   const int t = threadIdx.x;
   const uint2 rg = A.ranges[blockIdx.x];
   const uint32_t a0 = rg.x, na = rg.y;
-  const uint32_t e0 = A.inc_ptr[a0], ne = A.inc_ptr[a0 + na] - e0;
+  const uint32_t e0 = A.inc_ptr[a0];
   const int64_t base = A.row_ptr[6 * size_t(a0)];
   const int nv = int(A.row_ptr[6 * size_t(a0 + na)] - base);  // <= BO_VALUES (cut on the host)
   if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
-  if (t < int(ne)) load_slot(A.slot + BO_SLOT * size_t(A.slot_ptr[blockIdx.x] + t), L, t);
   __syncthreads();
+  const double* __restrict__ slots = A.slot + BO_SLOT * size_t(A.slot_ptr[blockIdx.x]);
   const int j = t / 6, i = t % 6;
   if (j < int(na)) {
     const size_t a = size_t(a0) + j;
     const int r0 = int(A.row_ptr[6 * a + i] - base);
     const int s0 = int(L.first[j]), s1 = int(L.first[j + 1]);
     if (A.row_ptr[6 * a + i + 1] - base > r0) {
-      double acc[6];
+      double acc[6], cur[BO_SLOT];
+      uint64_t m0 = 0;  // block (a, a): mask and row offset, from the first entry
+      int o0 = 0;
+      if (s1 > s0) load_slot(slots, s0, cur);
       for (int sl = s0; sl < s1; ++sl) {
+        double nxt[BO_SLOT];
+        load_slot(slots, sl + 1 < s1 ? sl + 1 : sl, nxt);
         double self[6], nb[6];
-        entry_rows(L, sl, i, self, nb);
+        entry_rows(cur, slot_type(cur), i, self, nb);
 #pragma unroll
         for (int q = 0; q < 6; ++q) acc[q] = sl == s0 ? self[q] : acc[q] + self[q];
-        write_row(L, L.mask[sl][1], L.off[sl][1], i, r0, nb);
+        if (sl == s0) {
+          m0 = slot_mask(cur, 0);
+          o0 = slot_off(cur, 0, i);
+        }
+        write_row(L, slot_mask(cur, 1), slot_off(cur, 1, i), i, r0, nb);
+#pragma unroll
+        for (int k = 0; k < BO_SLOT; ++k) cur[k] = nxt[k];
       }
-      if (s1 > s0) write_row(L, L.mask[s0][0], L.off[s0][0], i, r0, acc);
+      if (s1 > s0) write_row(L, m0, o0, i, r0, acc);
     }
   }
   __syncthreads();
   double* __restrict__ dst = A.val + base;
-  for (int k = t; k < nv; k += BO_THREADS) dst[k] = L.out[k];
+  // 16-byte stores from the first even CSR position; the odd values at either end alone
+  const int head = int(base & 1), npair = (nv - head) >> 1;
+  for (int k = t; k < npair; k += BO_THREADS) {
+    const int o = head + 2 * k;
+    *reinterpret_cast<double2*>(dst + o) = make_double2(L.out[o], L.out[o + 1]);
+  }
+  if (t == 0 && head && nv > 0) dst[0] = L.out[0];
+  if (t == BO_THREADS - 1 && nv > head && ((nv - head) & 1)) dst[nv - 1] = L.out[nv - 1];
 }
 
 // row i < 3 of a truss block (columns 0..2) straight to the CSR
