@@ -4,14 +4,16 @@
 // nodes touch trusses only go to truss_owner_kernel (below: a compact entry
 // record, rows 0..2 only); the others to bar_owner_kernel:
 //
-// One workgroup per range of owned nodes (<= 42 nodes, <= 256 incident
-// (node, element) entries, <= 6144 CSR values).  Each entry's constants (the
+// One workgroup per range of owned nodes (<= 40 nodes, <= 256 incident
+// (node, element) entries, <= 5120 CSR values).  Each entry's constants (the
 // element frame and term scales, its blocks' masks and row offsets) sit in a
 // 160-byte slot formed once per model change (bar_slot_kernel), in the order
 // the workgroup reads them.  One lane per (owned node a, CSR row i) walks a's
-// entries in element-insertion order, loading the next entry's slot while it
-// computes row i of the current entry's global blocks (p, p) and (p, 1 - p)
-// (p = a's position in the element).  The diagonal block (a, a) is summed in
+// entries in element-insertion order and computes row i of each entry's global
+// blocks (p, p) and (p, 1 - p) (p = a's position in the element).  The six
+// lanes of a node sit in one wave and load each slot together (two 16-byte
+// loads per lane), one entry ahead of the arithmetic, into the node's LDS slot,
+// from which every lane reads it.  The diagonal block (a, a) is summed in
 // registers over all incident elements (insertion order, starting from the
 // first contribution); an off-diagonal block (a, b) has exactly one
 // contributing element (checked when the maps are built).  The rows are formed
@@ -35,10 +37,11 @@ namespace femdev {
 namespace {
 
 constexpr int BO_THREADS = 256;
-constexpr int BO_NODES = 42;     // owned nodes per workgroup (6 row lanes each)
+constexpr int BO_NODES = 40;     // owned nodes per workgroup (10 per wave, 6 row lanes each)
 constexpr int BO_ENTRIES = 256;  // (owned node, incident element) entries per workgroup
 constexpr int BO_DATA = 16;      // element constants per entry slot
-constexpr int BO_VALUES = 6144;  // CSR values of the owned rows per workgroup
+constexpr int BO_SLOT = 20;      // doubles per entry slot
+constexpr int BO_VALUES = 5120;  // CSR values of the owned rows per workgroup
 constexpr int TO_NODES = 85;     // truss kernel: owned nodes per workgroup (3 row lanes each)
 constexpr int TO_ENTRIES = 512;  // truss kernel: entries per workgroup
 
@@ -61,13 +64,13 @@ struct BarArgs {
   double* __restrict__ val;
 };
 
-// The entries' constants are read from their slots (global memory); LDS
-// holds the owned rows only, so that three workgroups fit per CU.
 struct BarLds {
-  uint32_t first[BO_NODES + 1];        // entry range of each owned node
-  double out[BO_VALUES];               // the owned rows
+  uint32_t first[BO_NODES + 1];         // entry range of each owned node
+  double2 slot[BO_NODES][BO_SLOT / 2];  // each owned node's current entry slot
+  double out[BO_VALUES];                // the owned rows
 };
 static_assert(3 * sizeof(BarLds) <= 160 * 1024, "three workgroups per CU");
+
 static_assert(BO_NODES == BAR_WG_NODES && BO_ENTRIES == BAR_WG_ENTRIES && BO_VALUES == BAR_WG_VALUES, "range cut");
 
 // Truss-only ranges: rows 3..5 are empty and an entry needs five constants,
@@ -138,8 +141,7 @@ __device__ inline void truss_constants(const BarArgs& A, size_t e, int p, double
 // then the offsets of rows 2..5 of both blocks (16 bits each).  The slots are
 // formed once per model change by bar_slot_kernel, so that the assembly reads
 // each entry with ten 16-byte loads instead of a chain of dependent ones.
-constexpr int BO_SLOT = 20;
-static_assert(BO_DATA + 4 == BO_SLOT, "slot layout");
+static_assert(BO_DATA + 4 == BO_SLOT && BO_SLOT == 20, "slot layout: ten 16-byte chunks");
 
 __device__ inline void form_slot(const BarArgs& A, uint32_t u, double* __restrict__ slot) {
   const int type = int(u >> 31), p = int(u & 1u);
@@ -201,17 +203,6 @@ __global__ __launch_bounds__(256) void bar_slot_kernel(BarArgs A, const uint32_t
   form_slot(A, A.inc[slot_inc[s]], slot + BO_SLOT * s);
 }
 
-// slot k of a range into registers (ten 16-byte loads)
-__device__ inline void load_slot(const double* __restrict__ slots, int k, double o[BO_SLOT]) {
-  const double2* s2 = reinterpret_cast<const double2*>(slots + BO_SLOT * size_t(k));
-#pragma unroll
-  for (int m = 0; m < BO_SLOT / 2; ++m) {
-    const double2 v = s2[m];
-    o[2 * m] = v.x;
-    o[2 * m + 1] = v.y;
-  }
-}
-
 // slot fields: element type, mask of block b (0: (p, p), 1: (p, 1 - p)) and
 // the offset of its row i within the owned node's row
 __device__ inline uint32_t slot_type(const double* o) {
@@ -259,6 +250,12 @@ __device__ inline void entry_rows(const double* __restrict__ o, uint32_t type, i
   }
 }
 
+__device__ inline void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // row i of one block into the staged rows (r0: the row's start within them)
 __device__ inline void write_row(BarLds& L, uint64_t mask, int off_i, int i, int r0, const double v[6]) {
   const uint32_t mi = uint32_t(mask >> (6 * i)) & 63u;
@@ -281,34 +278,66 @@ __global__ __launch_bounds__(BO_THREADS) void bar_owner_kernel(BarArgs A) {
   const int nv = int(A.row_ptr[6 * size_t(a0 + na)] - base);  // <= BO_VALUES (cut on the host)
   if (t <= int(na)) L.first[t] = A.inc_ptr[a0 + t] - e0;
   __syncthreads();
-  const double* __restrict__ slots = A.slot + BO_SLOT * size_t(A.slot_ptr[blockIdx.x]);
-  const int j = t / 6, i = t % 6;
-  if (j < int(na)) {
+  const double2* __restrict__ slots =
+      reinterpret_cast<const double2*>(A.slot + BO_SLOT * size_t(A.slot_ptr[blockIdx.x]));
+  // lane (owned node j, row i), ten nodes per wave (lanes 60..63 idle here).
+  // The six lanes of a node load one entry slot together (lane i: 16-byte
+  // chunks i and i + 6) into the node's LDS slot, one entry ahead of the
+  // arithmetic, and each reads the whole slot from there.
+  const int jl = (t & 63) / 6, i = (t & 63) - 6 * jl;
+  const int j = 10 * (t >> 6) + jl;
+  if (jl < 10 && j < int(na)) {
     const size_t a = size_t(a0) + j;
     const int r0 = int(A.row_ptr[6 * a + i] - base);
+    const bool row = A.row_ptr[6 * a + i + 1] - base > r0;
     const int s0 = int(L.first[j]), s1 = int(L.first[j + 1]);
-    if (A.row_ptr[6 * a + i + 1] - base > r0) {
-      double acc[6], cur[BO_SLOT];
-      uint64_t m0 = 0;  // block (a, a): mask and row offset, from the first entry
-      int o0 = 0;
-      if (s1 > s0) load_slot(slots, s0, cur);
-      for (int sl = s0; sl < s1; ++sl) {
-        double nxt[BO_SLOT];
-        load_slot(slots, sl + 1 < s1 ? sl + 1 : sl, nxt);
+    double2* __restrict__ buf = L.slot[j];
+    double2 c0 = make_double2(0.0, 0.0), c1 = c0;
+    if (s1 > s0) {
+      c0 = slots[(BO_SLOT / 2) * s0 + i];
+      if (i < 4) c1 = slots[(BO_SLOT / 2) * s0 + 6 + i];
+      buf[i] = c0;
+      if (i < 4) buf[6 + i] = c1;
+      if (s0 + 1 < s1) {
+        c0 = slots[(BO_SLOT / 2) * (s0 + 1) + i];
+        if (i < 4) c1 = slots[(BO_SLOT / 2) * (s0 + 1) + 6 + i];
+      }
+    }
+    wave_lds_sync();
+    double acc[6];
+    uint64_t m0 = 0;  // block (a, a): mask and row offset, from the first entry
+    int o0 = 0;
+    for (int sl = s0; sl < s1; ++sl) {
+      double o[BO_SLOT];
+#pragma unroll
+      for (int m = 0; m < BO_SLOT / 2; ++m) {
+        const double2 v = buf[m];
+        o[2 * m] = v.x;
+        o[2 * m + 1] = v.y;
+      }
+      if (row) {
         double self[6], nb[6];
-        entry_rows(cur, slot_type(cur), i, self, nb);
+        entry_rows(o, slot_type(o), i, self, nb);
 #pragma unroll
         for (int q = 0; q < 6; ++q) acc[q] = sl == s0 ? self[q] : acc[q] + self[q];
         if (sl == s0) {
-          m0 = slot_mask(cur, 0);
-          o0 = slot_off(cur, 0, i);
+          m0 = slot_mask(o, 0);
+          o0 = slot_off(o, 0, i);
         }
-        write_row(L, slot_mask(cur, 1), slot_off(cur, 1, i), i, r0, nb);
-#pragma unroll
-        for (int k = 0; k < BO_SLOT; ++k) cur[k] = nxt[k];
+        write_row(L, slot_mask(o, 1), slot_off(o, 1, i), i, r0, nb);
       }
-      if (s1 > s0) write_row(L, m0, o0, i, r0, acc);
+      if (sl + 1 < s1) {
+        wave_lds_sync();  // every lane of the node has read the slot
+        buf[i] = c0;
+        if (i < 4) buf[6 + i] = c1;
+        wave_lds_sync();
+        if (sl + 2 < s1) {
+          c0 = slots[(BO_SLOT / 2) * (sl + 2) + i];
+          if (i < 4) c1 = slots[(BO_SLOT / 2) * (sl + 2) + 6 + i];
+        }
+      }
     }
+    if (row && s1 > s0) write_row(L, m0, o0, i, r0, acc);
   }
   __syncthreads();
   double* __restrict__ dst = A.val + base;

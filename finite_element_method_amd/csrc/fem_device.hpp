@@ -160,7 +160,7 @@ struct BarMaps {
   std::vector<uint32_t> inc_ptr;  // N + 1
   std::vector<uint32_t> inc;      // per node, insertion order: type << 31 | element << 1 | p
   // per workgroup: first node, node count.  The first n_bar_ranges go to the
-  // bar kernel (<= 42 nodes, <= 256 entries, <= 6144 values), the rest, whose
+  // bar kernel (<= 40 nodes, <= 256 entries, <= 5120 values), the rest, whose
   // nodes touch trusses only, to the truss kernel (<= 85 nodes, <= 512 entries)
   std::vector<uint32_t> ranges;
   size_t n_bar_ranges = 0;
@@ -168,7 +168,7 @@ struct BarMaps {
   // entry slot_inc[s] (an index into inc); range r owns slots slot_ptr[r] ..
   std::vector<uint32_t> slot_inc, slot_ptr;
 };
-constexpr int BAR_WG_NODES = 42, BAR_WG_ENTRIES = 256, BAR_WG_VALUES = 6144;
+constexpr int BAR_WG_NODES = 40, BAR_WG_ENTRIES = 256, BAR_WG_VALUES = 5120;
 constexpr int TRUSS_WG_NODES = 85, TRUSS_WG_ENTRIES = 512;
 std::string upload_bar_maps(Device* d, const BarMaps& bm);
 std::string assemble_bar_owner(Device* d, double* val, void* stream, bool timed = true);

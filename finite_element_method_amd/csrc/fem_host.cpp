@@ -1121,7 +1121,7 @@ std::string build_pattern(fem_model* m) {
       }
       // workgroup ranges of consecutive nodes.  Nodes whose incident elements
       // are all trusses go to the truss kernel (<= 85 nodes, <= 512 entries);
-      // the others to the bar kernel (<= 42 nodes, <= 256 entries, <= 6144
+      // the others to the bar kernel (<= 40 nodes, <= 256 entries, <= 5120
       // CSR values, since it stages the rows in LDS).
       auto truss_only = [&](uint32_t a) {
         for (uint32_t k = bm.inc_ptr[a]; k < bm.inc_ptr[a + 1]; ++k)

@@ -190,11 +190,11 @@ def _star(kind, spokes, seed):
 
 
 # limits of the bar kernels' node ranges: a truss-only node holds at most 512
-# incident elements, a node in the general bar kernel at most 256 and 6144
-# values in its six rows (151 dense 6 x 6 blocks: 5436; 181: 6516); past
+# incident elements, a node in the general bar kernel at most 256 and 5120
+# values in its six rows (141 dense 6 x 6 blocks: 5076; 151: 5436); past
 # them the model takes the generic path
-STARS = [("truss", 300, "bar_owner"), ("truss", 600, "generic"), ("beam", 150, "bar_owner"),
-         ("beam", 180, "generic")]
+STARS = [("truss", 300, "bar_owner"), ("truss", 600, "generic"), ("beam", 140, "bar_owner"),
+         ("beam", 150, "generic")]
 
 
 @pytest.mark.parametrize("kind,spokes,path", STARS)
