@@ -106,7 +106,7 @@ constexpr int PL_NODES = 16;         // owned nodes of a wave patch
 constexpr int PL_ELEMS = 32;         // elements touching a wave patch
 constexpr int PL_TRIPS = 4;          // contributions per lane
 constexpr int PL_MAX_NB = 9;         // node-pair blocks per owned node
-constexpr int PL_NODE_STRIDE = 120;  // staging doubles per owned node: 13 nb + 1 values and a parity pad
+constexpr int PL_STAGE_VALUES = 1920;  // staged values of a patch: its nodes' 13 nb + 1 values and a parity pad per run
 struct PlateLaneMaps {
   int n_waves = 0;
   // per wave PL_ELEMS element slots (insertion order within the wave; unused

@@ -718,6 +718,7 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
   std::vector<uint32_t> cur, cur_elems;
   std::vector<std::vector<std::vector<Item>>> cur_lanes;
   size_t cur_nlanes = 0;
+  bool stage_fits = true;  // always: at most 16 x (13 x 9 + 1) values and 16 parity pads
   auto flush = [&]() {
     if (cur.empty()) return;
     std::sort(cur_elems.begin(), cur_elems.end());  // insertion order
@@ -728,12 +729,20 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
     uint32_t* nd = pm.nodes.data() + pm.nodes.size() - 2 * femdev::PL_NODES;
     size_t ln = 0;
     uint32_t trips = 0;
+    // staging offset of each owned node's values: nodes whose CSR ranges
+    // follow each other (a run) follow each other in the staging too, and a
+    // run starts at the parity of its CSR position, so that 16-byte staging
+    // chunks are 16-byte CSR chunks
+    uint32_t lds = 0, prev_end = 0;
     for (size_t s = 0; s < cur.size(); ++s) {
       const uint32_t a = cur[s];
       const uint32_t base = uint32_t(row_ptr[6 * size_t(a)]);
       const uint32_t nb = uint32_t(bptr[a + 1] - bptr[a]);
+      if (s == 0 || base != prev_end) lds += (lds ^ base) & 1u;
       nd[2 * s] = base;
-      nd[2 * s + 1] = nb;
+      nd[2 * s + 1] = nb | lds << 18;
+      lds += 13 * nb + 1;
+      prev_end = base + 13 * nb + 1;
       for (const auto& lane : cur_lanes[s]) {
         uint64_t bits = 0;
         for (size_t t = 0; t < lane.size(); ++t) {
@@ -742,7 +751,7 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
                                 uint64_t(it.k) << 9;
           bits |= code << (13 * t);
         }
-        bits |= uint64_t(lane.size()) << 52 | uint64_t(s) << 55 | uint64_t(nb) << 59 | uint64_t(base & 1u) << 63;
+        bits |= uint64_t(lane.size()) << 52 | uint64_t(s) << 55 | uint64_t(nb) << 59;
         pm.lanes[lane_base + 2 * ln] = uint32_t(bits);
         pm.lanes[lane_base + 2 * ln + 1] = uint32_t(bits >> 32);
         trips = std::max(trips, uint32_t(lane.size()));
@@ -759,7 +768,8 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
         in_wave[e] = 0;
       }
     }
-    const uint32_t hdr = uint32_t(cur_elems.size()) | uint32_t(cur.size()) << 8 | trips << 16;
+    stage_fits = stage_fits && lds <= uint32_t(femdev::PL_STAGE_VALUES);
+    const uint32_t hdr = uint32_t(cur_elems.size()) | uint32_t(cur.size()) << 6 | trips << 11;
     for (int s = 0; s < femdev::PL_NODES; ++s) nd[2 * s + 1] |= hdr << 4;
     pm.n_waves++;
     cur.clear();
@@ -768,24 +778,50 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
     cur_nlanes = 0;
   };
   std::vector<std::vector<Item>> lanes;
-  for (auto& o : order) {
-    const uint32_t a = o.second;
-    if (!node_lanes(a, lanes)) return false;
-    uint32_t new_elems = 0;
-    for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_wave[nel[nptr[a] + r] & 0x3FFFFFFFu];
-    if (cur.size() == size_t(femdev::PL_NODES) || cur_nlanes + lanes.size() > 64 ||
-        cur_elems.size() + new_elems > size_t(femdev::PL_ELEMS))
-      flush();
-    cur.push_back(a);
-    cur_nlanes += lanes.size();
-    cur_lanes.push_back(lanes);
-    for (uint32_t r = 0; r < deg[a]; ++r) {
-      const uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
-      if (!in_wave[e]) { in_wave[e] = 1; cur_elems.push_back(e); }
+  // greedy patches along a node order
+  auto pack = [&](const std::vector<std::pair<uint64_t, uint32_t>>& ord) -> bool {
+    pm.lanes.clear();
+    pm.nodes.clear();
+    pm.seid.clear();
+    pm.sconn.clear();
+    pm.n_waves = 0;
+    for (auto& o : ord) {
+      const uint32_t a = o.second;
+      if (!node_lanes(a, lanes)) return false;
+      uint32_t new_elems = 0;
+      for (uint32_t r = 0; r < deg[a]; ++r) new_elems += !in_wave[nel[nptr[a] + r] & 0x3FFFFFFFu];
+      if (cur.size() == size_t(femdev::PL_NODES) || cur_nlanes + lanes.size() > 64 ||
+          cur_elems.size() + new_elems > size_t(femdev::PL_ELEMS))
+        flush();
+      cur.push_back(a);
+      cur_nlanes += lanes.size();
+      cur_lanes.push_back(lanes);
+      for (uint32_t r = 0; r < deg[a]; ++r) {
+        const uint32_t e = nel[nptr[a] + r] & 0x3FFFFFFFu;
+        if (!in_wave[e]) { in_wave[e] = 1; cur_elems.push_back(e); }
+      }
     }
-  }
-  flush();
-  return true;
+    flush();
+    return true;
+  };
+  // Node-index order first: where consecutive nodes are neighbours (a
+  // structured numbering) its patches are strips whose CSR values form one
+  // contiguous range, written out as one run.  When that order scatters the
+  // patches' elements (more than 5/4 of the fewest possible patches), the
+  // spatial (Morton) order instead.
+  std::vector<std::pair<uint64_t, uint32_t>> by_index(order);
+  std::stable_sort(by_index.begin(), by_index.end(), [&](const std::pair<uint64_t, uint32_t>& x,
+                                                         const std::pair<uint64_t, uint32_t>& y) {
+    if (m->node_part.size() == m->nodes_number) {
+      const uint32_t px = m->node_part[x.second], py = m->node_part[y.second];
+      if (px != py) return px < py;
+    }
+    return x.second < y.second;
+  });
+  const size_t fewest = (order.size() + femdev::PL_NODES - 1) / femdev::PL_NODES;
+  if (!pack(by_index)) return false;
+  if (4 * size_t(pm.n_waves) > 5 * fewest && !pack(order)) return false;
+  return stage_fits;
 }
 
 // Corner-lane maps (plate_corners.hip).  Returns false when the mesh does not

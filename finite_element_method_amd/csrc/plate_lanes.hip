@@ -50,17 +50,19 @@ typedef unsigned int pl_u32x4 __attribute__((ext_vector_type(4)));
 constexpr int PL_REC = 58;
 constexpr int R_MYY = 10, R_MXY = 20, R_A = 36, R_B = 39, R_SH = 42, R_MAT = 50;
 constexpr int PL_NINT = 42;
+constexpr int PL_STORE_NT = 2;  // buffer-store cache policy: non-temporal
 constexpr int PL_GEO = 10;  // per wave element slot: x1, y1, x2, y2, x4, y4 about node 3; E, nu, t, k
 static_assert(PL_REC % 2 == 0, "records start on 16-byte boundaries");
 static_assert(2 * PL_ELEMS <= 64, "two lanes per element");
-constexpr int PL_LDS = PL_ELEMS * PL_REC > PL_NODES * PL_NODE_STRIDE ? PL_ELEMS * PL_REC : PL_NODES * PL_NODE_STRIDE;
+constexpr int PL_LDS = PL_ELEMS * PL_REC > PL_STAGE_VALUES ? PL_ELEMS * PL_REC : PL_STAGE_VALUES;
+static_assert(PL_STAGE_VALUES % 2 == 0 && PL_STAGE_VALUES >= PL_NODES * (13 * PL_MAX_NB + 2), "staging");
 
 // Everything a wave reads sits at fixed offsets from its index, so all its
 // loads issue at once: no load waits for another's result.
 struct LaneArgs {
   const double* __restrict__ geo;    // per wave PL_ELEMS slots of PL_GEO doubles
   // per wave PL_NODES: CSR position of row 6a of owned node s,
-  // nb | (n_elems | n_nodes << 8 | trips << 16) << 4
+  // nb | (n_elems | n_nodes << 6 | trips << 11) << 4 | staging position << 18
   const uint2* __restrict__ nodes;
   const uint2* __restrict__ lanes;   // per wave 64 lane descriptors
   double* __restrict__ val;
@@ -234,9 +236,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER
   uint2 desc, nd;
   double2 g0, g1, g2, g3, g4;
   load_patch(A, w, l, le, desc, nd, g0, g1, g2, g3, g4);
-  // nb | (n_elems | n_nodes << 8 | trips << 16) << 4 in every node entry
+  // nb | (n_elems | n_nodes << 6 | trips << 11) << 4 | staging position << 18
   const uint32_t h = uint32_t(__builtin_amdgcn_readfirstlane(int(nd.y))) >> 4;
-  const int nel = int(h & 255u), nnod = int((h >> 8) & 255u), trips = int(h >> 16);
+  const int nel = int(h & 63u), nnod = int((h >> 6) & 31u), trips = int((h >> 11) & 7u);
 
   // ---- step 1: element records, two lanes per element
   {
@@ -377,8 +379,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER
   {
     const int slot = int((desc.y >> 23) & 15u);
     const int nb = int((desc.y >> 27) & 15u);
-    const int par = int(desc.y >> 31);
-    double* seg = stage + slot * PL_NODE_STRIDE + par;
+    // staging position of the lane's node, from the node table lane `slot`
+    const int pos = __builtin_amdgcn_ds_bpermute(slot << 2, int(nd.y >> 18));
+    double* seg = stage + pos;
 #pragma unroll
     for (int t = 0; t < PL_TRIPS; ++t) {
       if (t < trips && t < cnt && (t + 1 == cnt || item_block(items, t + 1) != item_block(items, t))) {
@@ -407,27 +410,35 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER
   }
   wave_lds_sync();
 
-  // ---- step 3: copy-out, one wave store per owned node (13 nb + 1 <= 118
-  // values: at most 60 chunks), full chunks as 16-byte stores, the at most
-  // two half chunks at the node's ends as 8-byte stores
+  // ---- step 3: copy-out by runs of owned nodes whose CSR ranges follow each
+  // other (staged contiguously, at the CSR parity): consecutive lanes store
+  // consecutive aligned 16-byte chunks, the at most two half chunks at a run's
+  // ends as 8-byte stores; non-temporal (nothing in this kernel reads the
+  // values again)
   const __amdgpu_buffer_rsrc_t rsrc = __builtin_amdgcn_make_buffer_rsrc(A.val, 0, int(A.val_bytes), 0x00020000);
-  static_assert(PL_NODE_STRIDE / 2 <= 64, "one chunk per lane and node");
-#pragma unroll 1
-  for (int s = 0; s < nnod; ++s) {
-    const uint32_t base = uint32_t(__builtin_amdgcn_readlane(int(nd.x), s));
-    const uint32_t end = base + 13u * (uint32_t(__builtin_amdgcn_readlane(int(nd.y), s)) & 15u) + 1u;
-    const uint32_t g = 2u * ((base >> 1) + uint32_t(l));  // first value of this lane's chunk
-    if (g + 1u < end) {
-      const double2 v = stage2[s * (PL_NODE_STRIDE / 2) + l];
-      if (g >= base) {
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(pl_u32x4, v), rsrc, 8u * g, 0, 0);
-      } else {  // the node starts at an odd position: the chunk's upper half
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.y), rsrc, 8u * g + 8u, 0, 0);
-      }
-    } else if (g < end) {  // the node ends at an even position: the chunk's lower half
-      const double2 v = stage2[s * (PL_NODE_STRIDE / 2) + l];
-      __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.x), rsrc, 8u * g, 0, 0);
+  int s0 = 0;
+  while (s0 < nnod) {
+    const uint32_t base = uint32_t(__builtin_amdgcn_readlane(int(nd.x), s0));
+    const uint32_t sp = uint32_t(__builtin_amdgcn_readlane(int(nd.y), s0)) >> 18;  // staging position of base
+    uint32_t end = base + 13u * (uint32_t(__builtin_amdgcn_readlane(int(nd.y), s0)) & 15u) + 1u;
+    int s1 = s0 + 1;
+    while (s1 < nnod && uint32_t(__builtin_amdgcn_readlane(int(nd.x), s1)) == end) {
+      end += 13u * (uint32_t(__builtin_amdgcn_readlane(int(nd.y), s1)) & 15u) + 1u;
+      ++s1;
     }
+    const int shift = int(sp >> 1) - int(base >> 1);  // staging chunk - CSR chunk (same parity)
+    for (uint32_t c = (base >> 1) + uint32_t(l); c < (end + 1u) >> 1; c += 64u) {
+      const double2 v = stage2[int(c) + shift];
+      const uint32_t g = 2u * c;  // first value of the chunk
+      if (g >= base && g + 1u < end) {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(pl_u32x4, v), rsrc, 8u * g, 0, PL_STORE_NT);
+      } else if (g < base) {  // the run starts at an odd position: the chunk's upper half
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.y), rsrc, 8u * g + 8u, 0, PL_STORE_NT);
+      } else {  // the run ends at an odd position: the chunk's lower half
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(pl_u32x2, v.x), rsrc, 8u * g, 0, PL_STORE_NT);
+      }
+    }
+    s0 = s1;
   }
 }
 
