@@ -67,6 +67,27 @@ def backward_bytes(mesh, nnz, compliance=False):
     return src + 24 * mesh.n_nodes + 4 * nn * mesh.n_elements + 16 * n_params * mesh.n_elements + 24 * mesh.n_nodes
 
 
+def settle(fn, stream, ms):
+    """Runs fn untimed until `ms` of GPU time has passed (at most 2000 calls),
+    so that the timed steps run at the clocks a sustained loop of them gets
+    rather than those of a GPU that was idle during the setup."""
+    import torch
+
+    if ms <= 0:
+        return 0
+    done, calls = 0.0, 0
+    while done < ms and calls < 2000:
+        ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        ev0.record(stream)
+        for _ in range(10):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        done += ev0.elapsed_time(ev1)
+        calls += 10
+    return calls
+
+
 def time_steps(fn, steps, warmup, stream):
     import torch
 
@@ -240,7 +261,9 @@ def run_batched(args):
         for h, v in handles:
             L.fem_assemble_into(h, v, sh)
 
+    settle(loop, stream, args.settle_ms)
     loop_ms = time_steps(loop, args.steps, args.warmup, stream)
+    settled = settle(lambda: batch.assemble(stream), stream, args.settle_ms)
     t_wall = time.perf_counter()
     batch_ms = time_steps(lambda: batch.assemble(stream), args.steps, args.warmup, stream)
     wall = time.perf_counter() - t_wall
@@ -256,7 +279,7 @@ def run_batched(args):
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None, "algorithmic_bytes_per_launch": alg},
         "per_mesh_loop_ms": loop_ms, "speedup_vs_loop": loop_ms / batch_ms,
-        "setup_s": setup_s, "wall_s": wall,
+        "setup_s": setup_s, "wall_s": wall, "settle": {"ms": args.settle_ms, "calls": settled},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline("plate")
@@ -306,6 +329,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--settle-ms", type=float, default=100.0,
+                    help="untimed GPU time spent on the step before the warm-up steps (clocks settle)")
     ap.add_argument("--workload", default="plate_1024", choices=sorted(WORKLOADS) + ["batched_256"])
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-grad", action="store_true", help="skip the backward / sensitivity timings")
@@ -362,6 +387,7 @@ def main():
     val = torch.empty(nnz, dtype=torch.float64, device=dev)
     stream = torch.cuda.current_stream()
 
+    settled = settle(lambda: model.assemble_into(val, stream), stream, args.settle_ms)
     for _ in range(args.warmup):
         model.assemble_into(val, stream)
     torch.cuda.synchronize()
@@ -401,6 +427,7 @@ def main():
                      "algorithmic_bytes_per_launch": alg},
         "setup_s": setup_s,
         "wall_s": wall,
+        "settle": {"ms": args.settle_ms, "calls": settled},
     }
     flops = profile_figure(model.assembly_path(), args.workload, "fp64_flops_per_launch")
     if flops is not None:

@@ -36,6 +36,8 @@ namespace femdev {
 
 namespace {
 
+typedef double bo_d2 __attribute__((ext_vector_type(2)));
+
 constexpr int BO_THREADS = 256;
 constexpr int BO_NODES = 40;     // owned nodes per workgroup (10 per wave, 6 row lanes each)
 constexpr int BO_ENTRIES = 256;  // (owned node, incident element) entries per workgroup
@@ -341,11 +343,15 @@ __global__ __launch_bounds__(BO_THREADS) void bar_owner_kernel(BarArgs A) {
   }
   __syncthreads();
   double* __restrict__ dst = A.val + base;
-  // 16-byte stores from the first even CSR position; the odd values at either end alone
+  // 16-byte stores from the first even CSR position, non-temporal (nothing in
+  // this kernel reads the values again); the odd values at either end alone
   const int head = int(base & 1), npair = (nv - head) >> 1;
   for (int k = t; k < npair; k += BO_THREADS) {
     const int o = head + 2 * k;
-    *reinterpret_cast<double2*>(dst + o) = make_double2(L.out[o], L.out[o + 1]);
+    bo_d2 v;
+    v.x = L.out[o];
+    v.y = L.out[o + 1];
+    __builtin_nontemporal_store(v, reinterpret_cast<bo_d2*>(dst + o));
   }
   if (t == 0 && head && nv > 0) dst[0] = L.out[0];
   if (t == BO_THREADS - 1 && nv > head && ((nv - head) & 1)) dst[nv - 1] = L.out[nv - 1];

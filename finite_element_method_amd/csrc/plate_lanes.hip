@@ -427,7 +427,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER
       ++s1;
     }
     const int shift = int(sp >> 1) - int(base >> 1);  // staging chunk - CSR chunk (same parity)
-    for (uint32_t c = (base >> 1) + uint32_t(l); c < (end + 1u) >> 1; c += 64u) {
+    // lane l of a store covers chunk l of a 128-byte-aligned group of 64 chunks
+    for (uint32_t c = ((base >> 1) & ~7u) + uint32_t(l); c < (end + 1u) >> 1; c += 64u) {
+      if (c < (base >> 1)) continue;
       const double2 v = stage2[int(c) + shift];
       const uint32_t g = 2u * c;  // first value of the chunk
       if (g >= base && g + 1u < end) {
