@@ -46,6 +46,7 @@ struct AdjArgs {
   // flat plates, per (element, p): CSR position of row 6 a of corner p's node a,
   // and the block index k of each q (6 bits each) | nb << 24
   const uint2* kpack;
+  const uint32_t* eorder;  // flat plates: the elements in workgroup order
   const int64_t* row_ptr;
   double* dparam;  // n x params_per, may be null
   double* xs;      // coordinate-gradient rows in node-major slots (3 doubles each), may be null
@@ -430,9 +431,9 @@ __global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void pl
   const unsigned b = xcd_block(blockIdx.x, gridDim.x);
   const int le = int(threadIdx.x >> 2);
   const int p = int(threadIdx.x & 3u);
-  const size_t e0 = size_t(b) * 16 + le;
-  const bool live = e0 < A.n;
-  const size_t e = live ? e0 : A.n - 1;  // dead lanes still take part in the lane sums
+  const size_t i0 = size_t(b) * 16 + le;
+  const bool live = i0 < A.n;
+  const size_t e = A.eorder[live ? i0 : A.n - 1];  // dead lanes still take part in the lane sums
   // gradient-block positions first: the value loads wait on this one load only
   const uint2 kr = A.kpack[4 * e + p];
   const uint32_t g0 = kr.x, kp = kr.y;
@@ -843,6 +844,8 @@ std::string upload_adjoint_maps(Device* d, const AdjointMaps& am) {
   if (am.flat) {
     e = dupload(d->adj_kpack, am.kpack.data(), am.kpack.size());
     if (!e.empty()) return e;
+    e = dupload(d->adj_eorder, am.eorder.data(), am.eorder.size());
+    if (!e.empty()) return e;
   }
   d->adj_nxs = am.n_xs_rows;
   return dalloc(d->adj_xs, size_t(3 * am.n_xs_rows));
@@ -867,6 +870,7 @@ std::string run_adjoint(Device* d, int mode, const double* g, double* const dpar
     A.bmask = d->adj_bmask;
     A.boff = d->adj_boff;
     A.kpack = reinterpret_cast<const uint2*>(d->adj_kpack);
+    A.eorder = d->adj_eorder;
     A.row_ptr = d->row_ptr;
     A.dparam = dparam[t];
     A.xs = (dxyz && t != BEAM) ? d->adj_xs : nullptr;

@@ -60,7 +60,7 @@ struct Device {
   uint32_t* adj_bid[3] = {nullptr, nullptr, nullptr};
   uint64_t* adj_bmask = nullptr;
   uint16_t* adj_boff = nullptr;
-  uint32_t *adj_inc_ptr = nullptr, *adj_xpos = nullptr, *adj_kpack = nullptr;
+  uint32_t *adj_inc_ptr = nullptr, *adj_xpos = nullptr, *adj_kpack = nullptr, *adj_eorder = nullptr;
   double* adj_xs = nullptr;
   int64_t adj_xoff[3] = {0, 0, 0};
   int64_t adj_nxs = 0;

@@ -59,7 +59,7 @@ void destroy_device(Device* d) {
   for (int t = 0; t < 2; ++t) f(d->bo_bid[t]);
   f(d->bo_bmask); f(d->bo_boff); f(d->bo_inc_ptr); f(d->bo_inc); f(d->bo_ranges);
   f(d->bo_slot_inc); f(d->bo_slot_ptr); f(d->bo_slot);
-  f(d->adj_bmask); f(d->adj_boff); f(d->adj_inc_ptr); f(d->adj_xpos); f(d->adj_xs); f(d->adj_kpack);
+  f(d->adj_bmask); f(d->adj_boff); f(d->adj_inc_ptr); f(d->adj_xpos); f(d->adj_xs); f(d->adj_kpack); f(d->adj_eorder);
   f(d->sep_cons); f(d->sep_pack); f(d->sep_cls); f(d->sep_tmp); f(d->sep_disp); f(d->sep_force); f(d->sep_ra); f(d->sep_ub);
   f(d->sep_v); f(d->sep_flag); f(d->sep_pos); f(d->sep_key); f(d->sep_cnt); f(d->sep_off); f(d->sep_aa); f(d->sep_bb);
   f(d->sep_rp); f(d->sep_sum); f(d->sep_ci); f(d->sep_dpos);

@@ -224,6 +224,10 @@ struct AdjointMaps {
   // flat: per (element, p) two words: the CSR position of row 6 a of corner
   // p's node a; the block index within a's rows of each q (6 bits each) | nb << 24
   std::vector<uint32_t> kpack;
+  // flat: the elements in the order the flat backward takes them, 16 per
+  // workgroup (Morton order of node 3: compact tiles, whose lanes' gathers
+  // share cache lines)
+  std::vector<uint32_t> eorder;
 };
 std::string upload_adjoint_maps(Device* d, const AdjointMaps& am);
 // mode 0: g = dL/dK (nnz values, CSR order); mode 1: g = nodal displacements

@@ -1317,6 +1317,26 @@ int ensure_adjoint(fem_model* m) {
           am.kpack[8 * e + 2 * p] = uint32_t(m->row_ptr_host[6 * size_t(a)]);
           am.kpack[8 * e + 2 * p + 1] = w;
         }
+      double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+      for (size_t a = 0; a < N; ++a)
+        for (int k = 0; k < 3; ++k) {
+          lo[k] = std::min(lo[k], m->xyz[3 * a + k]);
+          hi[k] = std::max(hi[k], m->xyz[3 * a + k]);
+        }
+      std::vector<std::pair<uint64_t, uint32_t>> ord(S.size());
+      for (size_t e = 0; e < S.size(); ++e) {
+        const uint32_t a = S.conn[4 * e + 2];
+        uint64_t code = 0;
+        for (int k = 0; k < 3; ++k) {
+          const double span = hi[k] - lo[k];
+          const uint64_t qv = span > 0 ? uint64_t((m->xyz[3 * a + k] - lo[k]) / span * 2097151.0) : 0;
+          code |= spread3(qv) << k;
+        }
+        ord[e] = {code, uint32_t(e)};
+      }
+      std::sort(ord.begin(), ord.end());
+      am.eorder.resize(S.size());
+      for (size_t i = 0; i < S.size(); ++i) am.eorder[i] = ord[i].second;
     }
   }
   am.bmask = m->pat_bmask;
