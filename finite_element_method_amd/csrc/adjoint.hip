@@ -70,8 +70,6 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
 // buffer loads, range-checked against g: half the memory requests of one load
 // per value, and the requests, not the bytes, bound this gather.
 __device__ inline void gather_block(const AdjArgs& A, uint32_t k, uint32_t a, double G[36]) {
-  typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
-  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
   const uint64_t um = A.bmask[k];
   const __amdgpu_buffer_rsrc_t rsrc =
       __builtin_amdgcn_make_buffer_rsrc(const_cast<double*>(A.g), 0, int(A.g_bytes), 0x00020000);

@@ -323,11 +323,12 @@ int fem_batch_mesh_csr(fem_batch* b, size_t mesh, int64_t* n_rows, int64_t* nnz,
 int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int32_t* col_idx, double* val);
 
 /* Tuning options (no counterpart in the crate; they change speed, never values):
- *   "plate_kernel"       fused kernel of flat plate meshes: 1 = one lane per
- *                        owned node-pair block (plate_blocks, default),
- *                        0 = row staging (plate_fused), 2 = one lane per
- *                        element corner (plate_corners; meshes whose nodes
- *                        have at most four elements); bitwise equal values
+ *   "plate_kernel"       fused kernel of flat plate meshes: 3 = balanced lanes
+ *                        (plate_lanes, default; meshes whose nodes have at
+ *                        most four elements, else plate_blocks), 1 = one lane
+ *                        per owned node-pair block (plate_blocks), 0 = row
+ *                        staging (plate_fused), 2 = one lane per element
+ *                        corner (plate_corners); bitwise equal values
  *   "plate_patch_nodes"  owned nodes per workgroup of the row-staging kernel: 4..64
  *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernels,
  *                        looping over the patches: 0..64 (0 = one workgroup
@@ -342,12 +343,15 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
  *                        (the paths taken above 4 GiB).  Gradients are bitwise
  *                        equal either way; PCG iterates differ by rounding (the
  *                        SpMV sums a row in another order)
+ *   "time_assembly"      1 (default) = events around every assembly launch
+ *                        for fem_last_assembly_ms; 0 = none (two fewer
+ *                        packets between back-to-back launches)
  * Setting a plate or path option invalidates the pattern (rebuilt by the next
  * assembly). */
 int fem_set_option(fem_model* m, const char* key, int64_t value);
 
-/* Name of the value path of the last assembly: "plate_blocks", "plate_fused",
- * "plate_corners", "bar_owner" or "generic". */
+/* Name of the value path of the last assembly: "plate_lanes", "plate_blocks",
+ * "plate_fused", "plate_corners", "bar_owner" or "generic". */
 const char* fem_assembly_path(fem_model* m);
 
 #ifdef __cplusplus
