@@ -43,9 +43,13 @@ struct Device {
   uint32_t *pb_hdr = nullptr, *pb_conn = nullptr, *pb_eid = nullptr, *pb_blk = nullptr;
   // plate balanced-lane maps (the element lists share pb_conn / pb_eid)
   int pl_waves = 0;
-  uint32_t *pl_nodes = nullptr, *pl_lanes = nullptr;
+  uint32_t *pl_nodes = nullptr, *pl_lanes = nullptr, *pl_lane_set = nullptr;
   double* pl_geo = nullptr;   // per wave element slot: local coordinates and parameters
   bool pl_geo_dirty = false;  // node coordinates or plate parameters changed since pl_geo was filled
+  // every plate has the same E, nu and k (bitwise): the slots hold t only and
+  // the three values go to the kernel as arguments
+  bool pl_uniform = false;
+  double pl_mat[3] = {0.0, 0.0, 0.0};
   // bar owner maps
   uint32_t* bo_bid[2] = {nullptr, nullptr};
   uint64_t* bo_bmask = nullptr;

@@ -64,7 +64,7 @@ void destroy_device(Device* d) {
   f(d->sep_v); f(d->sep_flag); f(d->sep_pos); f(d->sep_key); f(d->sep_cnt); f(d->sep_off); f(d->sep_aa); f(d->sep_bb);
   f(d->sep_rp); f(d->sep_sum); f(d->sep_ci); f(d->sep_dpos);
   f(d->pb_hdr); f(d->pb_conn); f(d->pb_eid); f(d->pb_blk);
-  f(d->pl_nodes); f(d->pl_lanes); f(d->pl_geo);
+  f(d->pl_nodes); f(d->pl_lanes); f(d->pl_lane_set); f(d->pl_geo);
   for (int k = 0; k < 6; ++k) f(d->sol_vec[k]);
   f(d->sol_rr); f(d->sol_rb); f(d->sol_part); f(d->sol_s); f(d->sol_diag); f(d->sol_inv); f(d->sol_flag);
   f(d->sol_bid); f(d->sol_starts); f(d->sol_tmp); f(d->u_glob);

@@ -114,15 +114,21 @@ struct PlateLaneMaps {
   std::vector<uint32_t> sconn;
   std::vector<uint32_t> seid;
   // per wave PL_NODES x 2: CSR position of row 6a of owned node s (0 past the
-  // last owned node), nb | (n_elems | n_nodes << 8 | trips << 16) << 4
+  // last owned node), nb | (n_elems | n_nodes << 6 | trips << 11) << 4 |
+  // staging position << 18
   std::vector<uint32_t> nodes;
-  // per wave 64 lanes x 2 words (64 bits): up to four items of 13 bits
-  // (wave element | p << 5 | q << 7 | block k << 9) in bits 0..51, item count
-  // in 52..54, owned-node slot in 55..58, nb in 59..62, parity of the node's
-  // CSR position in 63.  Items are grouped by block, insertion order within.
+  // distinct lane-descriptor sets, 64 lanes x 2 words (64 bits) each: up to
+  // four items of 13 bits (wave element | p << 5 | q << 7 | block k << 9) in
+  // bits 0..51, item count in 52..54, owned-node slot in 55..58, nb in 59..62.
+  // Items are grouped by block, insertion order within.  Waves with the same
+  // set (all interior patches of a structured mesh) share one copy.
   std::vector<uint32_t> lanes;
+  std::vector<uint32_t> lane_set;  // per wave: its set in `lanes`
 };
 std::string upload_plate_lane_maps(Device* d, const PlateLaneMaps& pm);
+// E, nu and k shared by every plate (uniform) or per element: sets the slot
+// layout (8 or 10 doubles) and marks the slots for refilling
+std::string set_plate_lane_material(Device* d, bool uniform, const double mat[3]);
 std::string assemble_plate_lanes(Device* d, double* val, void* stream, bool timed = true);
 
 // Corner-lane fused plate path (plate_corners.hip): same preconditions as the

@@ -109,6 +109,7 @@ struct fem_model {
   bool force_generic = false;
   bool wide_loads = true;  // backward block gathers and PCG SpMV: 16-byte loads when the values fit 32-bit offsets
   bool time_assembly = true;  // events around each assembly launch (fem_last_assembly_ms)
+  int pl_order = 0;  // balanced-lane plate patches: 0 = automatic, 1 = node-index order, 2 = Morton order
   // node-pair blocks of the pattern (kept for the adjoint maps): node a owns
   // blocks pat_bptr[a] .. pat_bptr[a + 1], sorted by neighbour pat_bnode[k]
   std::vector<uint64_t> pat_bptr;
@@ -671,6 +672,21 @@ bool build_plate_block_maps(fem_model* m, const std::vector<uint64_t>& bptr, con
   return flush();
 }
 
+// true when every plate has the same E, nu and k, bit for bit (into mat); the
+// balanced-lane slots then hold t only
+bool plate_material_uniform(const fem_model* m, double mat[3]) {
+  const auto& S = m->set[PLATE];
+  if (!S.size()) return false;
+  const double* P = S.params.data();
+  const double first[3] = {P[0], P[1], P[3]};
+  for (size_t e = 0; e < S.size(); ++e) {
+    const double v[3] = {P[4 * e], P[4 * e + 1], P[4 * e + 3]};
+    if (std::memcmp(v, first, sizeof(v)) != 0) return false;
+  }
+  for (int k = 0; k < 3; ++k) mat[k] = first[k];
+  return true;
+}
+
 // Maps of the balanced-lane plate kernel (plate_lanes.hip).  Called only after
 // build_plate_maps accepted the mesh.  Returns false when a node has more than
 // PL_TRIPS elements or more than PL_MAX_NB node-pair blocks (the caller then
@@ -725,6 +741,7 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
     for (size_t i = 0; i < cur_elems.size(); ++i) elem_local[cur_elems[i]] = int32_t(i);
     size_t lane_base = pm.lanes.size();
     pm.lanes.resize(lane_base + 128, 0u);
+    pm.lane_set.push_back(uint32_t(pm.n_waves));
     pm.nodes.resize(pm.nodes.size() + 2 * femdev::PL_NODES, 0u);
     uint32_t* nd = pm.nodes.data() + pm.nodes.size() - 2 * femdev::PL_NODES;
     size_t ln = 0;
@@ -781,6 +798,7 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
   // greedy patches along a node order
   auto pack = [&](const std::vector<std::pair<uint64_t, uint32_t>>& ord) -> bool {
     pm.lanes.clear();
+    pm.lane_set.clear();
     pm.nodes.clear();
     pm.seid.clear();
     pm.sconn.clear();
@@ -802,6 +820,20 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
       }
     }
     flush();
+    // one copy of each distinct descriptor set
+    std::unordered_map<std::string, uint32_t> sets;
+    std::vector<uint32_t> uniq;
+    for (size_t w = 0; w < size_t(pm.n_waves); ++w) {
+      const uint32_t* src = pm.lanes.data() + 128 * w;
+      std::string key(reinterpret_cast<const char*>(src), 128 * sizeof(uint32_t));
+      auto it = sets.find(key);
+      if (it == sets.end()) {
+        it = sets.emplace(std::move(key), uint32_t(uniq.size() / 128)).first;
+        uniq.insert(uniq.end(), src, src + 128);
+      }
+      pm.lane_set[w] = it->second;
+    }
+    pm.lanes.swap(uniq);
     return true;
   };
   // Node-index order first: where consecutive nodes are neighbours (a
@@ -819,8 +851,9 @@ bool build_plate_lane_maps(fem_model* m, const std::vector<uint64_t>& bptr, cons
     return x.second < y.second;
   });
   const size_t fewest = (order.size() + femdev::PL_NODES - 1) / femdev::PL_NODES;
+  if (m->pl_order == 2) return pack(order) && stage_fits;
   if (!pack(by_index)) return false;
-  if (4 * size_t(pm.n_waves) > 5 * fewest && !pack(order)) return false;
+  if (m->pl_order == 0 && 4 * size_t(pm.n_waves) > 5 * fewest && !pack(order)) return false;
   return stage_fits;
 }
 
@@ -1101,6 +1134,9 @@ std::string build_pattern(fem_model* m) {
       femdev::PlateBlockMaps pbm;
       femdev::PlateLaneMaps plm;
       if (m->pf_kernel == 3 && build_plate_lane_maps(m, bptr, blocks, row_ptr, plm)) {
+        double mat[3];
+        e = femdev::set_plate_lane_material(m->dev, plate_material_uniform(m, mat), mat);
+        if (!e.empty()) return e;
         e = femdev::upload_plate_lane_maps(m->dev, plm);
         if (!e.empty()) return e;
         m->path = "plate_lanes";
@@ -2145,6 +2181,11 @@ int fem_update_params(fem_model* m, int type, const double* params_device, void*
   auto e = femdev::replace_params(m->dev, type, params_device, m->set[type].params.data(), stream);
   if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
   m->own_vals = false;
+  if (type == PLATE && m->path == "plate_lanes") {  // the slot layout follows the new parameters
+    double mat[3];
+    e = femdev::set_plate_lane_material(m->dev, plate_material_uniform(m, mat), mat);
+    if (!e.empty()) return fail(m, FEM_ERR_DEVICE, e);
+  }
   return type == BEAM ? rerun_setup(m, false, BEAM) : FEM_OK;
 }
 
@@ -2403,6 +2444,9 @@ int fem_set_option(fem_model* m, const char* key, int64_t value) {
   } else if (k == "plate_kernel") {
     if (value < 0 || value > 3) return fail(m, FEM_ERR_ARGUMENT, "plate_kernel must be 0, 1, 2 or 3");
     m->pf_kernel = int(value);
+  } else if (k == "plate_patch_order") {
+    if (value < 0 || value > 2) return fail(m, FEM_ERR_ARGUMENT, "plate_patch_order must be 0, 1 or 2");
+    m->pl_order = int(value);
   } else if (k == "separate_lanes") {
     if (value != 0 && value != 4 && value != 8 && value != 16 && value != 32)
       return fail(m, FEM_ERR_ARGUMENT, "separate_lanes must be 0 (automatic), 4, 8, 16 or 32");

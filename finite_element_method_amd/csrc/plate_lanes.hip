@@ -51,7 +51,8 @@ constexpr int PL_REC = 58;
 constexpr int R_MYY = 10, R_MXY = 20, R_A = 36, R_B = 39, R_SH = 42, R_MAT = 50;
 constexpr int PL_NINT = 42;
 constexpr int PL_STORE_NT = 2;  // buffer-store cache policy: non-temporal
-constexpr int PL_GEO = 10;  // per wave element slot: x1, y1, x2, y2, x4, y4 about node 3; E, nu, t, k
+constexpr int PL_GEO = 10;   // per wave element slot: x1, y1, x2, y2, x4, y4 about node 3; E, nu, t, k
+constexpr int PL_GEO_U = 8;  // the same when E, nu, k are uniform: x1 .. y4; t; pad
 static_assert(PL_REC % 2 == 0, "records start on 16-byte boundaries");
 static_assert(2 * PL_ELEMS <= 64, "two lanes per element");
 constexpr int PL_LDS = PL_ELEMS * PL_REC > PL_STAGE_VALUES ? PL_ELEMS * PL_REC : PL_STAGE_VALUES;
@@ -64,40 +65,48 @@ struct LaneArgs {
   // per wave PL_NODES: CSR position of row 6a of owned node s,
   // nb | (n_elems | n_nodes << 6 | trips << 11) << 4 | staging position << 18
   const uint2* __restrict__ nodes;
-  const uint2* __restrict__ lanes;   // per wave 64 lane descriptors
+  const uint2* __restrict__ lanes;   // 64 lane descriptors per distinct set
+  const uint32_t* __restrict__ lane_set;  // per wave: its descriptor set
   double* __restrict__ val;
   uint32_t val_bytes;
+  uint32_t geo_stride;  // PL_GEO, or PL_GEO_U when E, nu, k are uniform (then in mat)
+  double mat[3];
 };
 
 // inputs of wave patch w: the lane's descriptor, owned node (l mod 16) and
 // the geometry slot of element l / 2
 __device__ inline void load_patch(const LaneArgs& A, uint32_t w, int l, int le, uint2& desc, uint2& nd, double2& g0,
                                   double2& g1, double2& g2, double2& g3, double2& g4) {
-  desc = A.lanes[size_t(w) * 64 + l];
+  desc = A.lanes[size_t(A.lane_set[w]) * 64 + l];
   nd = A.nodes[size_t(w) * PL_NODES + (l & (PL_NODES - 1))];
-  const double2* gs = reinterpret_cast<const double2*>(A.geo + (size_t(w) * PL_ELEMS + le) * PL_GEO);
+  const double2* gs = reinterpret_cast<const double2*>(A.geo + (size_t(w) * PL_ELEMS + le) * A.geo_stride);
   g0 = gs[0];
   g1 = gs[1];
   g2 = gs[2];
-  g3 = gs[3];
-  g4 = gs[4];
+  if (A.geo_stride == PL_GEO_U) {  // (E, nu) and (t, k) from the uniform values
+    g3 = make_double2(A.mat[0], A.mat[1]);
+    g4 = make_double2(gs[3].x, A.mat[2]);
+  } else {
+    g3 = gs[3];
+    g4 = gs[4];
+  }
 }
 
 // Fills the wave slots' geometry: local coordinates of nodes 1, 2, 4 about
 // node 3 (the same differences the crate's plate frame takes, quad:340-378,
-// with an identity frame) and the element parameters.
+// with an identity frame) and the element parameters (t only when E, nu and k
+// are uniform: stride PL_GEO_U).
 __global__ __launch_bounds__(256) void plate_lanes_gather_kernel(const double* __restrict__ xyz,
                                                                  const double* __restrict__ params,
                                                                  const uint32_t* __restrict__ sconn,
                                                                  const uint32_t* __restrict__ seid, size_t n_slots,
-                                                                 double* __restrict__ geo) {
+                                                                 int stride, double* __restrict__ geo) {
   const size_t i = blockIdx.x * size_t(blockDim.x) + threadIdx.x;
   if (i >= n_slots) return;
   const uint32_t eid = seid[i];
-  double* g = geo + i * PL_GEO;
+  double* g = geo + i * stride;
   if (eid == 0xFFFFFFFFu) {  // unused slot
-#pragma unroll
-    for (int k = 0; k < PL_GEO; ++k) g[k] = 0.0;
+    for (int k = 0; k < stride; ++k) g[k] = 0.0;
     return;
   }
   const uint4 c = *reinterpret_cast<const uint4*>(sconn + 4 * i);
@@ -114,8 +123,13 @@ __global__ __launch_bounds__(256) void plate_lanes_gather_kernel(const double* _
     g[2 * m] = X[a][0] - X[2][0];
     g[2 * m + 1] = X[a][1] - X[2][1];
   }
+  if (stride == PL_GEO_U) {
+    g[6] = params[4 * size_t(eid) + 2];
+    g[7] = 0.0;
+  } else {
 #pragma unroll
-  for (int k = 0; k < 4; ++k) g[6 + k] = params[4 * size_t(eid) + k];
+    for (int k = 0; k < 4; ++k) g[6 + k] = params[4 * size_t(eid) + k];
+  }
 }
 
 // index of the symmetric pair (a, b), a <= b, among the 10 pairs of 4 nodes
@@ -448,7 +462,8 @@ std::string plate_lanes_gather(Device* d, hipStream_t s) {
   const size_t n_slots = size_t(d->pl_waves) * PL_ELEMS;
   if (n_slots) {
     plate_lanes_gather_kernel<<<unsigned((n_slots + 255) / 256), 256, 0, s>>>(d->xyz, d->params[PLATE], d->pb_conn,
-                                                                            d->pb_eid, n_slots, d->pl_geo);
+                                                                            d->pb_eid, n_slots,
+                                                                            d->pl_uniform ? PL_GEO_U : PL_GEO, d->pl_geo);
     HIP_TRY(hipGetLastError());
   }
   d->pl_geo_dirty = false;
@@ -464,10 +479,18 @@ std::string upload_plate_lane_maps(Device* d, const PlateLaneMaps& pm) {
   if (!(e = dupload(d->pb_eid, pm.seid.data(), pm.seid.size())).empty()) return e;
   if (!(e = dupload(d->pl_nodes, pm.nodes.data(), pm.nodes.size())).empty()) return e;
   if (!(e = dupload(d->pl_lanes, pm.lanes.data(), pm.lanes.size())).empty()) return e;
-  if (!(e = dalloc(d->pl_geo, size_t(pm.n_waves) * PL_ELEMS * PL_GEO)).empty()) return e;
+  if (!(e = dupload(d->pl_lane_set, pm.lane_set.data(), pm.lane_set.size())).empty()) return e;
+  if (!(e = dalloc(d->pl_geo, size_t(pm.n_waves) * PL_ELEMS * PL_GEO)).empty()) return e;  // either layout fits
   e = plate_lanes_gather(d, nullptr);
   if (!e.empty()) return e;
   HIP_TRY(hipDeviceSynchronize());
+  return "";
+}
+
+std::string set_plate_lane_material(Device* d, bool uniform, const double mat[3]) {
+  d->pl_uniform = uniform;
+  for (int k = 0; k < 3; ++k) d->pl_mat[k] = uniform ? mat[k] : 0.0;
+  d->pl_geo_dirty = true;
   return "";
 }
 
@@ -483,6 +506,9 @@ std::string assemble_plate_lanes(Device* d, double* val, void* stream, bool time
     a.geo = d->pl_geo;
     a.nodes = reinterpret_cast<const uint2*>(d->pl_nodes);
     a.lanes = reinterpret_cast<const uint2*>(d->pl_lanes);
+    a.lane_set = d->pl_lane_set;
+    a.geo_stride = uint32_t(d->pl_uniform ? PL_GEO_U : PL_GEO);
+    for (int k = 0; k < 3; ++k) a.mat[k] = d->pl_mat[k];
     a.val = val;
     a.val_bytes = uint32_t(d->nnz * 8);
     plate_lanes_kernel<<<d->pl_waves, 64, 0, s>>>(a);

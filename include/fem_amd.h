@@ -329,6 +329,10 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
  *                        per owned node-pair block (plate_blocks), 0 = row
  *                        staging (plate_fused), 2 = one lane per element
  *                        corner (plate_corners); bitwise equal values
+ *   "plate_patch_order"  patches of the balanced-lane kernel: 0 = automatic
+ *                        (node-index order unless it scatters the patches'
+ *                        elements, then Morton order), 1 = node-index order,
+ *                        2 = Morton order; bitwise equal values
  *   "plate_patch_nodes"  owned nodes per workgroup of the row-staging kernel: 4..64
  *   "plate_wg_per_cu"    resident workgroups per CU of the fused plate kernels,
  *                        looping over the patches: 0..64 (0 = one workgroup

@@ -113,6 +113,24 @@ def test_plate_kernels_agree(gpu, name):
     assert np.abs(out[3][2] - out[0][2]).max() <= 1e-15 * np.abs(out[0][2]).max()
 
 
+@pytest.mark.parametrize("name", ["plate_17x9", "plate_64_shuffled"])
+def test_plate_patch_orders_agree(gpu, name):
+    """The balanced-lane kernel's patches along node-index order (strips on a
+    structured numbering, one run each) and along Morton order (several runs
+    per patch) give bitwise identical values, and the automatic choice takes
+    one of them."""
+    mesh = CASES[name]()
+    out = {}
+    for order in (0, 1, 2):
+        m = FEM.from_mesh(mesh)
+        m.set_option("plate_patch_order", order)
+        out[order] = m.stiffness_csr()
+        assert m.assembly_path() == "plate_lanes"
+    for order in (1, 2):
+        for a, b in zip(out[0], out[order]):
+            assert np.array_equal(a, b), order
+
+
 def _mixed_bars(model_cls):
     """A 5^3 beam frame and a 5^3 truss lattice beside it, truss diagonals
     inside the frame (tapered and not) and trusses bridging the two, added to

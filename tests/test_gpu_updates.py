@@ -142,3 +142,28 @@ def test_rebuilt_pattern_is_not_assembled(gpu):
     with pytest.raises(DeviceError, match="not assembled"):
         m.separate(assemble=False)
     m.separate(assemble=True)  # assembling again makes it valid
+
+
+def test_plate_material_layout_switch(gpu):
+    """Balanced-lane plates whose E, nu and k are the same for every element
+    keep only t per slot; an update that makes E vary per element switches the
+    slots to the full layout, and one that makes it uniform again switches
+    back.  Every step gives the CSR of a model built from scratch."""
+    mesh = meshgen.plate_grid(20, seed=11)
+    m = FEM.from_mesh(mesh)
+    assert _same(m.stiffness_csr(), FEM.from_mesh(mesh).stiffness_csr())
+    assert m.assembly_path() == "plate_lanes"
+    rng = np.random.default_rng(12)
+    varied = Mesh(mesh.kind, mesh.node_numbers.copy(), mesh.xyz.copy(), mesh.numbers.copy(), mesh.conn.copy(),
+                  mesh.props.copy(), mesh.rel_tol, mesh.abs_tol)
+    varied.props[:, 0] *= rng.uniform(0.9, 1.1, mesh.n_elements)  # E per element
+    uniform = Mesh(mesh.kind, mesh.node_numbers.copy(), mesh.xyz.copy(), mesh.numbers.copy(), mesh.conn.copy(),
+                   mesh.props.copy(), mesh.rel_tol, mesh.abs_tol)
+    uniform.props[:, 0] = 2.0e11  # a new shared E
+    uniform.props[:, 2] *= rng.uniform(0.8, 1.25, mesh.n_elements)
+    for new in (varied, uniform):
+        m.update_params(PLATE, _cuda(new.props))
+        got = m.stiffness_csr()
+        fresh = FEM.from_mesh(new)
+        assert _same(got, fresh.stiffness_csr())
+        assert m.assembly_path() == fresh.assembly_path() == "plate_lanes"
