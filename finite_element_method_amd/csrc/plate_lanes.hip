@@ -234,10 +234,7 @@ __device__ inline void lane_points(const double xl[6], int half, fem::PlatePoint
 __device__ inline int item_block(uint64_t items, int t) { return int(uint32_t(items >> (13 * t + 9)) & 15u); }
 
 
-#ifndef PL_WAVES_PER_EU
-#define PL_WAVES_PER_EU 2
-#endif
-__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(PL_WAVES_PER_EU))) void plate_lanes_kernel(
+__global__ __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2))) void plate_lanes_kernel(
     LaneArgs A) {
   // element records during steps 1-2, the owned nodes' staged values after
   __shared__ double2 lds2[PL_LDS / 2];
