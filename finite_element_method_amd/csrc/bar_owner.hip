@@ -18,9 +18,9 @@
 // first contribution); an off-diagonal block (a, b) has exactly one
 // contributing element (checked when the maps are built).  The rows are formed
 // in LDS and the workgroup's owned rows, one contiguous CSR range, are then
-// written out by consecutive lanes as 16-byte stores (a scattered row-by-row
-// write costs one memory request per value).  Every CSR value is written once:
-// no contribution buffer, no atomics, bitwise reproducible.
+// written out by consecutive lanes as non-temporal 16-byte stores (a scattered
+// row-by-row write costs one memory request per value).  Every CSR value is
+// written once: no contribution buffer, no atomics, bitwise reproducible.
 //
 // Beam blocks use the rank-one form of beam.rs:495-605: with T = diag(R, R),
 //   row i of T^T (sum_t s_t v_t,p v_t,q^T) T = sum_t s_t (T^T v_t,p)_i (T^T v_t,q)^T.
