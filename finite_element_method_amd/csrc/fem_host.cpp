@@ -1359,14 +1359,17 @@ int ensure_adjoint(fem_model* m) {
           lo[k] = std::min(lo[k], m->xyz[3 * a + k]);
           hi[k] = std::max(hi[k], m->xyz[3 * a + k]);
         }
+      // Morton order of the element centroids on a 2^21 grid over the bounding
+      // box (a regular 2^k x 2^k mesh then falls into aligned 4 x 4 patches)
       std::vector<std::pair<uint64_t, uint32_t>> ord(S.size());
       for (size_t e = 0; e < S.size(); ++e) {
-        const uint32_t a = S.conn[4 * e + 2];
         uint64_t code = 0;
         for (int k = 0; k < 3; ++k) {
+          double c = 0.0;
+          for (int p = 0; p < 4; ++p) c += m->xyz[3 * size_t(S.conn[4 * e + p]) + k];
           const double span = hi[k] - lo[k];
-          const uint64_t qv = span > 0 ? uint64_t((m->xyz[3 * a + k] - lo[k]) / span * 2097151.0) : 0;
-          code |= spread3(qv) << k;
+          const double qd = span > 0 ? (0.25 * c - lo[k]) / span * 2097152.0 : 0.0;
+          code |= spread3(uint64_t(std::min(std::max(qd, 0.0), 2097151.0))) << k;
         }
         ord[e] = {code, uint32_t(e)};
       }
