@@ -1,0 +1,147 @@
+"""Writes plate_mitc4_golden.json: the local stiffness of single MITC4 plates,
+evaluated in 50-digit arithmetic (mpmath), as fixtures for the FP64 paths.
+
+A restatement of the crate's plate element written for this purpose: it shares
+no code with the FP64 oracle or the product.  It follows:
+- shape functions and Jacobian (src/fem/quadrilateral_4n_element_functions.rs:
+  252-338, 380-503, 505-653): node order (r, s) = 1 (+, +), 2 (-, +), 3 (-, -),
+  4 (+, -), h_i = (1 + r r_i)(1 + s s_i) / 4, J = [[x_r, y_r], [x_s, y_s]],
+  [dh/dx; dh/dy] = J^-1 [dh/dr; dh/ds];
+- B matrices (src/fem/structs/plate.rs:160-274 membrane, 276-390 bending,
+  392-511 MITC4 shear, with gamma multipliers sqrt(...) / (8 det J));
+- constitutive matrices and the sum over points (plate.rs:513-672: membrane
+  E t / (1 - nu^2), bending E t^3 / (12 (1 - nu^2)), both times
+  [[1, nu, 0], [nu, 1, 0], [0, 0, (1 - nu) / 2]]; shear E t k / (2 (1 + nu))
+  times the weight again, then every part times det J and the weight);
+- the 2 x 2 rule with the crate's point coordinate: sqrt of the f32 value of
+  1/3 taken in f64 (plate.rs:1066-1091), weights 1; KROT6 = 1 added to each
+  node's drilling diagonal (plate.rs:25, 714-721).
+
+The quads lie in z = 0 with node 3 at the origin and (X4 - X3) x (X2 - X3)
+along +z, so the element frame is the identity and the local matrix is also the
+element's global one.  Inputs are dyadic (exact in binary); the FP64 inputs E,
+nu, t, k are taken at their exact binary values.  Each nonzero entry is stored
+to 30 significant digits.
+
+Run from the repository root: python tests/golden/make_plate_golden.py
+"""
+
+import json
+import os
+
+import mpmath as mp
+import numpy as np
+
+mp.mp.dps = 50
+
+CASES = [
+    # (x1, y1, x2, y2, x4, y4), E, nu, t, k  -- x3 = y3 = 0
+    ((1.0, 1.0, 0.0, 1.0, 1.0, 0.0), 2.0e11, 0.3, 0.01, 5.0 / 6.0),  # unit square
+]
+
+
+def _random_cases(seed, n):
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        sx, sy = rng.uniform(0.5, 3.0, 2)
+        j = rng.uniform(-0.25, 0.25, 6)
+        xl = np.array([sx * (1 + j[0]), sy * (1 + j[1]), sx * j[2], sy * (1 + j[3]), sx * (1 + j[4]), sy * j[5]])
+        xl = np.round(xl * 1024.0) / 1024.0  # dyadic
+        x1, y1, x2, y2, x4, y4 = xl
+        # convex and counterclockwise 1 -> 2 -> 3 -> 4 (positive cross products at every corner)
+        p = [(x1, y1), (x2, y2), (0.0, 0.0), (x4, y4)]
+        cross = [(p[(i + 1) % 4][0] - p[i][0]) * (p[(i + 2) % 4][1] - p[(i + 1) % 4][1])
+                 - (p[(i + 1) % 4][1] - p[i][1]) * (p[(i + 2) % 4][0] - p[(i + 1) % 4][0]) for i in range(4)]
+        if min(cross) <= 0.05 * sx * sy:
+            continue
+        E = float(rng.choice([70e9, 2.0e11, 2.1e11]))
+        nu = float(rng.choice([0.25, 0.3, 0.33]))
+        t = float(np.round(rng.uniform(0.002, 0.2), 4))
+        out.append((tuple(float(v) for v in xl), E, nu, t, 5.0 / 6.0))
+    return out
+
+
+def gauss_point():
+    """The crate's 2 x 2 point coordinate: f32(1/3) widened, then sqrt in f64."""
+    return mp.mpf(float(np.sqrt(np.float64(np.float32(1.0 / 3.0)))))
+
+
+def plate_k(xl, E, nu, t, kf):
+    x1, y1, x2, y2, x4, y4 = (mp.mpf(v) for v in xl)
+    x3 = y3 = mp.mpf(0)
+    X, Y = [x1, x2, x3, x4], [y1, y2, y3, y4]
+    E, nu, t, kf = mp.mpf(E), mp.mpf(nu), mp.mpf(t), mp.mpf(kf)
+    g = gauss_point()
+    one, two, four = mp.mpf(1), mp.mpf(2), mp.mpf(4)
+    cmem = E * t / (one - nu ** 2)
+    cbend = E * t ** 3 / (12 * (one - nu ** 2))
+    cshear = E * t * kf / (two * (one + nu))
+    C = mp.matrix([[1, nu, 0], [nu, 1, 0], [0, 0, (one - nu) / two]])
+    ax, bx, cx = x1 - x2 - x3 + x4, x1 - x2 + x3 - x4, x1 + x2 - x3 - x4
+    ay, by, cy = y1 - y2 - y3 + y4, y1 - y2 + y3 - y4, y1 + y2 - y3 - y4
+    K = mp.matrix(24, 24)
+    for r, s in ((g, g), (-g, g), (-g, -g), (g, -g)):
+        alpha = one  # alpha_r * alpha_s
+        dr = [(one + s) / four, -(one + s) / four, -(one - s) / four, (one - s) / four]
+        ds = [(one + r) / four, (one - r) / four, -(one - r) / four, -(one + r) / four]
+        xr = sum(dr[i] * X[i] for i in range(4))
+        yr = sum(dr[i] * Y[i] for i in range(4))
+        xs = sum(ds[i] * X[i] for i in range(4))
+        ys = sum(ds[i] * Y[i] for i in range(4))
+        det = xr * ys - yr * xs
+        dx = [(ys * dr[i] - yr * ds[i]) / det for i in range(4)]
+        dy = [(-xs * dr[i] + xr * ds[i]) / det for i in range(4)]
+        bm = mp.matrix(3, 24)
+        bb = mp.matrix(3, 24)
+        for i in range(4):
+            bm[0, 6 * i] = dx[i]
+            bm[1, 6 * i + 1] = dy[i]
+            bm[2, 6 * i] = dy[i]
+            bm[2, 6 * i + 1] = dx[i]
+            bb[0, 6 * i + 4] = -dx[i]
+            bb[1, 6 * i + 3] = dy[i]
+            bb[2, 6 * i + 3] = dx[i]
+            bb[2, 6 * i + 4] = -dy[i]
+        mr = mp.sqrt((cx + r * bx) ** 2 + (cy + r * by) ** 2) / (8 * det)
+        ms = mp.sqrt((ax + s * bx) ** 2 + (ay + s * by) ** 2) / (8 * det)
+        bs = mp.matrix(2, 24)
+        # gamma_rz: edge 1-2 carries (1 + s), edge 4-3 carries (1 - s)
+        for i, sign, f, ex, ey in ((0, 1, one + s, x1 - x2, y1 - y2), (1, -1, one + s, x1 - x2, y1 - y2),
+                                   (2, -1, one - s, x4 - x3, y4 - y3), (3, 1, one - s, x4 - x3, y4 - y3)):
+            bs[0, 6 * i + 2] = sign * f / two * mr
+            bs[0, 6 * i + 3] = -f * ey / four * mr
+            bs[0, 6 * i + 4] = f * ex / four * mr
+        # gamma_sz: edge 1-4 carries (1 + r), edge 2-3 carries (1 - r)
+        for i, sign, f, ex, ey in ((0, 1, one + r, x1 - x4, y1 - y4), (1, 1, one - r, x2 - x3, y2 - y3),
+                                   (2, -1, one - r, x2 - x3, y2 - y3), (3, -1, one + r, x1 - x4, y1 - y4)):
+            bs[1, 6 * i + 2] = sign * f / two * ms
+            bs[1, 6 * i + 3] = -f * ey / four * ms
+            bs[1, 6 * i + 4] = f * ex / four * ms
+        K += (bm.T * (C * cmem) * bm) * (det * alpha)
+        K += (bb.T * (C * cbend) * bb) * (det * alpha)
+        K += (bs.T * (cshear * alpha) * bs) * (det * alpha)
+    for i in range(4):
+        K[6 * i + 5, 6 * i + 5] += 1
+    return K
+
+
+def main():
+    cases = CASES + _random_cases(20251203, 3)
+    out = {"description": "MITC4 plate local stiffness, 50-digit evaluation (tests/golden/make_plate_golden.py)",
+           "cases": []}
+    for xl, E, nu, t, kf in cases:
+        K = plate_k(xl, E, nu, t, kf)
+        entries = [[i, j, mp.nstr(K[i, j], 30)] for i in range(24) for j in range(24) if K[i, j] != 0]
+        x1, y1, x2, y2, x4, y4 = xl
+        out["cases"].append({"xyz": [[x1, y1, 0.0], [x2, y2, 0.0], [0.0, 0.0, 0.0], [x4, y4, 0.0]],
+                             "props": [E, nu, t, kf], "entries": entries})
+        print("case", xl, E, nu, t, "nonzeros", len(entries))
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plate_mitc4_golden.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=0)
+    print("wrote", path)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,114 @@
+"""The FP64 oracle's plate stiffness against the 50-digit MITC4 fixtures
+(tests/golden/plate_mitc4_golden.json, written by make_plate_golden.py from
+src/fem/structs/plate.rs:160-724 and quadrilateral_4n_element_functions.rs).
+
+Each case is one flat quad with the identity frame, so the oracle's CSR of the
+one-element model is the local matrix.  The fixtures are checked first for
+what the mathematics fixes regardless of the restatement (symmetry, the three
+rigid translations, nullity 6 with the drilling term, positive
+semi-definiteness); then the oracle must meet the parity bound of
+tests/parity.py entry by entry with an identical sparsity pattern.  The device
+path is held to the same fixtures in test_gpu_golden_plate.py."""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleModel
+from tests.parity import compare
+
+FIXTURE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plate_mitc4_golden.json")
+
+
+def cases():
+    with open(FIXTURE) as f:
+        return json.load(f)["cases"]
+
+
+def golden_dense(case):
+    K = np.zeros((24, 24))
+    for i, j, v in case["entries"]:
+        K[i, j] = float(v)
+    return K
+
+
+def one_plate_oracle(case):
+    om = OracleModel(1e-4, 1e-12, 4)
+    for n, p in enumerate(case["xyz"]):
+        om.add_node(n + 1, *p)
+    om.add_plate(1, 1, 2, 3, 4, *case["props"])
+    return om
+
+
+def golden_reference(case, om):
+    """(row_ptr, col_idx, val, magnitude) of the fixture on the oracle's
+    structural pattern, after checking that the two patterns agree."""
+    rp, ci, _ = om.csr(structural=True)
+    rows = np.repeat(np.arange(24), np.diff(rp))
+    K = golden_dense(case)
+    assert set(zip(rows.tolist(), ci.tolist())) == {(i, j) for i, j, _ in case["entries"]}
+    return rp, ci, K[rows, ci], om.csr_magnitude()
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_fixture_invariants(idx):
+    case = cases()[idx]
+    K = golden_dense(case)
+    scale = np.abs(K).max()
+    assert np.abs(K - K.T).max() <= 1e-15 * scale
+    xyz = np.array(case["xyz"])
+    for d in range(3):  # rigid translations
+        u = np.zeros(24)
+        u[d::6] = 1.0
+        assert np.abs(K @ u).max() <= 1e-12 * scale
+    # rigid rotations (right-hand rule, small angle phi = 1): about z, u = -y,
+    # v = x with the drilling rotation left at 0 (KROT6 resists it); about x,
+    # w = y with theta_x = 1; about y, w = -x with theta_y = 1
+    rz = np.zeros(24)
+    rz[0::6], rz[1::6] = -xyz[:, 1], xyz[:, 0]
+    assert np.abs(K @ rz).max() <= 1e-12 * scale
+    rx = np.zeros(24)
+    rx[2::6], rx[3::6] = xyz[:, 1], 1.0
+    ry = np.zeros(24)
+    ry[2::6], ry[4::6] = -xyz[:, 0], 1.0
+    assert np.abs(K @ rx).max() <= 1e-12 * scale and np.abs(K @ ry).max() <= 1e-12 * scale
+    # the drilling dofs are decoupled with stiffness KROT6 = 1: four eigenvalues
+    # exactly 1; of the other 20, six vanish (to rounding: ~1e-16 of the
+    # largest, 1e-5 here) and the rest are positive
+    drill = [6 * a + 5 for a in range(4)]
+    assert np.array_equal(K[drill][:, drill], np.eye(4))
+    keep = [i for i in range(24) if i not in drill]
+    assert not K[drill][:, keep].any()
+    w = np.linalg.eigvalsh(K[np.ix_(keep, keep)])
+    tiny = 1e-13 * np.abs(w).max()
+    assert np.sum(np.abs(w) < tiny) == 6
+    assert np.all(w > -tiny)
+    assert len(case["entries"]) == 212
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_oracle_matches_golden(idx):
+    case = cases()[idx]
+    om = one_plate_oracle(case)
+    ref = golden_reference(case, om)
+    r = compare(om.csr(structural=True), ref)
+    print("case", idx, r)
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r
+    assert r["normwise"] < 1e-14, r
+
+
+def test_fixture_keeps_the_f32_point():
+    """The fixtures use the crate's f32 Gauss point; with the exact 1/sqrt(3)
+    the unit-square matrix moves by ~1e-8 relative, far above the parity
+    bound, so a restatement with exact points would fail the tests above."""
+    case = cases()[0]
+    g_crate = float(np.sqrt(np.float64(np.float32(1.0 / 3.0))))
+    rel = abs(g_crate - 1.0 / np.sqrt(3.0)) * np.sqrt(3.0)
+    assert 1e-9 < rel < 1e-7
+    K = golden_dense(case)
+    om = one_plate_oracle(case)
+    rp, ci, v = om.csr(structural=True)
+    rows = np.repeat(np.arange(24), np.diff(rp))
+    assert np.abs(v - K[rows, ci]).max() < 1e-3 * rel * np.abs(K).max()

@@ -1,0 +1,82 @@
+"""The device assembly of single MITC4 plates against the 50-digit fixtures
+(tests/golden/plate_mitc4_golden.json): the CSR pattern bit for bit and every
+value within the parity bound of tests/parity.py (term magnitudes from the
+oracle).  Each case also runs as a 2 x 2 patch of copies of the quad shifted
+by its edge vectors from node 3 (sharing the nodes that coincide), whose rows
+are sums of the fixture's blocks in element order."""
+
+import numpy as np
+import pytest
+
+from finite_element_method_amd import FEM
+from finite_element_method_amd.meshgen import Mesh
+from oracle.oracle import OracleModel
+from tests.parity import compare
+from tests.test_golden_plate import cases, golden_dense, golden_reference, one_plate_oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def one_plate_mesh(case):
+    return Mesh("plate", np.arange(1, 5, dtype=np.uint32), np.array(case["xyz"], float),
+                np.array([1], dtype=np.uint32), np.array([[1, 2, 3, 4]], dtype=np.uint32),
+                np.array([case["props"]], float))
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_device_matches_golden(gpu, idx):
+    case = cases()[idx]
+    ref = golden_reference(case, one_plate_oracle(case))
+    m = FEM.from_mesh(one_plate_mesh(case))
+    r = compare(m.stiffness_csr(), ref)
+    print("golden case", idx, m.assembly_path(), r, flush=True)
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r
+    assert r["normwise"] < 1e-14, r
+
+
+def _patch(case):
+    """Four copies of the quad on a parallelogram lattice spanned by its two
+    edge vectors from node 3 (X4 - X3 and X2 - X3).  For a parallelogram the
+    copies tile the plane; for a general quad they only share the corner
+    nodes listed here (the element matrices stay those of the fixture,
+    translated)."""
+    x = np.array(case["xyz"], float)
+    a, b = x[3] - x[2], x[1] - x[2]
+    nodes, conn = {}, []
+
+    def node(p):
+        key = tuple(np.round(p, 12))
+        if key not in nodes:
+            nodes[key] = (len(nodes) + 1, p)
+        return nodes[key][0]
+
+    for i in range(2):
+        for j in range(2):
+            off = i * a + j * b
+            conn.append([node(x[k] + off) for k in range(4)])
+    xyz = np.array([v[1] for v in sorted(nodes.values())])
+    n = len(xyz)
+    return Mesh("plate", np.arange(1, n + 1, dtype=np.uint32), xyz, np.arange(1, 5, dtype=np.uint32),
+                np.array(conn, dtype=np.uint32), np.array([case["props"]] * 4, float)), conn
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_device_patch_sums_golden_blocks(gpu, idx):
+    case = cases()[idx]
+    mesh, conn = _patch(case)
+    Kg = golden_dense(case)
+    n = len(mesh.xyz)
+    K = np.zeros((6 * n, 6 * n))
+    for c in conn:  # element insertion order
+        dofs = (6 * (np.array(c) - 1)[:, None] + np.arange(6)).reshape(-1)
+        K[np.ix_(dofs, dofs)] += Kg
+    om = OracleModel(1e-4, 1e-12, n)
+    om.add_nodes(mesh.node_numbers, mesh.xyz)
+    om.add_plates(mesh.numbers, mesh.conn, mesh.props)
+    rp, ci, _ = om.csr(structural=True)
+    rows = np.repeat(np.arange(6 * n), np.diff(rp))
+    ref = (rp, ci, K[rows, ci], om.csr_magnitude())
+    m = FEM.from_mesh(mesh)
+    r = compare(m.stiffness_csr(), ref)
+    print("golden patch", idx, m.assembly_path(), r, flush=True)
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r
