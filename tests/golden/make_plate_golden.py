@@ -23,6 +23,13 @@ element's global one.  Inputs are dyadic (exact in binary); the FP64 inputs E,
 nu, t, k are taken at their exact binary values.  Each nonzero entry is stored
 to 30 significant digits.
 
+Gradients, for the backward: with G random on the 212 nonzeros and
+L = sum G_ij K_ij, and with L_u = -u^T K u for a random u (the compliance
+sensitivities), dL/dp for p = E, nu, t, k and for the in-plane coordinates
+x, y of the four nodes (the frame stays the identity under in-plane moves),
+by central differences at 50 digits with steps 1e-20 relative (truncation
+~1e-40, cancellation ~1e-30): 30 significant digits.
+
 Run from the repository root: python tests/golden/make_plate_golden.py
 """
 
@@ -67,8 +74,11 @@ def gauss_point():
     return mp.mpf(float(np.sqrt(np.float64(np.float32(1.0 / 3.0)))))
 
 
-def plate_k(xl, E, nu, t, kf):
-    x1, y1, x2, y2, x4, y4 = (mp.mpf(v) for v in xl)
+def plate_k(xl, E, nu, t, kf, x3=0, y3=0):
+    """Local stiffness; xl = (x1, y1, x2, y2, x4, y4) in the plane, node 3 at
+    (x3, y3) (local coordinates are taken relative to node 3)."""
+    x3, y3 = mp.mpf(x3), mp.mpf(y3)
+    x1, y1, x2, y2, x4, y4 = (mp.mpf(v) - (x3 if i % 2 == 0 else y3) for i, v in enumerate(xl))
     x3 = y3 = mp.mpf(0)
     X, Y = [x1, x2, x3, x4], [y1, y2, y3, y4]
     E, nu, t, kf = mp.mpf(E), mp.mpf(nu), mp.mpf(t), mp.mpf(kf)
@@ -126,16 +136,60 @@ def plate_k(xl, E, nu, t, kf):
     return K
 
 
+def gradients(xl, props, weights):
+    """dL/d(E, nu, t, k) and dL/d(x_a, y_a) for a = 1..4 of L = sum w_ij K_ij,
+    by central differences at 50 digits."""
+    def L(xl_, props_, x3=0, y3=0):
+        K = plate_k(xl_, *props_, x3=x3, y3=y3)
+        return sum(w * K[i, j] for (i, j), w in weights.items())
+    dp = []
+    for q in range(4):
+        h = mp.mpf(props[q]) * mp.mpf("1e-20")
+        up = [mp.mpf(v) for v in props]
+        dn = [mp.mpf(v) for v in props]
+        up[q] += h
+        dn[q] -= h
+        dp.append((L(xl, up) - L(xl, dn)) / (2 * h))
+    scale = max(abs(mp.mpf(v)) for v in xl)
+    h = scale * mp.mpf("1e-20")
+    dx = []
+    for a in range(4):  # node a + 1; node 3 (a = 2) moves through x3, y3
+        row = []
+        for c in range(2):
+            if a == 2:
+                args = [(xl, props, h if c == 0 else 0, h if c == 1 else 0),
+                        (xl, props, -h if c == 0 else 0, -h if c == 1 else 0)]
+            else:
+                k = {0: 0, 1: 2, 3: 4}[a] + c
+                xu = [mp.mpf(v) for v in xl]
+                xd = [mp.mpf(v) for v in xl]
+                xu[k] += h
+                xd[k] -= h
+                args = [(xu, props, 0, 0), (xd, props, 0, 0)]
+            row.append((L(*args[0]) - L(*args[1])) / (2 * h))
+        dx.append(row)
+    return dp, dx
+
+
 def main():
     cases = CASES + _random_cases(20251203, 3)
+    rng = np.random.default_rng(7)
     out = {"description": "MITC4 plate local stiffness, 50-digit evaluation (tests/golden/make_plate_golden.py)",
            "cases": []}
     for xl, E, nu, t, kf in cases:
         K = plate_k(xl, E, nu, t, kf)
         entries = [[i, j, mp.nstr(K[i, j], 30)] for i in range(24) for j in range(24) if K[i, j] != 0]
         x1, y1, x2, y2, x4, y4 = xl
+        G = [float(v) for v in rng.normal(size=len(entries))]
+        u = [float(v) for v in rng.normal(size=24)]
+        props = (E, nu, t, kf)
+        gp, gx = gradients(xl, props, {(i, j): mp.mpf(g) for (i, j, _), g in zip(entries, G)})
+        up, ux = gradients(xl, props, {(i, j): -mp.mpf(u[i]) * mp.mpf(u[j]) for i, j, _ in entries})
+        s30 = lambda v: mp.nstr(v, 30)
         out["cases"].append({"xyz": [[x1, y1, 0.0], [x2, y2, 0.0], [0.0, 0.0, 0.0], [x4, y4, 0.0]],
-                             "props": [E, nu, t, kf], "entries": entries})
+                             "props": [E, nu, t, kf], "entries": entries,
+                             "G": G, "grad_G": {"params": [s30(v) for v in gp], "xy": [[s30(v) for v in r] for r in gx]},
+                             "u": u, "grad_u": {"params": [s30(v) for v in up], "xy": [[s30(v) for v in r] for r in ux]}})
         print("case", xl, E, nu, t, "nonzeros", len(entries))
     path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plate_mitc4_golden.json")
     with open(path, "w") as f:

@@ -112,3 +112,39 @@ def test_fixture_keeps_the_f32_point():
     rp, ci, v = om.csr(structural=True)
     rows = np.repeat(np.arange(24), np.diff(rp))
     assert np.abs(v - K[rows, ci]).max() < 1e-3 * rel * np.abs(K).max()
+
+
+def weights(case, key):
+    """w_ij on the entries: G, or -u_i u_j for the compliance."""
+    if key == "G":
+        return np.array(case["G"])
+    u = np.array(case["u"])
+    return np.array([-u[i] * u[j] for i, j, _ in case["entries"]])
+
+
+def grad_scale(case, key):
+    """(sum |w_ij K_ij|, L = sum w_ij K_ij): the first is the natural size of
+    L, hence of p dL/dp."""
+    K = golden_dense(case)
+    w = weights(case, key)
+    vals = np.array([K[i, j] for i, j, _ in case["entries"]])
+    return float(np.abs(w * vals).sum()), float(np.dot(w, vals))
+
+
+@pytest.mark.parametrize("key", ["G", "u"])
+@pytest.mark.parametrize("idx", range(4))
+def test_fixture_gradients_self_consistent(idx, key):
+    """K is E times a matrix free of E plus KROT6 = 1 on the drilling
+    diagonals, so E dL/dE = L - sum of w over those four entries; K depends on
+    positions only through differences, so the in-plane coordinate gradients
+    sum to zero."""
+    case = cases()[idx]
+    g = case["grad_" + key]
+    lab, L = grad_scale(case, key)
+    w = weights(case, key)
+    drill = sum(wk for wk, (i, j, _) in zip(w, case["entries"]) if i == j and i % 6 == 5)
+    E = case["props"][0]
+    assert abs(E * float(g["params"][0]) - (L - drill)) <= 1e-14 * lab
+    xy = np.array([[float(v) for v in r] for r in g["xy"]])
+    size = max(abs(v) for p in case["xyz"] for v in p)
+    assert np.abs(xy.sum(axis=0)).max() <= 1e-14 * lab / size
