@@ -30,6 +30,11 @@ x, y of the four nodes (the frame stays the identity under in-plane moves),
 by central differences at 50 digits with steps 1e-20 relative (truncation
 ~1e-40, cancellation ~1e-30): 30 significant digits.
 
+plate_tilted_golden.json holds the same quads rotated and translated into
+general position (FP64 coordinates, taken exactly): the global matrix (all 576
+entries) and the gradients with respect to E, nu, t, k and all twelve node
+coordinates.
+
 Run from the repository root: python tests/golden/make_plate_golden.py
 """
 
@@ -197,5 +202,112 @@ def main():
     print("wrote", path)
 
 
+
+# ----------------------------------------------------------------- tilted plates
+# Quads in general position: the same element in global axes,
+# K = T^T k(local coordinates) T with T = diag(R, R, ...), R's rows (e1, e2, e3),
+# e3 along n = (X4 - X3) x (X2 - X3) and local coordinates the first two
+# components of R (X_i - X3) (quadrilateral_4n_element_functions.rs:131-171,
+# 340-378; plate.rs:726-1002).  The crate fixes e1 by its alignment rotation;
+# K is the same for every e1 in the plane (the element is isotropic in its
+# plane: an in-plane rotation of the frame rotates k and T together), so here
+# e1 = (X4 - X3) projected on the plane, normalised.
+
+
+def frame(X):
+    X = [[mp.mpf(v) for v in p] for p in X]
+    a = [X[3][k] - X[2][k] for k in range(3)]
+    b = [X[1][k] - X[2][k] for k in range(3)]
+    n = [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    nn = mp.sqrt(sum(v * v for v in n))
+    e3 = [v / nn for v in n]
+    d = sum(a[k] * e3[k] for k in range(3))
+    p = [a[k] - d * e3[k] for k in range(3)]
+    pn = mp.sqrt(sum(v * v for v in p))
+    e1 = [v / pn for v in p]
+    e2 = [e3[1] * e1[2] - e3[2] * e1[1], e3[2] * e1[0] - e3[0] * e1[2], e3[0] * e1[1] - e3[1] * e1[0]]
+    return X, [e1, e2, e3]
+
+
+def plate_k_global(X, props):
+    X, R = frame(X)
+    loc = []
+    for i in (0, 1, 3):
+        d = [X[i][k] - X[2][k] for k in range(3)]
+        loc += [sum(R[0][k] * d[k] for k in range(3)), sum(R[1][k] * d[k] for k in range(3))]
+    k = plate_k(loc, *props)
+    T = mp.matrix(24, 24)
+    for blk in range(8):
+        for r in range(3):
+            for c in range(3):
+                T[3 * blk + r, 3 * blk + c] = R[r][c]
+    return T.T * k * T
+
+
+def tilted_gradients(X, props, weight_sets):
+    """dL/d(E, nu, t, k) and dL/dX (4 x 3) for each weight set, by central
+    differences at 50 digits (steps 1e-20 relative)."""
+    def Ls(X_, p_):
+        K = plate_k_global(X_, p_)
+        return [sum(w * K[i, j] for (i, j), w in ws.items()) for ws in weight_sets]
+    nset = len(weight_sets)
+    gp = [[None] * 4 for _ in range(nset)]
+    for q in range(4):
+        h = mp.mpf(props[q]) * mp.mpf("1e-20")
+        up = [mp.mpf(v) for v in props]
+        dn = list(up)
+        up[q] += h
+        dn[q] -= h
+        lu, ld = Ls(X, up), Ls(X, dn)
+        for s in range(nset):
+            gp[s][q] = (lu[s] - ld[s]) / (2 * h)
+    scale = max(abs(mp.mpf(v)) for p in X for v in p)
+    h = scale * mp.mpf("1e-20")
+    gx = [[[None] * 3 for _ in range(4)] for _ in range(nset)]
+    for a in range(4):
+        for c in range(3):
+            Xu = [[mp.mpf(v) for v in p] for p in X]
+            Xd = [list(p) for p in Xu]
+            Xu[a][c] += h
+            Xd[a][c] -= h
+            lu, ld = Ls(Xu, props), Ls(Xd, props)
+            for s in range(nset):
+                gx[s][a][c] = (lu[s] - ld[s]) / (2 * h)
+    return gp, gx
+
+
+def main_tilted():
+    rng = np.random.default_rng(29)
+    out = {"description": "MITC4 plates in general position: global stiffness and gradients, 50-digit "
+                          "evaluation (tests/golden/make_plate_golden.py)", "cases": []}
+    s30 = lambda v: mp.nstr(v, 30)
+    for xl, E, nu, t, kf in CASES + _random_cases(20251203, 3):
+        q = rng.normal(size=4)
+        q /= np.linalg.norm(q)
+        w, x, y, z = q
+        Q = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                      [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                      [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+        flat = np.array([[xl[0], xl[1], 0.0], [xl[2], xl[3], 0.0], [0.0, 0.0, 0.0], [xl[4], xl[5], 0.0]])
+        X = (flat @ Q.T + rng.uniform(-2, 2, 3)).tolist()  # the FP64 coordinates are the inputs
+        props = (E, nu, t, kf)
+        K = plate_k_global(X, props)
+        entries = [[i, j, s30(K[i, j])] for i in range(24) for j in range(24) if K[i, j] != 0]
+        G = [float(v) for v in rng.normal(size=len(entries))]
+        u = [float(v) for v in rng.normal(size=24)]
+        wG = {(i, j): mp.mpf(g) for (i, j, _), g in zip(entries, G)}
+        wu = {(i, j): -mp.mpf(u[i]) * mp.mpf(u[j]) for i, j, _ in entries}
+        gp, gx = tilted_gradients(X, props, [wG, wu])
+        pack = lambda s: {"params": [s30(v) for v in gp[s]], "xyz": [[s30(v) for v in r] for r in gx[s]]}
+        out["cases"].append({"xyz": X, "props": list(props), "entries": entries,
+                             "G": G, "grad_G": pack(0), "u": u, "grad_u": pack(1)})
+        print("tilted case", E, nu, t, "nonzeros", len(entries))
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "plate_tilted_golden.json")
+    with open(path, "w") as f:
+        json.dump(out, f, indent=0)
+    print("wrote", path)
+
+
 if __name__ == "__main__":
     main()
+    main_tilted()

@@ -148,3 +148,53 @@ def test_fixture_gradients_self_consistent(idx, key):
     xy = np.array([[float(v) for v in r] for r in g["xy"]])
     size = max(abs(v) for p in case["xyz"] for v in p)
     assert np.abs(xy.sum(axis=0)).max() <= 1e-14 * lab / size
+
+
+# ----------------------------------------------------------------- tilted plates
+TILTED = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "plate_tilted_golden.json")
+
+
+def tilted_cases():
+    with open(TILTED) as f:
+        return json.load(f)["cases"]
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_tilted_fixture_is_the_rotated_flat_one(idx):
+    """The tilted fixture (own frame, projection, T^T k T) equals the flat
+    fixture rotated into place, T_Q K T_Q^T with Q recovered from the node
+    coordinates; and its coordinate gradients sum to zero (translation)."""
+    flat, tilt = cases()[idx], tilted_cases()[idx]
+    Kf, Kt = golden_dense(flat), np.zeros((24, 24))
+    for i, j, v in tilt["entries"]:
+        Kt[i, j] = float(v)
+    Xf, Xt = np.array(flat["xyz"]), np.array(tilt["xyz"])
+    A = Xf[[0, 1, 3]] - Xf[2]
+    B = Xt[[0, 1, 3]] - Xt[2]
+    Q = np.linalg.lstsq(A, B, rcond=None)[0].T  # B = A Q^T; A has rank 2: fix the normal
+    n = np.cross(Xt[3] - Xt[2], Xt[1] - Xt[2])
+    Q[:, 2] = n / np.linalg.norm(n)
+    T = np.kron(np.eye(8), Q)
+    assert np.abs(Kt - T @ Kf @ T.T).max() <= 1e-12 * np.abs(Kf).max()
+    assert np.abs(Kt - Kt.T).max() <= 1e-15 * np.abs(Kt).max()
+    for key in ("G", "u"):
+        gx = np.array([[float(v) for v in r] for r in tilt["grad_" + key]["xyz"]])
+        assert np.abs(gx.sum(axis=0)).max() <= 1e-13 * np.abs(gx).max()
+
+
+@pytest.mark.parametrize("idx", range(4))
+def test_oracle_matches_tilted_golden(idx):
+    case = tilted_cases()[idx]
+    om = OracleModel(1e-4, 1e-12, 4)
+    for n, p in enumerate(case["xyz"]):
+        om.add_node(n + 1, *p)
+    om.add_plate(1, 1, 2, 3, 4, *case["props"])
+    rp, ci, v = om.csr(structural=True)
+    rows = np.repeat(np.arange(24), np.diff(rp))
+    K = np.zeros((24, 24))
+    for i, j, val in case["entries"]:
+        K[i, j] = float(val)
+    assert set(zip(rows.tolist(), ci.tolist())) == {(i, j) for i, j, _ in case["entries"]}
+    r = compare((rp, ci, v), (rp, ci, K[rows, ci], om.csr_magnitude()))
+    print("tilted case", idx, r)
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r

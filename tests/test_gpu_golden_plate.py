@@ -117,3 +117,47 @@ def test_device_gradients_match_golden(gpu, idx, key):
     err_x = np.abs(gx - ref_x).max() * size / lab
     print("golden gradients", idx, key, "params", err_p, "xy", err_x, flush=True)
     assert err_p.max() <= 1e-14 and err_x <= 1e-14
+
+
+@pytest.mark.parametrize("key", ["G", "u"])
+@pytest.mark.parametrize("idx", range(4))
+def test_device_tilted_plate_matches_golden(gpu, idx, key):
+    """Plates in general position (the general frame, assembly and adjoint
+    kernels): values within the parity bound and the gradients with respect to
+    E, nu, t, k and all twelve coordinates within 1e-14 of their natural size
+    (as the flat cases)."""
+    import torch
+
+    from finite_element_method_amd import PLATE
+    from tests.test_golden_plate import tilted_cases
+
+    case = tilted_cases()[idx]
+    mesh = one_plate_mesh(case)
+    om = OracleModel(1e-4, 1e-12, 4)
+    om.add_nodes(mesh.node_numbers, mesh.xyz)
+    om.add_plates(mesh.numbers, mesh.conn, mesh.props)
+    rp0, ci0, _ = om.csr(structural=True)
+    rows = np.repeat(np.arange(24), np.diff(rp0))
+    K = np.zeros((24, 24))
+    for i, j, v in case["entries"]:
+        K[i, j] = float(v)
+    m = FEM.from_mesh(mesh)
+    r = compare(m.stiffness_csr(), (rp0, ci0, K[rows, ci0], om.csr_magnitude()))
+    assert r["pattern_equal"] and r["max_ratio"] <= 1.0, r
+    assert list(zip(rows.tolist(), ci0.tolist())) == [(i, j) for i, j, _ in case["entries"]]
+    if key == "G":
+        w = np.array(case["G"])
+        dp, dx = m.backward(torch.tensor(case["G"], dtype=torch.float64, device="cuda"))
+    else:
+        u = np.array(case["u"])
+        w = np.array([-u[i] * u[j] for i, j, _ in case["entries"]])
+        dp, dx = m.compliance_sensitivity(torch.tensor(case["u"], dtype=torch.float64, device="cuda"))
+    torch.cuda.synchronize()
+    lab = float(np.abs(w * K[rows, ci0]).sum())
+    ref_p = np.array([float(v) for v in case["grad_" + key]["params"]])
+    ref_x = np.array([[float(v) for v in r] for r in case["grad_" + key]["xyz"]])
+    size = np.abs(np.array(case["xyz"])).max()
+    err_p = np.abs(dp[PLATE].cpu().numpy().reshape(4) - ref_p) * np.abs(np.array(case["props"])) / lab
+    err_x = np.abs(dx.cpu().numpy() - ref_x).max() * size / lab
+    print("golden tilted", idx, key, m.assembly_path(), r["u_of_M"], "params", err_p, "xyz", err_x, flush=True)
+    assert err_p.max() <= 1e-14 and err_x <= 1e-14
