@@ -4,11 +4,14 @@ src/fem/structs/plate.rs:160-724 and quadrilateral_4n_element_functions.rs).
 
 Each case is one flat quad with the identity frame, so the oracle's CSR of the
 one-element model is the local matrix.  The fixtures are checked first for
-what the mathematics fixes regardless of the restatement (symmetry, the three
-rigid translations, nullity 6 with the drilling term, positive
-semi-definiteness); then the oracle must meet the parity bound of
-tests/parity.py entry by entry with an identical sparsity pattern.  The device
-path is held to the same fixtures in test_gpu_golden_plate.py."""
+what the mathematics fixes regardless of the restatement (symmetry, the rigid
+translations and rotations, nullity 6 with the drilling term, positive
+semi-definiteness, the gradients' homogeneity in E and translation
+invariance); then the oracle must meet the parity bound of tests/parity.py
+entry by entry with an identical sparsity pattern.  The same quads in general
+position (plate_tilted_golden.json) must equal the flat fixtures rotated into
+place, and the oracle must meet them too.  The device path is held to both in
+test_gpu_golden_plate.py."""
 
 import json
 import os
