@@ -192,11 +192,14 @@ __global__ void sep_pack(const uint8_t* __restrict__ cls, const uint64_t* __rest
   pack[r] = k == 0 ? 0u : (k << 30) | (k == 1 ? lo32(pos[r]) : hi32(pos[r]));
 }
 
-// 4 lanes per row, each taking four neighbouring entries per step: two
+// 8 lanes per row, each taking four neighbouring entries per step: two
 // 16-byte value loads and one 16-byte column load, range-checked buffer loads
 // (entries past the row end are loaded but not used; their columns are valid
-// indices or 0), and one 4-byte gather per non-zero entry.
-constexpr int SEP_QL = 4;
+// indices or 0), and one 4-byte gather per non-zero entry.  8 lanes (32
+// entries per step) ran 9-12 % faster than 4 on plate and beam-frame rows of
+// 18-42 entries, whose step tails then idle more lanes but whose loads
+// coalesce over wider spans; 1, 2 and 16 lanes were slower.
+constexpr int SEP_QL = 8;
 typedef unsigned int sep_u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline void sep_load4(__amdgpu_buffer_rsrc_t rv, __amdgpu_buffer_rsrc_t rc, int64_t e, double w[4],
@@ -256,7 +259,7 @@ __global__ void sep_count_quad(const int64_t* __restrict__ rp, const int32_t* __
 }
 
 // Output slots in entry order: an exclusive scan of the per-lane counts over
-// the row's four lanes (lane l holds entries 4l .. 4l + 3 of the step), then
+// the row's SEP_QL lanes (lane l holds entries 4l .. 4l + 3 of the step), then
 // each lane's entries in turn.
 __global__ void sep_fill_quad(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                               const double* __restrict__ val, const uint32_t* __restrict__ pack,
