@@ -192,14 +192,15 @@ __global__ void sep_pack(const uint8_t* __restrict__ cls, const uint64_t* __rest
   pack[r] = k == 0 ? 0u : (k << 30) | (k == 1 ? lo32(pos[r]) : hi32(pos[r]));
 }
 
-// 8 lanes per row, each taking four neighbouring entries per step: two
+// QL lanes per row, each taking four neighbouring entries per step: two
 // 16-byte value loads and one 16-byte column load, range-checked buffer loads
 // (entries past the row end are loaded but not used; their columns are valid
-// indices or 0), and one 4-byte gather per non-zero entry.  8 lanes (32
-// entries per step) ran 9-12 % faster than 4 on plate and beam-frame rows of
-// 18-42 entries, whose step tails then idle more lanes but whose loads
-// coalesce over wider spans; 1, 2 and 16 lanes were slower.
-constexpr int SEP_QL = 8;
+// indices or 0), and one 4-byte gather per non-zero entry.  On plate and
+// beam-frame rows of 18-42 entries the count runs fastest with 4 lanes and
+// the fill, whose stores then cover wider spans, with 8 (1024^2 plate: count
+// 460 vs 531 us, fill 807 vs 1039 us); 1, 2 and 16 lanes were slower for both.
+constexpr int SEP_QL_COUNT = 4;
+constexpr int SEP_QL_FILL = 8;
 typedef unsigned int sep_u32x4 __attribute__((ext_vector_type(4)));
 
 __device__ inline void sep_load4(__amdgpu_buffer_rsrc_t rv, __amdgpu_buffer_rsrc_t rc, int64_t e, double w[4],
@@ -212,6 +213,7 @@ __device__ inline void sep_load4(__amdgpu_buffer_rsrc_t rv, __amdgpu_buffer_rsrc
   c[0] = cc.x; c[1] = cc.y; c[2] = cc.z; c[3] = cc.w;
 }
 
+template <int SEP_QL>
 __global__ void sep_count_quad(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                                const double* __restrict__ val, const uint32_t* __restrict__ pack,
                                const uint64_t* __restrict__ pos, const double* __restrict__ force,
@@ -261,6 +263,7 @@ __global__ void sep_count_quad(const int64_t* __restrict__ rp, const int32_t* __
 // Output slots in entry order: an exclusive scan of the per-lane counts over
 // the row's SEP_QL lanes (lane l holds entries 4l .. 4l + 3 of the step), then
 // each lane's entries in turn.
+template <int SEP_QL>
 __global__ void sep_fill_quad(const int64_t* __restrict__ rp, const int32_t* __restrict__ ci,
                               const double* __restrict__ val, const uint32_t* __restrict__ pack,
                               const uint64_t* __restrict__ pos, const int64_t* __restrict__ off, int64_t n,
@@ -431,7 +434,7 @@ std::string separate_sparse(Device* d, SepResult& res, void* stream, bool timed,
   const bool quad = lanes_opt == 0 && size_t(d->nnz) * 8 < (size_t(1) << 32);
   const uint32_t v_bytes = quad ? uint32_t(size_t(d->nnz) * 8) : 0u;
   if (quad && !(e = dgrow(d->sep_pack, d->sep_cap[19], un)).empty()) return e;
-  const unsigned GQ = unsigned((n * SEP_QL + B - 1) / B);
+  const unsigned GQC = unsigned((n * SEP_QL_COUNT + B - 1) / B), GQF = unsigned((n * SEP_QL_FILL + B - 1) / B);
   const int lanes = lanes_opt ? lanes_opt : (d->nnz / n >= 8 ? 8 : 4);
   const unsigned G = unsigned((n * lanes + B - 1) / B);
   auto count = lanes == 32 ? sep_count<32> : lanes == 16 ? sep_count<16> : lanes == 8 ? sep_count<8> : sep_count<4>;
@@ -443,7 +446,7 @@ std::string separate_sparse(Device* d, SepResult& res, void* stream, bool timed,
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d->sep_tmp, b1, d->sep_flag, d->sep_pos, int(n + 1), s));
   if (quad) {
     sep_pack<<<unsigned((n + B - 1) / B), B, 0, s>>>(d->sep_cls, d->sep_pos, n, d->sep_pack);
-    sep_count_quad<<<GQ, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_pack, d->sep_pos, d->sep_force,
+    sep_count_quad<SEP_QL_COUNT><<<GQC, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_pack, d->sep_pos, d->sep_force,
                                     d->sep_disp, n, d->sep_cnt, d->sep_aa, d->sep_bb, d->sep_ra, d->sep_ub, v_bytes);
   } else {
     count<<<G, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_cls, d->sep_pos, d->sep_force, d->sep_disp, n,
@@ -452,7 +455,7 @@ std::string separate_sparse(Device* d, SepResult& res, void* stream, bool timed,
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(d->sep_tmp, b2, d->sep_cnt, d->sep_off, int(2 * n + 1), s));
   if (quad)
-    sep_fill_quad<<<GQ, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_pack, d->sep_pos, d->sep_off, n, d->sep_ci,
+    sep_fill_quad<SEP_QL_FILL><<<GQF, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_pack, d->sep_pos, d->sep_off, n, d->sep_ci,
                                    d->sep_v, v_bytes);
   else
     fill<<<G, B, 0, s>>>(d->row_ptr, d->col_idx, d->val, d->sep_cls, d->sep_pos, d->sep_off, n, d->sep_ci, d->sep_v);
