@@ -338,8 +338,12 @@ int fem_batch_mesh_csr_to_host(fem_batch* b, size_t mesh, int64_t* row_ptr, int3
  *                        looping over the patches: 0..64 (0 = one workgroup
  *                        per patch; default 0 for plate_blocks, 2 for plate_fused)
  *   "force_generic"      1 = always use the generic value path
- *   "separate_lanes"     lanes per CSR row in fem_separate: 4, 8, 16, 32
- *                        (0 = automatic, from the mean row length)
+ *   "separate_lanes"     0 (default) = fem_separate's four-entry kernels where
+ *                        the values span under 4 GiB (count 4 and fill 8 lanes
+ *                        per CSR row), else one entry per lane with 4 or 8
+ *                        lanes per row from the mean row length; 4, 8, 16, 32 =
+ *                        the one-entry kernels with that many lanes.  The
+ *                        separated matrices are bitwise equal either way
  *   "wide_loads"         1 (default) = where the values span under 4 GiB,
  *                        the backward reads each row run of a dL/dK block, and
  *                        the PCG's SpMV four values per lane, with 16-byte
